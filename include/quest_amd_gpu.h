@@ -1,0 +1,260 @@
+/*
+ * quest_amd_gpu.h -- the C ABI of the MI355X (gfx950) kernel layer.
+ *
+ * This is the drop-in boundary for QuEST's GPU backend.  QuEST's host code
+ * reaches its GPU backend only through the gpu_* functions declared in
+ * quest/src/gpu/gpu_subroutines.hpp:24-196 (state updates and reductions)
+ * and quest/src/gpu/gpu_config.hpp:41-120 (device, memory and copies),
+ * called from quest/src/core/accelerator.cpp (e.g. :313-336, :345-365,
+ * :513-532, :729-778, :857-1007, :1104-1143) and quest/src/api/qureg.cpp.
+ * Each entry point below replaces one of those, with C linkage and plain
+ * arguments only:
+ *
+ *   - a qureg is (amps, prec, numQubits): `amps` is a device pointer to
+ *     2^numQubits interleaved {re,im} amplitudes, `prec` is QA_FP64
+ *     (complex128) or QA_FP32 (complex64), and numQubits is the number of
+ *     index bits (n for a statevector, 2N for an N-qubit density matrix,
+ *     whose amplitude (row r, column c) sits at r + c*2^N);
+ *   - qubit lists are (const int* list, int count) host arrays, controls
+ *     come with an optional list of control states (NULL = all ones);
+ *   - 1- and 2-qubit matrices travel by value as host arrays of doubles
+ *     (row-major, re/im interleaved), larger matrices as device pointers to
+ *     complex128 row-major arrays (the gpuElemsFlat of QuEST's CompMatr,
+ *     DiagMatr and SuperOp);
+ *   - every function returns QA_OK (0) or an error code; qa_lastError()
+ *     gives the message of the last failure on the calling thread;
+ *   - updates are enqueued on the library's stream and return at once;
+ *     reductions write FP64 results (also for FP32 amplitudes) into host
+ *     memory and therefore synchronise that stream.
+ *
+ * Nothing here allocates device memory per call: scratch for reductions
+ * and staging is owned by the library and created by qa_init().
+ */
+#ifndef QUEST_AMD_GPU_H
+#define QUEST_AMD_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QA_FP32 1
+#define QA_FP64 2
+
+#define QA_OK            0
+#define QA_ERR_ARG       1
+#define QA_ERR_HIP       2
+#define QA_ERR_OOM       3
+#define QA_ERR_NOT_INIT  4
+#define QA_ERR_NO_DEVICE 5
+
+/* ------------------------------------------------------------------ */
+/* device, memory, copies        (gpu_config.hpp:41-120)              */
+/* ------------------------------------------------------------------ */
+
+const char* qa_lastError(void);
+
+/* gpu_isGpuCompiled / gpu_isGpuAvailable (gpu_config.hpp:41-45) */
+int qa_isGpuAvailable(void);
+/* gpu_bindLocalGPUsToNodes (gpu_config.cpp:332-353): selects the device,
+   creates the library stream and the reduction scratch. Idempotent. */
+int qa_init(int device);
+int qa_finalize(void);
+/* gpu_getCurrentAvailableMemoryInBytes / gpu_getTotalMemoryInBytes */
+int qa_getMemInfo(size_t* freeBytes, size_t* totalBytes);
+/* number of compute units and device name, for reporting */
+int qa_getDeviceInfo(int* numCUs, char* name, int nameLen);
+/* gpu_allocArray / gpu_deallocArray (gpu_config.cpp:399-433): returns
+   QA_ERR_OOM (and *ptr = NULL) when the allocation does not fit */
+int qa_malloc(void** ptr, size_t numBytes);
+int qa_free(void* ptr);
+/* gpu_copyCpuToGpu / gpu_copyGpuToCpu / gpu_copyArray: synchronous */
+int qa_memcpyHtoD(void* dst, const void* src, size_t numBytes);
+int qa_memcpyDtoH(void* dst, const void* src, size_t numBytes);
+int qa_memcpyDtoD(void* dst, const void* src, size_t numBytes);
+/* gpu_sync (gpu_config.cpp:382-390) */
+int qa_sync(void);
+/* the hipStream_t every entry point enqueues on */
+void* qa_getStream(void);
+
+/* amplitude transfer with conversion to/from complex128 host arrays
+   (accel_statevec_getAmps_sub / setAmps_sub, accelerator.cpp:203-230) */
+int qa_getAmps(double* outComplex128, const void* amps, int prec, int64_t startInd, int64_t numAmps);
+int qa_setAmps(void* amps, int prec, int64_t startInd, const double* inComplex128, int64_t numAmps);
+/* strided 2D block of a density matrix: rows [r0, r0+nr) x cols [c0, c0+nc),
+   written row-major into out (numRows x numCols) */
+int qa_getDensityAmps(double* outComplex128, const void* amps, int prec, int numQubits,
+                      int64_t startRow, int64_t startCol, int64_t numRows, int64_t numCols);
+/* copy between quregs of possibly different precision */
+int qa_copyAmps(void* dst, int dstPrec, const void* src, int srcPrec, int64_t numAmps);
+
+/* event timing on the library stream (for benchmarks) */
+int qa_timerStart(void);
+int qa_timerStop(double* elapsedMs);
+int qa_eventCreate(void** event);
+int qa_eventRecord(void* event);
+/* waits for `stop`, then gives the milliseconds between the two records */
+int qa_eventElapsedMs(void* start, void* stop, double* elapsedMs);
+int qa_eventDestroy(void* event);
+
+/* ------------------------------------------------------------------ */
+/* statevector gates             (gpu_subroutines.hpp:33-70)          */
+/* ------------------------------------------------------------------ */
+
+/* gpu_statevec_anyCtrlSwap_subA */
+int qa_statevec_anyCtrlSwap(void* amps, int prec, int numQubits,
+                            const int* ctrls, const int* ctrlStates, int numCtrls,
+                            int targ1, int targ2);
+
+/* gpu_statevec_anyCtrlOneTargDenseMatr_subA; matr = 8 doubles */
+int qa_statevec_anyCtrlOneTargDenseMatr(void* amps, int prec, int numQubits,
+                                        const int* ctrls, const int* ctrlStates, int numCtrls,
+                                        int targ, const double* matr);
+
+/* gpu_statevec_anyCtrlTwoTargDenseMatr_sub; matr = 32 doubles; matrix
+   index bit 0 <-> targ1, bit 1 <-> targ2 */
+int qa_statevec_anyCtrlTwoTargDenseMatr(void* amps, int prec, int numQubits,
+                                        const int* ctrls, const int* ctrlStates, int numCtrls,
+                                        int targ1, int targ2, const double* matr);
+
+/* gpu_statevec_anyCtrlAnyTargDenseMatr_sub; devMatr = device complex128
+   2^k x 2^k row-major; matrix index bit i <-> targs[i] (unsorted) */
+int qa_statevec_anyCtrlAnyTargDenseMatr(void* amps, int prec, int numQubits,
+                                        const int* ctrls, const int* ctrlStates, int numCtrls,
+                                        const int* targs, int numTargs,
+                                        const void* devMatr, int applyConj);
+
+/* gpu_statevec_anyCtrlOneTargDiagMatr_sub; elems = 4 doubles */
+int qa_statevec_anyCtrlOneTargDiagMatr(void* amps, int prec, int numQubits,
+                                       const int* ctrls, const int* ctrlStates, int numCtrls,
+                                       int targ, const double* elems);
+
+/* gpu_statevec_anyCtrlTwoTargDiagMatr_sub; elems = 8 doubles */
+int qa_statevec_anyCtrlTwoTargDiagMatr(void* amps, int prec, int numQubits,
+                                       const int* ctrls, const int* ctrlStates, int numCtrls,
+                                       int targ1, int targ2, const double* elems);
+
+/* gpu_statevec_anyCtrlAnyTargDiagMatr_sub; devElems = device complex128[2^k];
+   element index bit i <-> targs[i]; optional complex power and conjugation */
+int qa_statevec_anyCtrlAnyTargDiagMatr(void* amps, int prec, int numQubits,
+                                       const int* ctrls, const int* ctrlStates, int numCtrls,
+                                       const int* targs, int numTargs,
+                                       const void* devElems, int applyConj,
+                                       int hasPower, double expRe, double expIm);
+
+/* gpu_statevec_allTargDiagMatr_sub / gpu_densmatr_allTargDiagMatr_sub */
+int qa_statevec_allTargDiagMatr(void* amps, int prec, int numQubits, const void* devElems,
+                                int hasPower, double expRe, double expIm);
+int qa_densmatr_allTargDiagMatr(void* amps, int prec, int numQubitsN, const void* devElems,
+                                int hasPower, double expRe, double expIm, int multiplyOnly);
+
+/* gpu_statevector_anyCtrlPauliTensorOrGadget_subA: for every pair
+   (iA, iB = iA ^ maskXY) a_A <- ampFac a_A + pairFac i^#Y (+-) a_B with the
+   sign of (pair index & maskYZ); requires maskX|maskY != 0 */
+int qa_statevec_anyCtrlPauliTensorOrGadget(void* amps, int prec, int numQubits,
+                                           const int* ctrls, const int* ctrlStates, int numCtrls,
+                                           uint64_t maskX, uint64_t maskY, uint64_t maskZ,
+                                           double ampFacRe, double ampFacIm,
+                                           double pairFacRe, double pairFacIm);
+
+/* gpu_statevector_anyCtrlAnyTargZOrPhaseGadget_sub:
+   a_i <- a_i * (parity(i & targMask) ? fac1 : fac0) */
+int qa_statevec_anyCtrlAnyTargZOrPhaseGadget(void* amps, int prec, int numQubits,
+                                             const int* ctrls, const int* ctrlStates, int numCtrls,
+                                             uint64_t targMask,
+                                             double fac0Re, double fac0Im,
+                                             double fac1Re, double fac1Im);
+
+/* gpu_statevec_setQuregToSuperposition_sub: out <- fOut out + f1 in1 + f2 in2 */
+int qa_statevec_setQuregToSuperposition(int prec, int numQubits,
+                                        double facOutRe, double facOutIm, void* out,
+                                        double fac1Re, double fac1Im, const void* in1,
+                                        double fac2Re, double fac2Im, const void* in2);
+
+/* ------------------------------------------------------------------ */
+/* density-matrix channels       (gpu_subroutines.hpp:75-112)         */
+/* numQubitsN is N (the matrix is 2^N x 2^N, 2N index bits)           */
+/* ------------------------------------------------------------------ */
+
+int qa_densmatr_mixQureg(double outProb, void* outDM, double inProb, const void* inDM,
+                         int prec, int numQubitsN);
+int qa_densmatr_mixQuregWithStatevec(double outProb, void* outDM, double inProb, const void* inSV,
+                                     int prec, int numQubitsN);
+int qa_densmatr_oneQubitDephasing(void* amps, int prec, int numQubitsN, int qubit, double prob);
+int qa_densmatr_twoQubitDephasing(void* amps, int prec, int numQubitsN, int qubit1, int qubit2, double prob);
+int qa_densmatr_oneQubitDepolarising(void* amps, int prec, int numQubitsN, int qubit, double prob);
+int qa_densmatr_twoQubitDepolarising(void* amps, int prec, int numQubitsN, int qubit1, int qubit2, double prob);
+int qa_densmatr_oneQubitPauliChannel(void* amps, int prec, int numQubitsN, int qubit,
+                                     double pI, double pX, double pY, double pZ);
+int qa_densmatr_oneQubitDamping(void* amps, int prec, int numQubitsN, int qubit, double prob);
+/* gpu_densmatr_partialTrace_sub: out (numQubitsN - numTargs qubits) */
+int qa_densmatr_partialTrace(const void* inAmps, int prec, int inNumQubitsN, void* outAmps,
+                             const int* targs, int numTargs);
+
+/* ------------------------------------------------------------------ */
+/* reductions (FP64 results)     (gpu_subroutines.hpp:136-183)        */
+/* ------------------------------------------------------------------ */
+
+int qa_statevec_calcTotalProb(const void* amps, int prec, int numQubits, double* out);
+int qa_densmatr_calcTotalProb(const void* amps, int prec, int numQubitsN, double* out);
+int qa_statevec_calcProbOfMultiQubitOutcome(const void* amps, int prec, int numQubits,
+                                            const int* qubits, const int* outcomes, int num,
+                                            double* out);
+int qa_densmatr_calcProbOfMultiQubitOutcome(const void* amps, int prec, int numQubitsN,
+                                            const int* qubits, const int* outcomes, int num,
+                                            double* out);
+int qa_statevec_calcProbsOfAllMultiQubitOutcomes(double* outProbs, const void* amps, int prec,
+                                                 int numQubits, const int* qubits, int num);
+int qa_densmatr_calcProbsOfAllMultiQubitOutcomes(double* outProbs, const void* amps, int prec,
+                                                 int numQubitsN, const int* qubits, int num);
+/* out[2] = sum conj(a_i) b_i */
+int qa_statevec_calcInnerProduct(const void* ampsA, const void* ampsB, int prec, int numQubits,
+                                 double* out);
+int qa_densmatr_calcHilbertSchmidtDistance(const void* ampsA, const void* ampsB, int prec,
+                                           int numQubitsN, double* out);
+/* out[2] = sum_rc rho_rc conj(psi_r) psi_c (conj=0) or its conjugate form (conj=1) */
+int qa_densmatr_calcFidelityWithPureState(const void* rho, const void* psi, int prec,
+                                          int numQubitsN, int conj, double* out);
+int qa_statevec_calcExpecAnyTargZ(const void* amps, int prec, int numQubits, uint64_t targMask,
+                                  double* out);
+/* out[2] */
+int qa_densmatr_calcExpecAnyTargZ(const void* amps, int prec, int numQubitsN, uint64_t targMask,
+                                  double* out);
+/* out[2], includes the i^#Y factor */
+int qa_statevec_calcExpecPauliStr(const void* amps, int prec, int numQubits,
+                                  uint64_t maskX, uint64_t maskY, uint64_t maskZ, double* out);
+int qa_densmatr_calcExpecPauliStr(const void* amps, int prec, int numQubitsN,
+                                  uint64_t maskX, uint64_t maskY, uint64_t maskZ, double* out);
+/* out[2]; powMode 0 = none, 1 = complex power, 2 = real power of the real part */
+int qa_statevec_calcExpecFullStateDiagMatr(const void* amps, int prec, int numQubits,
+                                           const void* devElems, int powMode,
+                                           double expRe, double expIm, double* out);
+int qa_densmatr_calcExpecFullStateDiagMatr(const void* amps, int prec, int numQubitsN,
+                                           const void* devElems, int powMode,
+                                           double expRe, double expIm, double* out);
+
+/* ------------------------------------------------------------------ */
+/* projectors and initialisation (gpu_subroutines.hpp:185-196)        */
+/* ------------------------------------------------------------------ */
+
+int qa_statevec_multiQubitProjector(void* amps, int prec, int numQubits,
+                                    const int* qubits, const int* outcomes, int num, double prob);
+int qa_densmatr_multiQubitProjector(void* amps, int prec, int numQubitsN,
+                                    const int* qubits, const int* outcomes, int num, double prob);
+int qa_statevec_initUniformState(void* amps, int prec, int numQubits, double re, double im);
+int qa_statevec_initDebugState(void* amps, int prec, int numQubits);
+/* unnormalised Haar-distributed amplitudes from a counter-based generator
+   keyed by (seed, amplitude index): reproducible for a given seed,
+   independent of launch shape */
+int qa_statevec_initUnnormalisedUniformlyRandomPureStateAmps(void* amps, int prec, int numQubits,
+                                                             uint64_t seed);
+/* rho <- |psi><psi| */
+int qa_densmatr_initPureState(void* rho, const void* psi, int prec, int numQubitsN);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QUEST_AMD_GPU_H */

@@ -1,0 +1,427 @@
+"""ctypes mirror of include/quest.h (the QuEST v4 API subset) and of the
+kernel-layer C ABI include/quest_amd_gpu.h.
+
+Structs are declared with the reference layouts (quest/include/qureg.h:49-80,
+matrices.h:68-186, paulis.h:53-80, channels.h:74-118); functions keep the
+reference names and argument order.  A QuEST input error raises
+QuESTError(func, msg) instead of exiting, through setInputErrorHandler
+(quest/include/debug.h), the hook the reference's own tests use to turn
+validation errors into exceptions (tests/main.cpp:63-66).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import threading
+
+import numpy as np
+
+from . import GPU_LIB, HOST_LIB
+
+if not HOST_LIB.exists() or not GPU_LIB.exists():
+    raise ImportError(
+        f"quest_amd native libraries are not built ({HOST_LIB.name}, {GPU_LIB.name}); "
+        "run `python -m quest_amd.build`")
+
+_gpu = C.CDLL(str(GPU_LIB), mode=C.RTLD_GLOBAL)
+_lib = C.CDLL(str(HOST_LIB), mode=C.RTLD_GLOBAL)
+
+qreal = C.c_double
+qindex = C.c_longlong
+cint = C.c_int
+cdbl = C.c_double
+ptr = C.c_void_p
+IntP = C.POINTER(C.c_int)
+u64 = C.c_uint64
+
+
+class qcomp(C.Structure):
+    """std::complex<double> / double _Complex: two doubles."""
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
+    @classmethod
+    def of(cls, z):
+        z = complex(z)
+        return cls(z.real, z.imag)
+
+    def value(self) -> complex:
+        return complex(self.re, self.im)
+
+
+class QuESTEnv(C.Structure):
+    _fields_ = [("isMultithreaded", cint), ("isGpuAccelerated", cint), ("isDistributed", cint),
+                ("rank", cint), ("numNodes", cint)]
+
+
+class Qureg(C.Structure):
+    _fields_ = [("isMultithreaded", cint), ("isGpuAccelerated", cint), ("isDistributed", cint),
+                ("rank", cint), ("numNodes", cint), ("logNumNodes", cint), ("isDensityMatrix", cint),
+                ("numQubits", cint), ("numAmps", qindex), ("logNumAmps", qindex),
+                ("numAmpsPerNode", qindex), ("logNumAmpsPerNode", qindex), ("logNumColsPerNode", qindex),
+                ("cpuAmps", ptr), ("gpuAmps", ptr), ("cpuCommBuffer", ptr), ("gpuCommBuffer", ptr)]
+
+
+class CompMatr1(C.Structure):
+    _fields_ = [("numQubits", cint), ("numRows", qindex), ("elems", qcomp * 2 * 2)]
+
+
+class CompMatr2(C.Structure):
+    _fields_ = [("numQubits", cint), ("numRows", qindex), ("elems", qcomp * 4 * 4)]
+
+
+class CompMatr(C.Structure):
+    _fields_ = [("numQubits", cint), ("numRows", qindex), ("isApproxUnitary", IntP),
+                ("isApproxHermitian", IntP), ("wasGpuSynced", IntP),
+                ("cpuElems", C.POINTER(C.POINTER(qcomp))), ("cpuElemsFlat", C.POINTER(qcomp)),
+                ("gpuElemsFlat", ptr)]
+
+
+class DiagMatr1(C.Structure):
+    _fields_ = [("numQubits", cint), ("numElems", qindex), ("elems", qcomp * 2)]
+
+
+class DiagMatr2(C.Structure):
+    _fields_ = [("numQubits", cint), ("numElems", qindex), ("elems", qcomp * 4)]
+
+
+class DiagMatr(C.Structure):
+    _fields_ = [("numQubits", cint), ("numElems", qindex), ("isApproxUnitary", IntP),
+                ("isApproxHermitian", IntP), ("isApproxNonZero", IntP), ("isStrictlyNonNegative", IntP),
+                ("wasGpuSynced", IntP), ("cpuElems", C.POINTER(qcomp)), ("gpuElems", ptr)]
+
+
+class FullStateDiagMatr(C.Structure):
+    _fields_ = [("numQubits", cint), ("numElems", qindex), ("isGpuAccelerated", cint),
+                ("isMultithreaded", cint), ("isDistributed", cint), ("numElemsPerNode", qindex),
+                ("isApproxUnitary", IntP), ("isApproxHermitian", IntP), ("isApproxNonZero", IntP),
+                ("isStrictlyNonNegative", IntP), ("wasGpuSynced", IntP), ("cpuElems", C.POINTER(qcomp)),
+                ("gpuElems", ptr)]
+
+
+class PauliStr(C.Structure):
+    _fields_ = [("lowPaulis", C.c_ulonglong), ("highPaulis", C.c_ulonglong)]
+
+
+class PauliStrSum(C.Structure):
+    _fields_ = [("numTerms", qindex), ("strings", C.POINTER(PauliStr)), ("coeffs", C.POINTER(qcomp)),
+                ("isApproxHermitian", IntP)]
+
+
+class SuperOp(C.Structure):
+    _fields_ = [("numQubits", cint), ("numRows", qindex), ("cpuElems", C.POINTER(C.POINTER(qcomp))),
+                ("cpuElemsFlat", C.POINTER(qcomp)), ("gpuElemsFlat", ptr), ("wasGpuSynced", IntP)]
+
+
+class KrausMap(C.Structure):
+    _fields_ = [("numQubits", cint), ("numMatrices", cint), ("numRows", qindex),
+                ("matrices", C.POINTER(C.POINTER(C.POINTER(qcomp)))), ("superop", SuperOp),
+                ("isApproxCPTP", IntP)]
+
+
+# ---------------------------------------------------------------- errors
+
+class QuESTError(RuntimeError):
+    def __init__(self, func: str, msg: str):
+        super().__init__(f"{func}: {msg}")
+        self.func = func
+        self.msg = msg
+
+
+_err = threading.local()
+_HANDLER_T = C.CFUNCTYPE(None, C.c_char_p, C.c_char_p)
+
+
+def _on_error(func, msg):
+    _err.last = (func.decode(), msg.decode())
+
+
+_handler = _HANDLER_T(_on_error)
+
+
+def _raise_pending():
+    e = getattr(_err, "last", None)
+    if e is not None:
+        _err.last = None
+        raise QuESTError(*e)
+
+
+# ------------------------------------------------------------- signatures
+
+_SIGS = {
+    # environment / debug
+    "initQuESTEnv": (None, []), "initCustomQuESTEnv": (None, [cint, cint, cint]),
+    "finalizeQuESTEnv": (None, []), "syncQuESTEnv": (None, []), "reportQuESTEnv": (None, []),
+    "isQuESTEnvInit": (cint, []), "getQuESTEnv": (QuESTEnv, []),
+    "getEnvironmentString": (None, [C.c_char_p]),
+    "setSeeds": (None, [C.POINTER(C.c_uint), cint]), "setSeedsToDefault": (None, []),
+    "getSeeds": (None, [C.POINTER(C.c_uint)]), "getNumSeeds": (cint, []),
+    "setInputErrorHandler": (None, [_HANDLER_T]),
+    "setValidationOn": (None, []), "setValidationOff": (None, []),
+    "setValidationEpsilon": (None, [qreal]), "setValidationEpsilonToDefault": (None, []),
+    "getValidationEpsilon": (qreal, []),
+    # qureg
+    "createQureg": (Qureg, [cint]), "createDensityQureg": (Qureg, [cint]),
+    "createForcedQureg": (Qureg, [cint]), "createForcedDensityQureg": (Qureg, [cint]),
+    "createCustomQureg": (Qureg, [cint, cint, cint, cint, cint]), "createCloneQureg": (Qureg, [Qureg]),
+    "createCustomPrecisionQureg": (Qureg, [cint, cint, cint]), "getQuregPrecision": (cint, [Qureg]),
+    "getQuregDeviceBytes": (qindex, [cint, cint, cint]),
+    "destroyQureg": (None, [Qureg]), "reportQuregParams": (None, [Qureg]), "reportQureg": (None, [Qureg]),
+    "syncQuregToGpu": (None, [Qureg]), "syncQuregFromGpu": (None, [Qureg]),
+    "getQuregAmps": (None, [ptr, Qureg, qindex, qindex]),
+    "_wrap_getQuregAmp": (None, [C.POINTER(qcomp), Qureg, qindex]),
+    "_wrap_getDensityQuregAmp": (None, [C.POINTER(qcomp), Qureg, qindex, qindex]),
+    # initialisation
+    "initBlankState": (None, [Qureg]), "initZeroState": (None, [Qureg]), "initPlusState": (None, [Qureg]),
+    "initPureState": (None, [Qureg, Qureg]), "initClassicalState": (None, [Qureg, qindex]),
+    "initDebugState": (None, [Qureg]), "initArbitraryPureState": (None, [Qureg, ptr]),
+    "initRandomPureState": (None, [Qureg]), "initRandomMixedState": (None, [Qureg, qindex]),
+    "setQuregAmps": (None, [Qureg, qindex, ptr, qindex]),
+    "setDensityQuregFlatAmps": (None, [Qureg, qindex, ptr, qindex]),
+    "setQuregToClone": (None, [Qureg, Qureg]),
+    "setQuregToSuperposition": (None, [qcomp, Qureg, qcomp, Qureg, qcomp, Qureg]),
+    "setQuregToRenormalized": (qreal, [Qureg]),
+    "setQuregToPartialTrace": (None, [Qureg, Qureg, IntP, cint]),
+    # matrices / paulis / channels
+    "createCompMatr": (CompMatr, [cint]), "destroyCompMatr": (None, [CompMatr]),
+    "setCompMatr": (None, [CompMatr, C.POINTER(C.POINTER(qcomp))]), "syncCompMatr": (None, [CompMatr]),
+    "createDiagMatr": (DiagMatr, [cint]), "destroyDiagMatr": (None, [DiagMatr]),
+    "setDiagMatr": (None, [DiagMatr, C.POINTER(qcomp)]),
+    "createFullStateDiagMatr": (FullStateDiagMatr, [cint]),
+    "destroyFullStateDiagMatr": (None, [FullStateDiagMatr]),
+    "setFullStateDiagMatr": (None, [FullStateDiagMatr, qindex, C.POINTER(qcomp), qindex]),
+    "getPauliStr": (PauliStr, [C.c_char_p, IntP, cint]),
+    "createPauliStrSum": (PauliStrSum, [C.POINTER(PauliStr), C.POINTER(qcomp), qindex]),
+    "createInlinePauliStrSum": (PauliStrSum, [C.c_char_p]),
+    "destroyPauliStrSum": (None, [PauliStrSum]),
+    "createKrausMap": (KrausMap, [cint, cint]), "destroyKrausMap": (None, [KrausMap]),
+    "setKrausMap": (None, [KrausMap, C.POINTER(C.POINTER(C.POINTER(qcomp)))]),
+    "createSuperOp": (SuperOp, [cint]), "destroySuperOp": (None, [SuperOp]),
+    "setSuperOp": (None, [SuperOp, C.POINTER(C.POINTER(qcomp))]),
+    # operations
+    "multiplyCompMatr1": (None, [Qureg, cint, CompMatr1]),
+    "applyCompMatr1": (None, [Qureg, cint, CompMatr1]),
+    "applyControlledCompMatr1": (None, [Qureg, cint, cint, CompMatr1]),
+    "applyMultiControlledCompMatr1": (None, [Qureg, IntP, cint, cint, CompMatr1]),
+    "applyMultiStateControlledCompMatr1": (None, [Qureg, IntP, IntP, cint, cint, CompMatr1]),
+    "applyCompMatr2": (None, [Qureg, cint, cint, CompMatr2]),
+    "applyControlledCompMatr2": (None, [Qureg, cint, cint, cint, CompMatr2]),
+    "applyMultiControlledCompMatr2": (None, [Qureg, IntP, cint, cint, cint, CompMatr2]),
+    "applyCompMatr": (None, [Qureg, IntP, cint, CompMatr]),
+    "applyMultiControlledCompMatr": (None, [Qureg, IntP, cint, IntP, cint, CompMatr]),
+    "applyDiagMatr1": (None, [Qureg, cint, DiagMatr1]),
+    "applyMultiControlledDiagMatr1": (None, [Qureg, IntP, cint, cint, DiagMatr1]),
+    "applyDiagMatr2": (None, [Qureg, cint, cint, DiagMatr2]),
+    "applyDiagMatr": (None, [Qureg, IntP, cint, DiagMatr]),
+    "applyFullStateDiagMatr": (None, [Qureg, FullStateDiagMatr]),
+    "applyHadamard": (None, [Qureg, cint]), "applyPauliX": (None, [Qureg, cint]),
+    "applyPauliY": (None, [Qureg, cint]), "applyPauliZ": (None, [Qureg, cint]),
+    "applyS": (None, [Qureg, cint]), "applyT": (None, [Qureg, cint]),
+    "applyControlledPauliX": (None, [Qureg, cint, cint]), "applyControlledPauliZ": (None, [Qureg, cint, cint]),
+    "applyPhaseShift": (None, [Qureg, cint, qreal]),
+    "applyTwoQubitPhaseShift": (None, [Qureg, cint, cint, qreal]),
+    "applyMultiQubitPhaseShift": (None, [Qureg, IntP, cint, qreal]),
+    "applyRotateX": (None, [Qureg, cint, qreal]), "applyRotateY": (None, [Qureg, cint, qreal]),
+    "applyRotateZ": (None, [Qureg, cint, qreal]),
+    "applySwap": (None, [Qureg, cint, cint]),
+    "applyMultiControlledSwap": (None, [Qureg, IntP, cint, cint, cint]),
+    "applyPauliStr": (None, [Qureg, PauliStr]),
+    "applyMultiControlledPauliStr": (None, [Qureg, IntP, cint, PauliStr]),
+    "applyPauliGadget": (None, [Qureg, PauliStr, qreal]),
+    "applyMultiControlledPauliGadget": (None, [Qureg, IntP, cint, PauliStr, qreal]),
+    "applyPhaseGadget": (None, [Qureg, IntP, cint, qreal]),
+    "applyQuantumFourierTransform": (None, [Qureg, IntP, cint]),
+    "applyFullQuantumFourierTransform": (None, [Qureg]),
+    "applyQubitProjector": (None, [Qureg, cint, cint]),
+    "applyQubitMeasurement": (cint, [Qureg, cint]),
+    "applyForcedQubitMeasurement": (qreal, [Qureg, cint, cint]),
+    # decoherence
+    "mixDephasing": (None, [Qureg, cint, qreal]), "mixTwoQubitDephasing": (None, [Qureg, cint, cint, qreal]),
+    "mixDepolarising": (None, [Qureg, cint, qreal]),
+    "mixTwoQubitDepolarising": (None, [Qureg, cint, cint, qreal]),
+    "mixDamping": (None, [Qureg, cint, qreal]), "mixPaulis": (None, [Qureg, cint, qreal, qreal, qreal]),
+    "mixQureg": (None, [Qureg, Qureg, qreal]), "mixKrausMap": (None, [Qureg, IntP, cint, KrausMap]),
+    "mixSuperOp": (None, [Qureg, IntP, cint, SuperOp]),
+    # calculations
+    "calcExpecPauliStr": (qreal, [Qureg, PauliStr]), "calcExpecPauliStrSum": (qreal, [Qureg, PauliStrSum]),
+    "calcProbOfBasisState": (qreal, [Qureg, qindex]),
+    "calcProbOfQubitOutcome": (qreal, [Qureg, cint, cint]),
+    "calcProbOfMultiQubitOutcome": (qreal, [Qureg, IntP, IntP, cint]),
+    "calcProbsOfAllMultiQubitOutcomes": (None, [C.POINTER(qreal), Qureg, IntP, cint]),
+    "calcTotalProb": (qreal, [Qureg]), "calcPurity": (qreal, [Qureg]),
+    "calcFidelity": (qreal, [Qureg, Qureg]), "calcDistance": (qreal, [Qureg, Qureg]),
+    "_wrap_calcInnerProduct": (None, [Qureg, Qureg, C.POINTER(qcomp)]),
+    "calcPartialTrace": (Qureg, [Qureg, IntP, cint]),
+    "getQuESTStream": (ptr, []),
+}
+
+_raw = {}
+for _name, (_res, _args) in _SIGS.items():
+    _f = getattr(_lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+    _raw[_name] = _f
+
+
+def _wrap(name):
+    f = _raw[name]
+
+    def call(*args):
+        _err.last = None
+        out = f(*args)
+        _raise_pending()
+        return out
+
+    call.__name__ = name
+    return call
+
+
+for _name in _SIGS:
+    if not _name.startswith("_"):
+        globals()[_name] = _wrap(_name)
+
+_raw["setInputErrorHandler"](_handler)
+
+
+# ------------------------------------------------------------- helpers
+
+def ints(values) -> "C.Array":
+    values = [int(v) for v in values]
+    return (C.c_int * max(1, len(values)))(*values)
+
+
+def compmatr1(m) -> CompMatr1:
+    m = np.asarray(m, dtype=complex).reshape(2, 2)
+    out = CompMatr1()
+    out.numQubits, out.numRows = 1, 2
+    for r in range(2):
+        for c in range(2):
+            out.elems[r][c] = qcomp.of(m[r, c])
+    return out
+
+
+def compmatr2(m) -> CompMatr2:
+    m = np.asarray(m, dtype=complex).reshape(4, 4)
+    out = CompMatr2()
+    out.numQubits, out.numRows = 2, 4
+    for r in range(4):
+        for c in range(4):
+            out.elems[r][c] = qcomp.of(m[r, c])
+    return out
+
+
+def diagmatr1(d) -> DiagMatr1:
+    out = DiagMatr1()
+    out.numQubits, out.numElems = 1, 2
+    for i, v in enumerate(d):
+        out.elems[i] = qcomp.of(v)
+    return out
+
+
+def diagmatr2(d) -> DiagMatr2:
+    out = DiagMatr2()
+    out.numQubits, out.numElems = 2, 4
+    for i, v in enumerate(d):
+        out.elems[i] = qcomp.of(v)
+    return out
+
+
+def _rows(m: np.ndarray):
+    """qcomp** view of a C-contiguous complex128 matrix (keeps refs alive)."""
+    m = np.ascontiguousarray(m, dtype=np.complex128)
+    rowptrs = (C.POINTER(qcomp) * m.shape[0])(
+        *[C.cast(m[r].ctypes.data, C.POINTER(qcomp)) for r in range(m.shape[0])])
+    return m, rowptrs
+
+
+def make_compmatr(m) -> CompMatr:
+    m = np.asarray(m, dtype=complex)
+    n = int(round(np.log2(m.shape[0])))
+    cm = createCompMatr(n)
+    keep, rows = _rows(m)
+    setCompMatr(cm, rows)
+    return cm
+
+
+def make_diagmatr(d) -> DiagMatr:
+    d = np.ascontiguousarray(np.asarray(d, dtype=np.complex128))
+    n = int(round(np.log2(d.shape[0])))
+    dm = createDiagMatr(n)
+    setDiagMatr(dm, C.cast(d.ctypes.data, C.POINTER(qcomp)))
+    return dm
+
+
+def make_krausmap(ops) -> KrausMap:
+    ops = [np.ascontiguousarray(np.asarray(k, dtype=np.complex128)) for k in ops]
+    n = int(round(np.log2(ops[0].shape[0])))
+    km = createKrausMap(n, len(ops))
+    keep = []
+    mats = (C.POINTER(C.POINTER(qcomp)) * len(ops))()
+    for i, k in enumerate(ops):
+        arr, rows = _rows(k)
+        keep.append((arr, rows))
+        mats[i] = C.cast(rows, C.POINTER(C.POINTER(qcomp)))
+    setKrausMap(km, mats)
+    return km
+
+
+def pauli_str(chars: str, targets) -> PauliStr:
+    t = ints(targets)
+    return getPauliStr(chars.encode(), t, len(chars))
+
+
+def get_amps(q: Qureg) -> np.ndarray:
+    """All amplitudes as complex128 (density matrices flat column-major)."""
+    out = np.empty(q.numAmps, dtype=np.complex128)
+    rc = _gpu.qa_getAmps(out.ctypes.data_as(C.c_void_p), C.c_void_p(q.gpuAmps),
+                         C.c_int(getQuregPrecision(q)), C.c_int64(0), C.c_int64(q.numAmps))
+    if rc != 0:
+        raise RuntimeError(_gpu_last_error())
+    return out
+
+
+def set_amps(q: Qureg, amps) -> None:
+    a = np.ascontiguousarray(np.asarray(amps, dtype=np.complex128).reshape(-1))
+    if a.size != q.numAmps:
+        raise ValueError(f"expected {q.numAmps} amplitudes, got {a.size}")
+    if q.isDensityMatrix:
+        setDensityQuregFlatAmps(q, 0, a.ctypes.data_as(C.c_void_p), a.size)
+    else:
+        setQuregAmps(q, 0, a.ctypes.data_as(C.c_void_p), a.size)
+
+
+def inner_product(a: Qureg, b: Qureg) -> complex:
+    out = qcomp()
+    _err.last = None
+    _raw["_wrap_calcInnerProduct"](a, b, C.byref(out))
+    _raise_pending()
+    return out.value()
+
+
+def probs_of_all_outcomes(q: Qureg, qubits) -> np.ndarray:
+    out = np.empty(1 << len(qubits), dtype=np.float64)
+    calcProbsOfAllMultiQubitOutcomes(out.ctypes.data_as(C.POINTER(qreal)), q, ints(qubits), len(qubits))
+    return out
+
+
+def set_seeds(seeds) -> None:
+    arr = (C.c_uint * len(seeds))(*[int(s) & 0xFFFFFFFF for s in seeds])
+    setSeeds(arr, len(seeds))
+
+
+# ------------------------------------------------- kernel-layer C ABI
+
+_gpu.qa_lastError.restype = C.c_char_p
+
+
+def _gpu_last_error() -> str:
+    return _gpu.qa_lastError().decode()
+
+
+def gpu():
+    """The raw kernel-layer library (include/quest_amd_gpu.h)."""
+    return _gpu
+
+
+def ensure_env() -> None:
+    if not isQuESTEnvInit():
+        initQuESTEnv()

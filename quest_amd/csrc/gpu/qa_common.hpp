@@ -1,0 +1,186 @@
+// Shared device/host helpers of the gfx950 kernel layer: amplitude vector
+// types, index bit algebra, launch shapes, error plumbing.
+//
+// Index algebra follows quest/src/core/bitwise.hpp:34-254 (insertBits over
+// ascending qubit positions, setBits over an unsorted target list, parity of
+// masked bits); it is restated here for 64-bit device indices.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "quest_amd_gpu.h"
+
+namespace qa {
+
+// ---- amplitude storage -------------------------------------------------
+// Interleaved {re, im}; native clang vectors so that a whole amplitude moves
+// with one global_load/store (dwordx4 for FP64, dwordx2 for FP32).
+typedef double v2d __attribute__((ext_vector_type(2)));
+typedef float v2f __attribute__((ext_vector_type(2)));
+
+template <typename T> struct Amp;
+template <> struct Amp<double> { typedef v2d V; };
+template <> struct Amp<float> { typedef v2f V; };
+
+// complex value held in FP64 registers regardless of storage precision
+struct cd {
+    double re, im;
+};
+
+__host__ __device__ __forceinline__ cd mk(double r, double i) { return cd{r, i}; }
+__host__ __device__ __forceinline__ cd cmul(cd a, cd b) {
+    return cd{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__host__ __device__ __forceinline__ cd cadd(cd a, cd b) { return cd{a.re + b.re, a.im + b.im}; }
+__host__ __device__ __forceinline__ cd cconj(cd a) { return cd{a.re, -a.im}; }
+__host__ __device__ __forceinline__ cd cscale(cd a, double s) { return cd{a.re * s, a.im * s}; }
+
+// complex arithmetic in the storage precision T (FP32 gates stay FP32)
+template <typename T>
+struct ct {
+    T re, im;
+};
+template <typename T>
+__device__ __forceinline__ ct<T> cmulT(ct<T> a, ct<T> b) {
+    return ct<T>{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T>
+__device__ __forceinline__ ct<T> caddT(ct<T> a, ct<T> b) {
+    return ct<T>{a.re + b.re, a.im + b.im};
+}
+template <typename T>
+__device__ __forceinline__ ct<T> cfmaT(ct<T> m, ct<T> a, ct<T> acc) {
+    // acc + m*a
+    return ct<T>{acc.re + m.re * a.re - m.im * a.im, acc.im + m.re * a.im + m.im * a.re};
+}
+
+template <typename T>
+__device__ __forceinline__ ct<T> ld(const T* __restrict__ amps, uint64_t i) {
+    typedef typename Amp<T>::V V;
+    V v = reinterpret_cast<const V*>(amps)[i];
+    return ct<T>{v.x, v.y};
+}
+template <typename T>
+__device__ __forceinline__ void st(T* __restrict__ amps, uint64_t i, ct<T> a) {
+    typedef typename Amp<T>::V V;
+    V v;
+    v.x = a.re;
+    v.y = a.im;
+    reinterpret_cast<V*>(amps)[i] = v;
+}
+// streaming (non-temporal) variants for single-pass kernels with no reuse
+template <typename T>
+__device__ __forceinline__ ct<T> ldnt(const T* __restrict__ amps, uint64_t i) {
+    typedef typename Amp<T>::V V;
+    V v = __builtin_nontemporal_load(reinterpret_cast<const V*>(amps) + i);
+    return ct<T>{v.x, v.y};
+}
+template <typename T>
+__device__ __forceinline__ void stnt(T* __restrict__ amps, uint64_t i, ct<T> a) {
+    typedef typename Amp<T>::V V;
+    V v;
+    v.x = a.re;
+    v.y = a.im;
+    __builtin_nontemporal_store(v, reinterpret_cast<V*>(amps) + i);
+}
+
+template <typename T>
+__host__ __device__ __forceinline__ ct<T> toT(cd a) {
+    return ct<T>{(T)a.re, (T)a.im};
+}
+
+// ---- index bit algebra -------------------------------------------------
+constexpr int MAXQ = 48;
+
+// Ascending qubit positions whose bits get inserted into a work-item index,
+// plus the values those bits take (control states, zero target bits).
+struct InsertSpec {
+    int num;
+    int pos[MAXQ];
+    uint64_t valueMask;
+};
+
+// insert a zero at every pos[j] (ascending), then OR in valueMask
+// (bitwise.hpp:insertBitsWithMaskedValues)
+__device__ __forceinline__ uint64_t insertBits(uint64_t n, const InsertSpec& s) {
+    for (int j = 0; j < s.num; j++) {
+        int q = s.pos[j];
+        uint64_t lo = n & ((1ull << q) - 1ull);
+        n = ((n >> q) << (q + 1)) | lo;
+    }
+    return n | s.valueMask;
+}
+
+// unsorted target list; bit i of a value <-> targ[i]
+struct TargSpec {
+    int num;
+    int targ[MAXQ];
+};
+
+// bitwise.hpp:setBits -- set bits targ[i] of n to bit i of v
+__device__ __forceinline__ uint64_t setBitsFrom(uint64_t n, const TargSpec& t, uint64_t v) {
+    for (int i = 0; i < t.num; i++) {
+        uint64_t b = 1ull << t.targ[i];
+        n = ((v >> i) & 1ull) ? (n | b) : (n & ~b);
+    }
+    return n;
+}
+
+// bitwise.hpp:getValueOfBits
+__device__ __forceinline__ uint64_t valueOfBits(uint64_t n, const TargSpec& t) {
+    uint64_t v = 0;
+    for (int i = 0; i < t.num; i++) v |= ((n >> t.targ[i]) & 1ull) << i;
+    return v;
+}
+
+__device__ __forceinline__ int parity64(uint64_t x) { return __popcll(x) & 1; }
+
+// ---- launch shapes -----------------------------------------------------
+constexpr int BLOCK = 256;   // 4 wave64s per workgroup
+constexpr int ITEMS = 4;     // work items per thread, strided by BLOCK
+
+inline unsigned gridFor(uint64_t work, int items = ITEMS) {
+    uint64_t per = (uint64_t)BLOCK * (uint64_t)items;
+    uint64_t g = (work + per - 1) / per;
+    return (unsigned)(g ? g : 1);
+}
+
+// ---- errors ------------------------------------------------------------
+void setError(const char* fmt, ...);
+hipStream_t stream();
+bool isInit();
+// reduction scratch: device partials and a pinned host landing area
+double* devScratch(size_t numDoubles);
+double* hostScratch(size_t numDoubles);
+
+}  // namespace qa
+
+#define QA_REQUIRE_INIT()                                                  \
+    do {                                                                   \
+        if (!qa::isInit()) {                                               \
+            qa::setError("%s: qa_init() has not been called", __func__);   \
+            return QA_ERR_NOT_INIT;                                        \
+        }                                                                  \
+    } while (0)
+
+#define QA_HIP(cmd)                                                                   \
+    do {                                                                              \
+        hipError_t e_ = (cmd);                                                        \
+        if (e_ != hipSuccess) {                                                       \
+            qa::setError("%s: %s failed: %s", __func__, #cmd, hipGetErrorString(e_)); \
+            return QA_ERR_HIP;                                                        \
+        }                                                                             \
+    } while (0)
+
+#define QA_LAUNCH_CHECK() QA_HIP(hipGetLastError())
+
+#define QA_ARG_CHECK(cond, ...)       \
+    do {                              \
+        if (!(cond)) {                \
+            qa::setError(__VA_ARGS__); \
+            return QA_ERR_ARG;        \
+        }                             \
+    } while (0)

@@ -1,0 +1,173 @@
+// State initialisation and projector kernels, replacing
+// quest/src/gpu/gpu_thrust.cuh:1015-1084 (projectors, uniform/debug/random
+// amplitudes) with plain streaming kernels.
+#include <cmath>
+
+#include "qa_common.hpp"
+
+namespace qa {
+namespace {
+
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_fill(T* __restrict__ amps, uint64_t num, T re, T im) {
+    const uint64_t base = (uint64_t)blockIdx.x * (BLOCK * ITEMS) + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        uint64_t i = base + (uint64_t)j * BLOCK;
+        if (i < num) stnt(amps, i, ct<T>{re, im});
+    }
+}
+
+// a_i = (2i/10, (2i+1)/10)  (cpu_subroutines.cpp: cpu_statevec_initDebugState_sub)
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_debug(T* __restrict__ amps, uint64_t num) {
+    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= num) return;
+    double re = (double)(2 * i) / 10.;
+    double im = (double)(2 * i + 1) / 10.;
+    st(amps, i, ct<T>{(T)re, (T)im});
+}
+
+// SplitMix64 finaliser (Steele, Lea & Flood 2014) as a counter-based hash
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ double unit(uint64_t x) {  // (0, 1]
+    return ((double)(x >> 11) + 1.0) * (1.0 / 9007199254740992.0);
+}
+
+// amplitude = sqrt(n1^2 + n2^2) e^{i phi}, n1,n2 ~ N(0,1) (Box-Muller), phi ~ U(0,2pi):
+// the distribution of randomiser.cpp rand_getThreadPrivateRandomAmp
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_random(T* __restrict__ amps, uint64_t num, uint64_t seed) {
+    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= num) return;
+    uint64_t k = seed * 0x9e3779b97f4a7c15ull + i * 0xd1b54a32d192ed03ull;
+    double u1 = unit(mix64(k ^ 0x1ull)), u3 = unit(mix64(k ^ 0x3ull));
+    // n1^2 + n2^2 = -2 ln u1 for a Box-Muller pair, whatever its angle
+    double mag = sqrt(-2.0 * log(u1));
+    double phi = 2.0 * M_PI * u3;
+    st(amps, i, ct<T>{(T)(mag * cos(phi)), (T)(mag * sin(phi))});
+}
+
+// keep amplitudes whose qubits equal the outcome bits, scaled by renorm
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_projSV(T* __restrict__ amps, uint64_t num, uint64_t qubitMask,
+                                                 uint64_t outcomeMask, T renorm) {
+    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= num) return;
+    T f = ((i & qubitMask) == outcomeMask) ? renorm : (T)0;
+    ct<T> a = ld(amps, i);
+    st(amps, i, ct<T>{a.re * f, a.im * f});
+}
+
+// rho_rc *= renorm [bits(r) == bits(c) == outcome]  (cpu_densmatr_multiQubitProjector_sub)
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_projDM(T* __restrict__ amps, uint64_t num, int N, uint64_t qubitMask,
+                                                 uint64_t outcomeMask, T renorm) {
+    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= num) return;
+    uint64_t r = i & ((1ull << N) - 1), c = i >> N;
+    bool keep = ((r & qubitMask) == outcomeMask) && ((c & qubitMask) == outcomeMask);
+    T f = keep ? renorm : (T)0;
+    ct<T> a = ld(amps, i);
+    st(amps, i, ct<T>{a.re * f, a.im * f});
+}
+
+#define QA_DISPATCH(prec, ...)   \
+    do {                         \
+        if ((prec) == QA_FP64) { \
+            typedef double T;    \
+            __VA_ARGS__;         \
+        } else {                 \
+            typedef float T;     \
+            __VA_ARGS__;         \
+        }                        \
+    } while (0)
+
+bool ok(const void* a, int prec, int nq, const char* caller) {
+    if (!a || (prec != QA_FP32 && prec != QA_FP64) || nq < 1 || nq > 62) {
+        setError("%s: bad qureg arguments", caller);
+        return false;
+    }
+    return true;
+}
+
+bool masks(int nq, const int* qubits, const int* outcomes, int num, uint64_t& qm, uint64_t& om,
+           const char* caller) {
+    qm = om = 0;
+    for (int i = 0; i < num; i++) {
+        int q = qubits[i];
+        if (q < 0 || q >= nq || ((qm >> q) & 1ull) || (outcomes[i] != 0 && outcomes[i] != 1)) {
+            setError("%s: bad qubit/outcome", caller);
+            return false;
+        }
+        qm |= 1ull << q;
+        if (outcomes[i]) om |= 1ull << q;
+    }
+    return true;
+}
+
+}  // namespace
+}  // namespace qa
+
+using namespace qa;
+
+extern "C" {
+
+int qa_statevec_initUniformState(void* amps, int prec, int nq, double re, double im) {
+    QA_REQUIRE_INIT();
+    if (!ok(amps, prec, nq, __func__)) return QA_ERR_ARG;
+    uint64_t n = 1ull << nq;
+    QA_DISPATCH(prec, (k_fill<T><<<gridFor(n), BLOCK, 0, stream()>>>((T*)amps, n, (T)re, (T)im)));
+    QA_LAUNCH_CHECK();
+    return QA_OK;
+}
+
+int qa_statevec_initDebugState(void* amps, int prec, int nq) {
+    QA_REQUIRE_INIT();
+    if (!ok(amps, prec, nq, __func__)) return QA_ERR_ARG;
+    uint64_t n = 1ull << nq;
+    QA_DISPATCH(prec, (k_debug<T><<<gridFor(n, 1), BLOCK, 0, stream()>>>((T*)amps, n)));
+    QA_LAUNCH_CHECK();
+    return QA_OK;
+}
+
+int qa_statevec_initUnnormalisedUniformlyRandomPureStateAmps(void* amps, int prec, int nq, uint64_t seed) {
+    QA_REQUIRE_INIT();
+    if (!ok(amps, prec, nq, __func__)) return QA_ERR_ARG;
+    uint64_t n = 1ull << nq;
+    QA_DISPATCH(prec, (k_random<T><<<gridFor(n, 1), BLOCK, 0, stream()>>>((T*)amps, n, seed)));
+    QA_LAUNCH_CHECK();
+    return QA_OK;
+}
+
+int qa_statevec_multiQubitProjector(void* amps, int prec, int nq, const int* qubits, const int* outcomes, int num,
+                                    double prob) {
+    QA_REQUIRE_INIT();
+    if (!ok(amps, prec, nq, __func__)) return QA_ERR_ARG;
+    uint64_t qm, om;
+    if (!masks(nq, qubits, outcomes, num, qm, om, __func__)) return QA_ERR_ARG;
+    uint64_t n = 1ull << nq;
+    double renorm = 1 / std::sqrt(prob);
+    QA_DISPATCH(prec, (k_projSV<T><<<gridFor(n, 1), BLOCK, 0, stream()>>>((T*)amps, n, qm, om, (T)renorm)));
+    QA_LAUNCH_CHECK();
+    return QA_OK;
+}
+
+int qa_densmatr_multiQubitProjector(void* amps, int prec, int N, const int* qubits, const int* outcomes, int num,
+                                    double prob) {
+    QA_REQUIRE_INIT();
+    if (!ok(amps, prec, 2 * N, __func__)) return QA_ERR_ARG;
+    uint64_t qm, om;
+    if (!masks(N, qubits, outcomes, num, qm, om, __func__)) return QA_ERR_ARG;
+    uint64_t n = 1ull << (2 * N);
+    double renorm = 1 / prob;
+    QA_DISPATCH(prec, (k_projDM<T><<<gridFor(n, 1), BLOCK, 0, stream()>>>((T*)amps, n, N, qm, om, (T)renorm)));
+    QA_LAUNCH_CHECK();
+    return QA_OK;
+}
+
+}  // extern "C"

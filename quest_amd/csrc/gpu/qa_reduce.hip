@@ -1,0 +1,584 @@
+// FP64 reductions (probabilities, purity, Pauli expectations, inner
+// products, fidelities) replacing the Thrust transform_reduce calls of
+// quest/src/gpu/gpu_thrust.cuh:745-1005 and gpu_subroutines.cpp:1440-1723.
+//
+// Two kernels per reduction, both deterministic for a given problem size:
+//  1. a fixed-size grid (at most RED_BLOCKS workgroups) sweeps the terms in a
+//     grid-stride loop, 4 independent loads in flight per lane; each lane
+//     keeps a compensated (Neumaier) FP64 sum, waves combine lanes with
+//     cross-lane shuffles and a two-sum merge, workgroups combine their 4
+//     waves through LDS and write one (sum, compensation) pair;
+//  2. one workgroup merges the partials in a fixed order and writes the final
+//     value, which is copied to pinned host memory.
+// Amplitudes are widened to FP64 before squaring, so FP32 states reduce in
+// FP64 too.
+#include "qa_common.hpp"
+
+namespace qa {
+namespace {
+
+constexpr int RED_BLOCKS = 1024;
+constexpr int MAXV = 2;  // values reduced at once (complex results)
+constexpr size_t RESULT_OFF = (size_t)RED_BLOCKS * MAXV * 2;
+
+struct KB {  // compensated accumulator
+    double s, c;
+};
+
+__device__ __forceinline__ void kadd(KB& a, double v) {
+    double t = a.s + v;
+    a.c += (fabs(a.s) >= fabs(v)) ? ((a.s - t) + v) : ((v - t) + a.s);
+    a.s = t;
+}
+__device__ __forceinline__ void kmerge(KB& a, KB b) {
+    double t = a.s + b.s;
+    double e = (fabs(a.s) >= fabs(b.s)) ? ((a.s - t) + b.s) : ((b.s - t) + a.s);
+    a.s = t;
+    a.c += b.c + e;
+}
+
+// reduce NV compensated accumulators across the workgroup; lane 0 of wave 0
+// ends holding the result
+template <int NV>
+__device__ void blockMerge(KB (&acc)[NV]) {
+    __shared__ double sh[BLOCK / 64][NV][2];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            KB o;
+            o.s = __shfl_xor(acc[v].s, off, 64);
+            o.c = __shfl_xor(acc[v].c, off, 64);
+            kmerge(acc[v], o);
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int v = 0; v < NV; v++) {
+            sh[wave][v][0] = acc[v].s;
+            sh[wave][v][1] = acc[v].c;
+        }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int w = 1; w < BLOCK / 64; w++)
+            for (int v = 0; v < NV; v++) kmerge(acc[v], KB{sh[w][v][0], sh[w][v][1]});
+}
+
+template <int NV, typename F>
+__global__ void __launch_bounds__(BLOCK) k_reduce(F f, uint64_t numWork, double* __restrict__ partials) {
+    KB acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) acc[v] = KB{0, 0};
+    const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+    uint64_t n = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    for (; n + 3 * stride < numWork; n += 4 * stride) {
+        double t[4][NV];
+#pragma unroll
+        for (int u = 0; u < 4; u++) f(n + (uint64_t)u * stride, t[u]);
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int v = 0; v < NV; v++) kadd(acc[v], t[u][v]);
+    }
+    for (; n < numWork; n += stride) {
+        double t[NV];
+        f(n, t);
+#pragma unroll
+        for (int v = 0; v < NV; v++) kadd(acc[v], t[v]);
+    }
+    blockMerge<NV>(acc);
+    if (threadIdx.x == 0)
+        for (int v = 0; v < NV; v++) {
+            partials[(blockIdx.x * NV + v) * 2] = acc[v].s;
+            partials[(blockIdx.x * NV + v) * 2 + 1] = acc[v].c;
+        }
+}
+
+template <int NV>
+__global__ void __launch_bounds__(BLOCK) k_finalize(const double* __restrict__ partials, int numParts,
+                                                   double* __restrict__ out) {
+    KB acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) acc[v] = KB{0, 0};
+    for (int p = threadIdx.x; p < numParts; p += BLOCK)
+        for (int v = 0; v < NV; v++) kmerge(acc[v], KB{partials[(p * NV + v) * 2], partials[(p * NV + v) * 2 + 1]});
+    blockMerge<NV>(acc);
+    if (threadIdx.x == 0)
+        for (int v = 0; v < NV; v++) out[v] = acc[v].s + acc[v].c;
+}
+
+template <int NV, typename F>
+int runReduce(F f, uint64_t numWork, double* out) {
+    double* scratch = devScratch(RESULT_OFF + 8);
+    double* host = hostScratch(8);
+    if (!scratch || !host) {
+        setError("reduction scratch unavailable");
+        return QA_ERR_OOM;
+    }
+    uint64_t want = (numWork + (uint64_t)BLOCK * 4 - 1) / ((uint64_t)BLOCK * 4);
+    int blocks = (int)(want < 1 ? 1 : (want > RED_BLOCKS ? RED_BLOCKS : want));
+    k_reduce<NV, F><<<blocks, BLOCK, 0, stream()>>>(f, numWork, scratch);
+    QA_LAUNCH_CHECK();
+    k_finalize<NV><<<1, BLOCK, 0, stream()>>>(scratch, blocks, scratch + RESULT_OFF);
+    QA_LAUNCH_CHECK();
+    QA_HIP(hipMemcpyAsync(host, scratch + RESULT_OFF, NV * sizeof(double), hipMemcpyDeviceToHost, stream()));
+    QA_HIP(hipStreamSynchronize(stream()));
+    for (int v = 0; v < NV; v++) out[v] = host[v];
+    return QA_OK;
+}
+
+template <typename T>
+__device__ __forceinline__ cd ldd(const T* __restrict__ a, uint64_t i) {
+    ct<T> v = ld(a, i);
+    return cd{(double)v.re, (double)v.im};
+}
+__device__ __forceinline__ double norm2(cd a) { return a.re * a.re + a.im * a.im; }
+
+// ---- term functors --------------------------------------------------------
+
+template <typename T>
+struct FNormAll {  // |a_n|^2
+    const T* a;
+    __device__ void operator()(uint64_t n, double* o) const { o[0] = norm2(ldd(a, n)); }
+};
+template <typename T>
+struct FNormOutcome {  // |a_i|^2, i = n with outcome bits inserted
+    const T* a;
+    InsertSpec ins;
+    __device__ void operator()(uint64_t n, double* o) const { o[0] = norm2(ldd(a, insertBits(n, ins))); }
+};
+template <typename T>
+struct FDiagReal {  // Re rho_rr, r = n with outcome bits inserted
+    const T* a;
+    InsertSpec ins;
+    uint64_t diagStride;  // 2^N + 1
+    __device__ void operator()(uint64_t n, double* o) const {
+        o[0] = ldd(a, insertBits(n, ins) * diagStride).re;
+    }
+};
+template <typename T>
+struct FExpecZ {
+    const T* a;
+    uint64_t mask;
+    __device__ void operator()(uint64_t n, double* o) const {
+        double v = norm2(ldd(a, n));
+        o[0] = parity64(n & mask) ? -v : v;
+    }
+};
+template <typename T>
+struct FDensExpecZ {
+    const T* a;
+    uint64_t mask, diagStride;
+    __device__ void operator()(uint64_t n, double* o) const {
+        cd v = ldd(a, n * diagStride);
+        double s = parity64(n & mask) ? -1.0 : 1.0;
+        o[0] = s * v.re;
+        o[1] = s * v.im;
+    }
+};
+template <typename T>
+struct FExpecPauli {  // sign(j & maskYZ) conj(a_n) a_j, j = n ^ maskXY
+    const T* a;
+    uint64_t maskXY, maskYZ;
+    __device__ void operator()(uint64_t n, double* o) const {
+        uint64_t j = n ^ maskXY;
+        cd x = ldd(a, n), y = ldd(a, j);
+        cd t = cmul(cconj(x), y);
+        double s = parity64(j & maskYZ) ? -1.0 : 1.0;
+        o[0] = s * t.re;
+        o[1] = s * t.im;
+    }
+};
+template <typename T>
+struct FDensExpecPauli {  // column n, row i = n ^ maskXY: sign(i & maskYZ) rho_{i,n}
+    const T* a;
+    uint64_t maskXY, maskYZ;
+    int N;
+    __device__ void operator()(uint64_t n, double* o) const {
+        uint64_t i = n ^ maskXY;
+        cd v = ldd(a, i + (n << N));
+        double s = parity64(i & maskYZ) ? -1.0 : 1.0;
+        o[0] = s * v.re;
+        o[1] = s * v.im;
+    }
+};
+template <typename T>
+struct FInner {
+    const T* a;
+    const T* b;
+    __device__ void operator()(uint64_t n, double* o) const {
+        cd t = cmul(cconj(ldd(a, n)), ldd(b, n));
+        o[0] = t.re;
+        o[1] = t.im;
+    }
+};
+template <typename T>
+struct FHSDist {
+    const T* a;
+    const T* b;
+    __device__ void operator()(uint64_t n, double* o) const {
+        cd x = ldd(a, n), y = ldd(b, n);
+        o[0] = norm2(cd{x.re - y.re, x.im - y.im});
+    }
+};
+template <typename T>
+struct FFidelity {  // cpu_densmatr_calcFidelityWithPureState_sub
+    const T* rho;
+    const T* psi;
+    int N, conj;
+    __device__ void operator()(uint64_t n, double* o) const {
+        uint64_t r = n & ((1ull << N) - 1), c = n >> N;
+        cd rv = ldd(rho, n), pr = ldd(psi, r), pc = ldd(psi, c);
+        if (conj) {
+            rv = cconj(rv);
+            pc = cconj(pc);
+        } else {
+            pr = cconj(pr);
+        }
+        cd t = cmul(cmul(rv, pr), pc);
+        o[0] = t.re;
+        o[1] = t.im;
+    }
+};
+__device__ __forceinline__ cd diagElem(const double* __restrict__ e, uint64_t k, int powMode, double xr,
+                                       double xi) {
+    double er = e[2 * k], ei = e[2 * k + 1];
+    if (powMode == 1) {
+        double lr = log(hypot(er, ei)), li = atan2(ei, er);
+        double pr = xr * lr - xi * li, pi = xr * li + xi * lr;
+        double mag = exp(pr);
+        return cd{mag * cos(pi), mag * sin(pi)};
+    }
+    if (powMode == 2) return cd{pow(er, xr), 0};
+    return cd{er, ei};
+}
+template <typename T>
+struct FExpecDiag {
+    const T* a;
+    const double* e;
+    int powMode;
+    double xr, xi;
+    __device__ void operator()(uint64_t n, double* o) const {
+        cd t = cscale(diagElem(e, n, powMode, xr, xi), norm2(ldd(a, n)));
+        o[0] = t.re;
+        o[1] = t.im;
+    }
+};
+template <typename T>
+struct FDensExpecDiag {
+    const T* a;
+    const double* e;
+    int powMode;
+    double xr, xi;
+    uint64_t diagStride;
+    __device__ void operator()(uint64_t n, double* o) const {
+        cd t = cmul(diagElem(e, n, powMode, xr, xi), ldd(a, n * diagStride));
+        o[0] = t.re;
+        o[1] = t.im;
+    }
+};
+
+// all outcomes of k qubits: register bins for k <= 4 (deterministic),
+// global atomics beyond
+template <typename T, int K>
+__global__ void __launch_bounds__(BLOCK) k_allOutcomesReg(const T* __restrict__ a, uint64_t numTerms,
+                                                         TargSpec qs, uint64_t diagStride,
+                                                         double* __restrict__ partials) {
+    constexpr int B = 1 << K;
+    KB acc[B];
+#pragma unroll
+    for (int b = 0; b < B; b++) acc[b] = KB{0, 0};
+    const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+    for (uint64_t n = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; n < numTerms; n += stride) {
+        double p = diagStride ? ldd(a, n * diagStride).re : norm2(ldd(a, n));
+        uint64_t v = valueOfBits(n, qs);
+#pragma unroll
+        for (int b = 0; b < B; b++) kadd(acc[b], (v == (uint64_t)b) ? p : 0.0);
+    }
+    blockMerge<B>(acc);
+    if (threadIdx.x == 0)
+        for (int b = 0; b < B; b++) {
+            partials[(blockIdx.x * B + b) * 2] = acc[b].s;
+            partials[(blockIdx.x * B + b) * 2 + 1] = acc[b].c;
+        }
+}
+
+template <int B>
+__global__ void __launch_bounds__(BLOCK) k_finalizeBins(const double* __restrict__ partials, int numParts,
+                                                       double* __restrict__ out) {
+    // one thread per bin, fixed order over workgroups
+    int b = threadIdx.x;
+    if (b >= B) return;
+    KB acc = {0, 0};
+    for (int p = 0; p < numParts; p++) kmerge(acc, KB{partials[(p * B + b) * 2], partials[(p * B + b) * 2 + 1]});
+    out[b] = acc.s + acc.c;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_allOutcomesAtomic(const T* __restrict__ a, uint64_t numTerms,
+                                                            TargSpec qs, uint64_t diagStride,
+                                                            double* __restrict__ bins) {
+    const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+    for (uint64_t n = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; n < numTerms; n += stride) {
+        double p = diagStride ? ldd(a, n * diagStride).re : norm2(ldd(a, n));
+        atomicAdd(&bins[valueOfBits(n, qs)], p);
+    }
+}
+
+__global__ void k_zero(double* __restrict__ p, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+template <typename T>
+int allOutcomes(double* out, const T* a, uint64_t numTerms, const int* qubits, int num, uint64_t diagStride) {
+    TargSpec ts;
+    ts.num = num;
+    for (int i = 0; i < num; i++) ts.targ[i] = qubits[i];
+    uint64_t nb = 1ull << num;
+    uint64_t want = (numTerms + (uint64_t)BLOCK * 4 - 1) / ((uint64_t)BLOCK * 4);
+    int blocks = (int)(want < 1 ? 1 : (want > 256 ? 256 : want));
+    if (num <= 4) {
+        double* scratch = devScratch((size_t)blocks * 16 * 2 + 64);
+        double* host = hostScratch(16);
+        if (!scratch || !host) return QA_ERR_OOM;
+        double* res = scratch + (size_t)blocks * 16 * 2;
+        switch (num) {
+#define QA_CASE(K)                                                                                   \
+    case K:                                                                                          \
+        k_allOutcomesReg<T, K><<<blocks, BLOCK, 0, stream()>>>(a, numTerms, ts, diagStride, scratch); \
+        k_finalizeBins<1 << K><<<1, BLOCK, 0, stream()>>>(scratch, blocks, res);                     \
+        break;
+            QA_CASE(1)
+            QA_CASE(2)
+            QA_CASE(3)
+            QA_CASE(4)
+#undef QA_CASE
+        }
+        QA_LAUNCH_CHECK();
+        QA_HIP(hipMemcpyAsync(host, res, nb * sizeof(double), hipMemcpyDeviceToHost, stream()));
+        QA_HIP(hipStreamSynchronize(stream()));
+        for (uint64_t b = 0; b < nb; b++) out[b] = host[b];
+        return QA_OK;
+    }
+    double* bins = devScratch((size_t)nb);
+    if (!bins) return QA_ERR_OOM;
+    k_zero<<<(unsigned)((nb + 255) / 256), 256, 0, stream()>>>(bins, nb);
+    k_allOutcomesAtomic<T><<<blocks * 4, BLOCK, 0, stream()>>>(a, numTerms, ts, diagStride, bins);
+    QA_LAUNCH_CHECK();
+    QA_HIP(hipMemcpyAsync(out, bins, nb * sizeof(double), hipMemcpyDeviceToHost, stream()));
+    QA_HIP(hipStreamSynchronize(stream()));
+    return QA_OK;
+}
+
+bool checkArgs(const void* a, int prec, int nq, const char* caller) {
+    if (!a || (prec != QA_FP32 && prec != QA_FP64) || nq < 1 || nq > 62) {
+        setError("%s: bad qureg arguments", caller);
+        return false;
+    }
+    return true;
+}
+
+bool outcomeSpec(InsertSpec& s, int nq, const int* qubits, const int* outcomes, int num, const char* caller) {
+    if (num < 0 || num > nq || num > MAXQ) {
+        setError("%s: bad number of qubits", caller);
+        return false;
+    }
+    int idx[MAXQ];
+    for (int i = 0; i < num; i++) idx[i] = i;
+    // insertion sort by qubit
+    for (int i = 1; i < num; i++)
+        for (int j = i; j > 0 && qubits[idx[j]] < qubits[idx[j - 1]]; j--) {
+            int t = idx[j];
+            idx[j] = idx[j - 1];
+            idx[j - 1] = t;
+        }
+    s.num = num;
+    s.valueMask = 0;
+    for (int i = 0; i < num; i++) {
+        int q = qubits[idx[i]];
+        if (q < 0 || q >= nq || (i && q == s.pos[i - 1])) {
+            setError("%s: bad qubit %d", caller, q);
+            return false;
+        }
+        s.pos[i] = q;
+        if (outcomes && outcomes[idx[i]]) s.valueMask |= 1ull << q;
+    }
+    return true;
+}
+
+#define QA_DISPATCH(prec, ...)   \
+    do {                         \
+        if ((prec) == QA_FP64) { \
+            typedef double T;    \
+            __VA_ARGS__;         \
+        } else {                 \
+            typedef float T;     \
+            __VA_ARGS__;         \
+        }                        \
+    } while (0)
+
+}  // namespace
+}  // namespace qa
+
+using namespace qa;
+
+extern "C" {
+
+int qa_statevec_calcTotalProb(const void* amps, int prec, int nq, double* out) {
+    QA_REQUIRE_INIT();
+    if (!checkArgs(amps, prec, nq, __func__)) return QA_ERR_ARG;
+    int rc;
+    QA_DISPATCH(prec, rc = runReduce<1>(FNormAll<T>{(const T*)amps}, 1ull << nq, out));
+    return rc;
+}
+
+int qa_densmatr_calcTotalProb(const void* amps, int prec, int N, double* out) {
+    QA_REQUIRE_INIT();
+    if (!checkArgs(amps, prec, 2 * N, __func__)) return QA_ERR_ARG;
+    InsertSpec none;
+    none.num = 0;
+    none.valueMask = 0;
+    int rc;
+    QA_DISPATCH(prec, rc = runReduce<1>(FDiagReal<T>{(const T*)amps, none, (1ull << N) + 1}, 1ull << N, out));
+    return rc;
+}
+
+int qa_statevec_calcProbOfMultiQubitOutcome(const void* amps, int prec, int nq, const int* qubits,
+                                            const int* outcomes, int num, double* out) {
+    QA_REQUIRE_INIT();
+    if (!checkArgs(amps, prec, nq, __func__)) return QA_ERR_ARG;
+    InsertSpec ins;
+    if (!outcomeSpec(ins, nq, qubits, outcomes, num, __func__)) return QA_ERR_ARG;
+    int rc;
+    QA_DISPATCH(prec, rc = runReduce<1>(FNormOutcome<T>{(const T*)amps, ins}, 1ull << (nq - num), out));
+    return rc;
+}
+
+int qa_densmatr_calcProbOfMultiQubitOutcome(const void* amps, int prec, int N, const int* qubits,
+                                            const int* outcomes, int num, double* out) {
+    QA_REQUIRE_INIT();
+    if (!checkArgs(amps, prec, 2 * N, __func__)) return QA_ERR_ARG;
+    InsertSpec ins;
+    if (!outcomeSpec(ins, N, qubits, outcomes, num, __func__)) return QA_ERR_ARG;
+    int rc;
+    QA_DISPATCH(prec, rc = runReduce<1>(FDiagReal<T>{(const T*)amps, ins, (1ull << N) + 1}, 1ull << (N - num), out));
+    return rc;
+}
+
+int qa_statevec_calcProbsOfAllMultiQubitOutcomes(double* out, const void* amps, int prec, int nq,
+                                                 const int* qubits, int num) {
+    QA_REQUIRE_INIT();
+    if (!checkArgs(amps, prec, nq, __func__)) return QA_ERR_ARG;
+    InsertSpec chk;
+    if (!outcomeSpec(chk, nq, qubits, nullptr, num, __func__)) return QA_ERR_ARG;
+    QA_ARG_CHECK(num >= 1 && num <= 24, "%s: %d qubits unsupported", __func__, num);
+    int rc;
+    QA_DISPATCH(prec, rc = allOutcomes<T>(out, (const T*)amps, 1ull << nq, qubits, num, 0));
+    return rc;
+}
+
+int qa_densmatr_calcProbsOfAllMultiQubitOutcomes(double* out, const void* amps, int prec, int N,
+                                                 const int* qubits, int num) {
+    QA_REQUIRE_INIT();
+    if (!checkArgs(amps, prec, 2 * N, __func__)) return QA_ERR_ARG;
+    InsertSpec chk;
+    if (!outcomeSpec(chk, N, qubits, nullptr, num, __func__)) return QA_ERR_ARG;
+    QA_ARG_CHECK(num >= 1 && num <= 24, "%s: %d qubits unsupported", __func__, num);
+    int rc;
+    QA_DISPATCH(prec, rc = allOutcomes<T>(out, (const T*)amps, 1ull << N, qubits, num, (1ull << N) + 1));
+    return rc;
+}
+
+int qa_statevec_calcInnerProduct(const void* a, const void* b, int prec, int nq, double* out) {
+    QA_REQUIRE_INIT();
+    if (!checkArgs(a, prec, nq, __func__) || !b) return QA_ERR_ARG;
+    int rc;
+    QA_DISPATCH(prec, rc = runReduce<2>(FInner<T>{(const T*)a, (const T*)b}, 1ull << nq, out));
+    return rc;
+}
+
+int qa_densmatr_calcHilbertSchmidtDistance(const void* a, const void* b, int prec, int N, double* out) {
+    QA_REQUIRE_INIT();
+    if (!checkArgs(a, prec, 2 * N, __func__) || !b) return QA_ERR_ARG;
+    int rc;
+    QA_DISPATCH(prec, rc = runReduce<1>(FHSDist<T>{(const T*)a, (const T*)b}, 1ull << (2 * N), out));
+    return rc;
+}
+
+int qa_densmatr_calcFidelityWithPureState(const void* rho, const void* psi, int prec, int N, int conj,
+                                          double* out) {
+    QA_REQUIRE_INIT();
+    if (!checkArgs(rho, prec, 2 * N, __func__) || !psi) return QA_ERR_ARG;
+    int rc;
+    QA_DISPATCH(prec, rc = runReduce<2>(FFidelity<T>{(const T*)rho, (const T*)psi, N, conj}, 1ull << (2 * N), out));
+    return rc;
+}
+
+int qa_statevec_calcExpecAnyTargZ(const void* amps, int prec, int nq, uint64_t mask, double* out) {
+    QA_REQUIRE_INIT();
+    if (!checkArgs(amps, prec, nq, __func__)) return QA_ERR_ARG;
+    int rc;
+    QA_DISPATCH(prec, rc = runReduce<1>(FExpecZ<T>{(const T*)amps, mask}, 1ull << nq, out));
+    return rc;
+}
+
+int qa_densmatr_calcExpecAnyTargZ(const void* amps, int prec, int N, uint64_t mask, double* out) {
+    QA_REQUIRE_INIT();
+    if (!checkArgs(amps, prec, 2 * N, __func__)) return QA_ERR_ARG;
+    int rc;
+    QA_DISPATCH(prec, rc = runReduce<2>(FDensExpecZ<T>{(const T*)amps, mask, (1ull << N) + 1}, 1ull << N, out));
+    return rc;
+}
+
+static void timesPowerOfI(double* v, int k) {
+    for (int i = 0; i < (k & 3); i++) {
+        double re = v[0], im = v[1];
+        v[0] = -im;
+        v[1] = re;
+    }
+}
+
+int qa_statevec_calcExpecPauliStr(const void* amps, int prec, int nq, uint64_t maskX, uint64_t maskY,
+                                  uint64_t maskZ, double* out) {
+    QA_REQUIRE_INIT();
+    if (!checkArgs(amps, prec, nq, __func__)) return QA_ERR_ARG;
+    int rc;
+    QA_DISPATCH(prec, rc = runReduce<2>(FExpecPauli<T>{(const T*)amps, maskX | maskY, maskY | maskZ}, 1ull << nq, out));
+    if (rc == QA_OK) timesPowerOfI(out, __builtin_popcountll(maskY));
+    return rc;
+}
+
+int qa_densmatr_calcExpecPauliStr(const void* amps, int prec, int N, uint64_t maskX, uint64_t maskY,
+                                  uint64_t maskZ, double* out) {
+    QA_REQUIRE_INIT();
+    if (!checkArgs(amps, prec, 2 * N, __func__)) return QA_ERR_ARG;
+    int rc;
+    QA_DISPATCH(prec, rc = runReduce<2>(FDensExpecPauli<T>{(const T*)amps, maskX | maskY, maskY | maskZ, N},
+                                        1ull << N, out));
+    if (rc == QA_OK) timesPowerOfI(out, __builtin_popcountll(maskY));
+    return rc;
+}
+
+int qa_statevec_calcExpecFullStateDiagMatr(const void* amps, int prec, int nq, const void* e, int powMode,
+                                           double xr, double xi, double* out) {
+    QA_REQUIRE_INIT();
+    if (!checkArgs(amps, prec, nq, __func__) || !e) return QA_ERR_ARG;
+    int rc;
+    QA_DISPATCH(prec, rc = runReduce<2>(FExpecDiag<T>{(const T*)amps, (const double*)e, powMode, xr, xi},
+                                        1ull << nq, out));
+    return rc;
+}
+
+int qa_densmatr_calcExpecFullStateDiagMatr(const void* amps, int prec, int N, const void* e, int powMode,
+                                           double xr, double xi, double* out) {
+    QA_REQUIRE_INIT();
+    if (!checkArgs(amps, prec, 2 * N, __func__) || !e) return QA_ERR_ARG;
+    int rc;
+    QA_DISPATCH(prec, rc = runReduce<2>(FDensExpecDiag<T>{(const T*)amps, (const double*)e, powMode, xr, xi,
+                                                          (1ull << N) + 1},
+                                        1ull << N, out));
+    return rc;
+}
+
+}  // extern "C"

@@ -182,7 +182,7 @@ qreal calcExpecPauliStr(Qureg q, PauliStr str) {
     QAH_API({
         validatePauliStrOnQureg(q, str);
         qcomp v = expecPauli(q, str);
-        if (validating() && std::abs(v.imag()) > REDUCTION_EPS_FACTOR * epsilon())
+        if (validating() && std::abs(v.imag()) > REDUCTION_EPS_FACTOR * epsilonFor(q))
             fail(q.isDensityMatrix ? "The calculated expectation value was not approximately real; the density matrix "
                                      "may be unnormalised or non-Hermitian."
                                    : "The calculated expectation value was not approximately real.");
@@ -210,7 +210,7 @@ qreal calcExpecPauliStrSum(Qureg q, PauliStrSum sum) {
                 if (std::abs(sum.coeffs[t].imag()) > epsilon()) fail("The PauliStrSum is not Hermitian.");
         qcomp v = 0;
         _wrap_calcExpecNonHermitianPauliStrSum(&v, q, sum);
-        if (validating() && std::abs(v.imag()) > REDUCTION_EPS_FACTOR * epsilon())
+        if (validating() && std::abs(v.imag()) > REDUCTION_EPS_FACTOR * epsilonFor(q))
             fail("The calculated expectation value was not approximately real.");
         return v.real();
     });

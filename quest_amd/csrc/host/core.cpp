@@ -58,6 +58,10 @@ void requireEnv() {
 }
 bool validating() { return g_validate; }
 double epsilon() { return g_eps; }
+double epsilonFor(const Qureg& q) {
+    const double FP32_EPS = 1e-5;
+    return (precOf(q) == QA_FP32 && g_eps < FP32_EPS) ? FP32_EPS : g_eps;
+}
 
 int precOf(const Qureg& q) {
     std::lock_guard<std::mutex> lk(g_regMutex);

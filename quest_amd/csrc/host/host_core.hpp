@@ -52,6 +52,10 @@ bool envInit();
 void requireEnv();
 bool validating();
 double epsilon();
+// numerical tolerance for results of one qureg: the reference's default for
+// its amplitude precision (precision.h:131-141: 1e-5 for FP32) unless the
+// user set a looser one
+double epsilonFor(const Qureg& q);
 
 // ---- qureg bookkeeping ------------------------------------------------------
 int precOf(const Qureg& q);  // QA_FP64 / QA_FP32

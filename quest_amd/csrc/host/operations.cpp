@@ -720,7 +720,7 @@ int applyQubitMeasurementAndGetProb(Qureg q, int t, qreal* probability) {
         validateTarget(q, t);
         double p0 = calcProbOfQubitOutcome(q, t, 0);
         double p1 = calcProbOfQubitOutcome(q, t, 1);
-        if (validating() && std::abs(p0 + p1 - 1) > epsilon())
+        if (validating() && std::abs(p0 + p1 - 1) > epsilonFor(q))
             fail("The state was unnormalised such that the probabilities of possible outcomes did not sum to one, and "
                  "ergo cannot be meaningfully sampled.");
         int outcome = randomQubitOutcome(p0);
@@ -739,7 +739,7 @@ qreal applyForcedQubitMeasurement(Qureg q, int t, int o) {
         validateTarget(q, t);
         validateOutcome(o);
         double p = calcProbOfQubitOutcome(q, t, o);
-        if (validating() && p <= epsilon()) fail("The forced measurement outcome has (near) zero probability.");
+        if (validating() && p <= epsilonFor(q)) fail("The forced measurement outcome has (near) zero probability.");
         projector(q, {t}, {o}, p);
         return p;
     });
@@ -754,7 +754,7 @@ qindex applyMultiQubitMeasurementAndGetProb(Qureg q, int* qs, int n, qreal* prob
         calcProbsOfAllMultiQubitOutcomes(probs.data(), q, qs, n);
         double tot = 0;
         for (double p : probs) tot += p;
-        if (validating() && std::abs(tot - 1) > epsilon())
+        if (validating() && std::abs(tot - 1) > epsilonFor(q))
             fail("The state was unnormalised such that the probabilities of possible outcomes did not sum to one, and "
                  "ergo cannot be meaningfully sampled.");
         qindex outcome = randomMultiQubitOutcome(probs);
@@ -774,7 +774,7 @@ qreal applyForcedMultiQubitMeasurement(Qureg q, int* qs, int* os, int n) {
         validateTargets(q, qs, n);
         for (int i = 0; i < n; i++) validateOutcome(os[i]);
         double p = calcProbOfMultiQubitOutcome(q, qs, os, n);
-        if (validating() && p <= epsilon()) fail("The forced measurement outcomes have (near) zero probability.");
+        if (validating() && p <= epsilonFor(q)) fail("The forced measurement outcomes have (near) zero probability.");
         projector(q, vec(qs, n), vec(os, n), p);
         return p;
     });

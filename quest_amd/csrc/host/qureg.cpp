@@ -357,7 +357,7 @@ qreal setQuregToRenormalized(Qureg qureg) {
         double prob = 0;
         if (qureg.isDensityMatrix) check(qa_densmatr_calcTotalProb(dev(qureg), p, qureg.numQubits, &prob));
         else check(qa_statevec_calcTotalProb(dev(qureg), p, qureg.numQubits, &prob));
-        if (std::abs(prob) <= epsilon()) fail("The Qureg has (near) zero probability and cannot be renormalised.");
+        if (std::abs(prob) <= epsilonFor(qureg)) fail("The Qureg has (near) zero probability and cannot be renormalised.");
         double f = qureg.isDensityMatrix ? 1 / prob : 1 / std::sqrt(prob);
         check(qa_statevec_setQuregToSuperposition(p, idxBits(qureg), f, 0, dev(qureg), 0, 0, dev(qureg), 0, 0,
                                                   dev(qureg)));

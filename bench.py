@@ -116,8 +116,51 @@ def cpu_baseline(circ, n: int) -> dict | None:
             "sample": desc + f", C restatement (OpenMP), {secs:.2f} s"}
 
 
+def kernel_sweep(Q, n: int, reps: int = 5) -> None:
+    """Per-target timing of the dominant kernel plus a plain device copy of
+    known size (the calibration dispatch for rocprofv3 --pmc FETCH_SIZE /
+    WRITE_SIZE runs): one JSON line per measurement on stdout."""
+    import ctypes as C
+    gpu = Q.gpu()
+
+    def ev():
+        e = C.c_void_p()
+        gpu.qa_eventCreate(C.byref(e))
+        return e
+
+    e0, e1 = ev(), ev()
+    ms = C.c_double()
+    q = Q.createQureg(n)
+    Q.initPlusState(q)
+    state = 16 * (1 << n)
+    # createCloneQureg copies the amplitudes with one device-to-device copy
+    # dispatch: reads `state` bytes, writes `state` bytes
+    for _ in range(2):
+        gpu.qa_eventRecord(e0)
+        c = Q.createCloneQureg(q)
+        gpu.qa_eventRecord(e1)
+        gpu.qa_eventElapsedMs(e0, e1, C.byref(ms))
+        Q.destroyQureg(c)
+        print(json.dumps({"kernel": "device copy (createCloneQureg)", "read_bytes": state, "write_bytes": state,
+                          "ms": ms.value, "GBps": 2 * state / (ms.value * 1e-3) / 1e9}), flush=True)
+    h = 2 ** -0.5
+    m = Q.compmatr1([[h, h], [h, -h]])
+    for t in range(n):
+        gpu.qa_eventRecord(e0)
+        for _ in range(reps):
+            Q.applyCompMatr1(q, t, m)
+        gpu.qa_eventRecord(e1)
+        gpu.qa_eventElapsedMs(e0, e1, C.byref(ms))
+        avg = ms.value / reps
+        print(json.dumps({"kernel": "k_oneTargDense<double>", "target": t, "bytes": 2 * state, "ms": avg,
+                          "GBps": 2 * state / (avg * 1e-3) / 1e9}), flush=True)
+    Q.destroyQureg(q)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
+    ap.add_argument("--mode", choices=["circuit", "sweep"], default="circuit",
+                    help="circuit: the headline workload; sweep: per-target dense-kernel timing + copy calibration")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
@@ -135,6 +178,10 @@ def main() -> int:
     from quest_amd import api as Q, circuits as qc
 
     n = args.qubits
+    if args.mode == "sweep":
+        Q.ensure_env()
+        kernel_sweep(Q, n)
+        return 0
     circ = qc.layered(n, args.depth, args.seed, ctrl3=True, qft=True, gadgets=64, gadget_range=(4, 12),
                       num_expec=16, probs=True)
     gates = qc.count_gates(circ)
@@ -205,6 +252,13 @@ def main() -> int:
     cpu = None if args.no_cpu_baseline else cpu_baseline(circ, n)
     Q.destroyQureg(q)
 
+    # HBM bytes per launch of the dominant kernel, from the calibrated
+    # rocprofv3 FETCH_SIZE / WRITE_SIZE runs committed under profiles/
+    traffic = None
+    tfile = ROOT / "profiles" / f"k_oneTargDense_{n}q_fp64_traffic.json"
+    if tfile.exists():
+        traffic = json.loads(tfile.read_text()).get("traffic_bytes_per_dispatch")
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -235,7 +289,7 @@ def main() -> int:
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
-            "traffic": None,
+            "traffic": traffic,
             "kernel": "k_oneTargDense<double> (applyCompMatr1)",
             "bytes_per_launch": dense_bytes,
             "avg_launch_ms": dense_avg_s * 1e3,

@@ -85,32 +85,35 @@ struct M2 {
 // ---- kernels ------------------------------------------------------------
 
 // [a0; a1] <- M [a0; a1] on pairs (i0, i0 | 1<<targ) with ctrls satisfied
-template <typename T>
+// IT items per thread; NT = streaming (non-temporal) accesses.  The host
+// picks <2, 1> when every touched index bit is >= 3 and <1, 0> otherwise
+// (see ldx/stx).
+template <typename T, int IT, int NT>
 __global__ void __launch_bounds__(BLOCK) k_oneTargDense(T* __restrict__ amps, uint64_t numWork,
                                                        InsertSpec ins, int targ, M1<T> m) {
-    const uint64_t base = (uint64_t)blockIdx.x * (BLOCK * ITEMS) + threadIdx.x;
+    const uint64_t base = (uint64_t)blockIdx.x * (BLOCK * IT) + threadIdx.x;
     const uint64_t tbit = 1ull << targ;
-    uint64_t i0[ITEMS];
-    ct<T> a0[ITEMS], a1[ITEMS];
+    uint64_t i0[IT];
+    ct<T> a0[IT], a1[IT];
 #pragma unroll
-    for (int j = 0; j < ITEMS; j++) {
+    for (int j = 0; j < IT; j++) {
         uint64_t n = base + (uint64_t)j * BLOCK;
         if (n < numWork) {
             i0[j] = insertBits(n, ins);
-            a0[j] = ldnt(amps, i0[j]);
-            a1[j] = ldnt(amps, i0[j] | tbit);
+            a0[j] = ldx<NT>(amps, i0[j]);
+            a1[j] = ldx<NT>(amps, i0[j] | tbit);
         }
     }
 #pragma unroll
-    for (int j = 0; j < ITEMS; j++) {
+    for (int j = 0; j < IT; j++) {
         uint64_t n = base + (uint64_t)j * BLOCK;
         if (n < numWork) {
             ct<T> b0 = cmulT(m.m00, a0[j]);
             b0 = cfmaT(m.m01, a1[j], b0);
             ct<T> b1 = cmulT(m.m10, a0[j]);
             b1 = cfmaT(m.m11, a1[j], b1);
-            stnt(amps, i0[j], b0);
-            stnt(amps, i0[j] | tbit, b1);
+            stx<NT>(amps, i0[j], b0);
+            stx<NT>(amps, i0[j] | tbit, b1);
         }
     }
 }
@@ -201,27 +204,28 @@ __global__ void __launch_bounds__(BLOCK) k_manyTargDense(T* __restrict__ amps, u
     }
 }
 
-// a <- a * d[bit targ] over control-satisfied amplitudes
-template <typename T>
+// a <- a * d[bit targ] over control-satisfied amplitudes (targ < 0: the
+// target bit was folded into the insertion spec and every amplitude takes d1)
+template <typename T, int IT, int NT>
 __global__ void __launch_bounds__(BLOCK) k_oneTargDiag(T* __restrict__ amps, uint64_t numWork,
                                                       InsertSpec ins, int targ, ct<T> d0, ct<T> d1) {
-    const uint64_t base = (uint64_t)blockIdx.x * (BLOCK * ITEMS) + threadIdx.x;
-    uint64_t idx[ITEMS];
-    ct<T> a[ITEMS];
+    const uint64_t base = (uint64_t)blockIdx.x * (BLOCK * IT) + threadIdx.x;
+    uint64_t idx[IT];
+    ct<T> a[IT];
 #pragma unroll
-    for (int j = 0; j < ITEMS; j++) {
+    for (int j = 0; j < IT; j++) {
         uint64_t n = base + (uint64_t)j * BLOCK;
         if (n < numWork) {
             idx[j] = insertBits(n, ins);
-            a[j] = ldnt(amps, idx[j]);
+            a[j] = ldx<NT>(amps, idx[j]);
         }
     }
 #pragma unroll
-    for (int j = 0; j < ITEMS; j++) {
+    for (int j = 0; j < IT; j++) {
         uint64_t n = base + (uint64_t)j * BLOCK;
         if (n < numWork) {
-            ct<T> d = ((idx[j] >> targ) & 1ull) ? d1 : d0;
-            stnt(amps, idx[j], cmulT(a[j], d));
+            ct<T> d = (targ < 0 || ((idx[j] >> targ) & 1ull)) ? d1 : d0;
+            stx<NT>(amps, idx[j], cmulT(a[j], d));
         }
     }
 }
@@ -299,27 +303,27 @@ __global__ void __launch_bounds__(BLOCK) k_allTargDiag(T* __restrict__ amps, uin
 }
 
 // swap amplitudes where (targ1, targ2) = (1, 0) and (0, 1)
-template <typename T>
+template <typename T, int IT, int NT>
 __global__ void __launch_bounds__(BLOCK) k_swap(T* __restrict__ amps, uint64_t numWork,
                                                InsertSpec ins, uint64_t flip) {
-    const uint64_t base = (uint64_t)blockIdx.x * (BLOCK * ITEMS) + threadIdx.x;
-    uint64_t ia[ITEMS];
-    ct<T> a[ITEMS], b[ITEMS];
+    const uint64_t base = (uint64_t)blockIdx.x * (BLOCK * IT) + threadIdx.x;
+    uint64_t ia[IT];
+    ct<T> a[IT], b[IT];
 #pragma unroll
-    for (int j = 0; j < ITEMS; j++) {
+    for (int j = 0; j < IT; j++) {
         uint64_t n = base + (uint64_t)j * BLOCK;
         if (n < numWork) {
             ia[j] = insertBits(n, ins);
-            a[j] = ldnt(amps, ia[j]);
-            b[j] = ldnt(amps, ia[j] ^ flip);
+            a[j] = ldx<NT>(amps, ia[j]);
+            b[j] = ldx<NT>(amps, ia[j] ^ flip);
         }
     }
 #pragma unroll
-    for (int j = 0; j < ITEMS; j++) {
+    for (int j = 0; j < IT; j++) {
         uint64_t n = base + (uint64_t)j * BLOCK;
         if (n < numWork) {
-            stnt(amps, ia[j], b[j]);
-            stnt(amps, ia[j] ^ flip, a[j]);
+            stx<NT>(amps, ia[j], b[j]);
+            stx<NT>(amps, ia[j] ^ flip, a[j]);
         }
     }
 }
@@ -327,24 +331,24 @@ __global__ void __launch_bounds__(BLOCK) k_swap(T* __restrict__ amps, uint64_t n
 // Pauli tensor / gadget on pairs (iA, iB = iA ^ maskXY): iA enumerates the
 // control-satisfied indices whose highest X/Y target bit is 0
 // (cpu_subroutines.cpp applyPauliUponAmpPair)
-template <typename T>
+template <typename T, int IT, int NT>
 __global__ void __launch_bounds__(BLOCK) k_pauli(T* __restrict__ amps, uint64_t numWork,
                                                 InsertSpec ins, uint64_t maskXY, uint64_t maskYZ,
                                                 ct<T> ampFac, ct<T> pairFac) {
-    const uint64_t base = (uint64_t)blockIdx.x * (BLOCK * ITEMS) + threadIdx.x;
-    uint64_t ia[ITEMS];
-    ct<T> a[ITEMS], b[ITEMS];
+    const uint64_t base = (uint64_t)blockIdx.x * (BLOCK * IT) + threadIdx.x;
+    uint64_t ia[IT];
+    ct<T> a[IT], b[IT];
 #pragma unroll
-    for (int j = 0; j < ITEMS; j++) {
+    for (int j = 0; j < IT; j++) {
         uint64_t n = base + (uint64_t)j * BLOCK;
         if (n < numWork) {
             ia[j] = insertBits(n, ins);
-            a[j] = ldnt(amps, ia[j]);
-            b[j] = ldnt(amps, ia[j] ^ maskXY);
+            a[j] = ldx<NT>(amps, ia[j]);
+            b[j] = ldx<NT>(amps, ia[j] ^ maskXY);
         }
     }
 #pragma unroll
-    for (int j = 0; j < ITEMS; j++) {
+    for (int j = 0; j < IT; j++) {
         uint64_t n = base + (uint64_t)j * BLOCK;
         if (n < numWork) {
             uint64_t ib = ia[j] ^ maskXY;
@@ -354,33 +358,53 @@ __global__ void __launch_bounds__(BLOCK) k_pauli(T* __restrict__ amps, uint64_t 
             ct<T> pA = {pairFac.re * sA, pairFac.im * sA};
             ct<T> na = cfmaT(pB, b[j], cmulT(ampFac, a[j]));
             ct<T> nb = cfmaT(pA, a[j], cmulT(ampFac, b[j]));
-            stnt(amps, ia[j], na);
-            stnt(amps, ib, nb);
+            stx<NT>(amps, ia[j], na);
+            stx<NT>(amps, ib, nb);
         }
     }
 }
 
 // a_i <- a_i * fac[parity(i & targMask)]
-template <typename T>
+template <typename T, int IT, int NT>
 __global__ void __launch_bounds__(BLOCK) k_phaseGadget(T* __restrict__ amps, uint64_t numWork,
                                                       InsertSpec ins, uint64_t targMask, ct<T> f0,
                                                       ct<T> f1) {
-    const uint64_t base = (uint64_t)blockIdx.x * (BLOCK * ITEMS) + threadIdx.x;
-    uint64_t idx[ITEMS];
-    ct<T> a[ITEMS];
+    const uint64_t base = (uint64_t)blockIdx.x * (BLOCK * IT) + threadIdx.x;
+    uint64_t idx[IT];
+    ct<T> a[IT];
 #pragma unroll
-    for (int j = 0; j < ITEMS; j++) {
+    for (int j = 0; j < IT; j++) {
         uint64_t n = base + (uint64_t)j * BLOCK;
         if (n < numWork) {
             idx[j] = insertBits(n, ins);
-            a[j] = ldnt(amps, idx[j]);
+            a[j] = ldx<NT>(amps, idx[j]);
         }
     }
 #pragma unroll
-    for (int j = 0; j < ITEMS; j++) {
+    for (int j = 0; j < IT; j++) {
         uint64_t n = base + (uint64_t)j * BLOCK;
-        if (n < numWork) stnt(amps, idx[j], cmulT(a[j], parity64(idx[j] & targMask) ? f1 : f0));
+        if (n < numWork) stx<NT>(amps, idx[j], cmulT(a[j], parity64(idx[j] & targMask) ? f1 : f0));
     }
+}
+
+// launch a streaming kernel in its best (items, hint) form given the lowest
+// index bit any of its accesses varies or is fixed at
+#define QA_LAUNCH_STREAMING(kernel, T, lowBit, work, ...)                                                 \
+    do {                                                                                                  \
+        if ((lowBit) >= STREAM_LOW_BIT)                                                                   \
+            kernel<T, 2, 1><<<gridFor((work), 2), BLOCK, 0, stream()>>>(__VA_ARGS__);                     \
+        else                                                                                              \
+            kernel<T, 1, 0><<<gridFor((work), 1), BLOCK, 0, stream()>>>(__VA_ARGS__);                     \
+    } while (0)
+
+int lowestBit(const InsertSpec& ins, uint64_t extraMask) {
+    int low = 64;
+    if (ins.num > 0) low = ins.pos[0];
+    if (extraMask) {
+        int e = __builtin_ctzll(extraMask);
+        if (e < low) low = e;
+    }
+    return low;
 }
 
 template <typename T>
@@ -399,7 +423,8 @@ __global__ void __launch_bounds__(BLOCK) k_superpose(T* __restrict__ out, const 
 
 template <typename T>
 void launchOneTargDense(void* amps, uint64_t work, const InsertSpec& ins, int targ, const double* m) {
-    k_oneTargDense<T><<<gridFor(work), BLOCK, 0, stream()>>>((T*)amps, work, ins, targ, toM1<T>(m));
+    // ins already holds the target, so its lowest position covers every access
+    QA_LAUNCH_STREAMING(k_oneTargDense, T, lowestBit(ins, 0), work, (T*)amps, work, ins, targ, toM1<T>(m));
 }
 
 template <typename T>
@@ -458,7 +483,7 @@ int qa_statevec_anyCtrlSwap(void* amps, int prec, int numQubits, const int* ctrl
     if (!buildInsert(ins, numQubits, ctrls, ctrlStates, numCtrls, ex, vals, 2, __func__)) return QA_ERR_ARG;
     uint64_t work = 1ull << (numQubits - numCtrls - 2);
     uint64_t flip = (1ull << targ1) | (1ull << targ2);
-    QA_DISPATCH(prec, (k_swap<T><<<gridFor(work), BLOCK, 0, stream()>>>((T*)amps, work, ins, flip)));
+    QA_DISPATCH(prec, QA_LAUNCH_STREAMING(k_swap, T, lowestBit(ins, 0), work, (T*)amps, work, ins, flip));
     QA_LAUNCH_CHECK();
     return QA_OK;
 }
@@ -532,12 +557,20 @@ int qa_statevec_anyCtrlOneTargDiagMatr(void* amps, int prec, int numQubits, cons
     QA_ARG_CHECK(targ >= 0 && targ < numQubits, "%s: target %d out of range", __func__, targ);
     for (int i = 0; i < numCtrls; i++)
         QA_ARG_CHECK(ctrls[i] != targ, "%s: control %d equals the target", __func__, targ);
+    // a phase-only diagonal diag(1, d1) (phase shifts, CZ, S, T) leaves every
+    // amplitude with a 0 target bit exactly unchanged: fold the target into the
+    // insertion spec as one more "control" on 1 and touch only the others
+    bool phaseOnly = (e[0] == 1.0 && e[1] == 0.0);
     InsertSpec ins;
-    if (!buildInsert(ins, numQubits, ctrls, ctrlStates, numCtrls, nullptr, nullptr, 0, __func__))
+    int one = 1;
+    if (!buildInsert(ins, numQubits, ctrls, ctrlStates, numCtrls, phaseOnly ? &targ : nullptr,
+                     phaseOnly ? &one : nullptr, phaseOnly ? 1 : 0, __func__))
         return QA_ERR_ARG;
-    uint64_t work = 1ull << (numQubits - numCtrls);
-    QA_DISPATCH(prec, (k_oneTargDiag<T><<<gridFor(work), BLOCK, 0, stream()>>>(
-                          (T*)amps, work, ins, targ, ct<T>{(T)e[0], (T)e[1]}, ct<T>{(T)e[2], (T)e[3]})));
+    uint64_t work = 1ull << (numQubits - numCtrls - (phaseOnly ? 1 : 0));
+    int kt = phaseOnly ? -1 : targ;
+    int low = lowestBit(ins, 0);  // runs of consecutive amplitudes break only at inserted bits
+    QA_DISPATCH(prec, QA_LAUNCH_STREAMING(k_oneTargDiag, T, low, work, (T*)amps, work, ins, kt,
+                                          (ct<T>{(T)e[0], (T)e[1]}), (ct<T>{(T)e[2], (T)e[3]})));
     QA_LAUNCH_CHECK();
     return QA_OK;
 }
@@ -633,9 +666,8 @@ int qa_statevec_anyCtrlPauliTensorOrGadget(void* amps, int prec, int numQubits, 
     int ny = __builtin_popcountll(maskY) & 3;
     for (int k = 0; k < ny; k++) pf = cmul(pf, cd{0, 1});
     uint64_t work = 1ull << (numQubits - numCtrls - 1);
-    QA_DISPATCH(prec, (k_pauli<T><<<gridFor(work), BLOCK, 0, stream()>>>(
-                          (T*)amps, work, ins, maskXY, maskY | maskZ, ct<T>{(T)ampFacRe, (T)ampFacIm},
-                          toT<T>(pf))));
+    QA_DISPATCH(prec, QA_LAUNCH_STREAMING(k_pauli, T, lowestBit(ins, maskXY), work, (T*)amps, work, ins, maskXY,
+                                          maskY | maskZ, (ct<T>{(T)ampFacRe, (T)ampFacIm}), toT<T>(pf)));
     QA_LAUNCH_CHECK();
     return QA_OK;
 }
@@ -649,8 +681,8 @@ int qa_statevec_anyCtrlAnyTargZOrPhaseGadget(void* amps, int prec, int numQubits
     if (!buildInsert(ins, numQubits, ctrls, ctrlStates, numCtrls, nullptr, nullptr, 0, __func__))
         return QA_ERR_ARG;
     uint64_t work = 1ull << (numQubits - numCtrls);
-    QA_DISPATCH(prec, (k_phaseGadget<T><<<gridFor(work), BLOCK, 0, stream()>>>(
-                          (T*)amps, work, ins, targMask, ct<T>{(T)fac0Re, (T)fac0Im}, ct<T>{(T)fac1Re, (T)fac1Im})));
+    QA_DISPATCH(prec, QA_LAUNCH_STREAMING(k_phaseGadget, T, lowestBit(ins, 0), work, (T*)amps, work, ins, targMask,
+                                          (ct<T>{(T)fac0Re, (T)fac0Im}), (ct<T>{(T)fac1Re, (T)fac1Im})));
     QA_LAUNCH_CHECK();
     return QA_OK;
 }

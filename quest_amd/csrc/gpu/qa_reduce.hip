@@ -189,6 +189,25 @@ struct FExpecPauli {  // sign(j & maskYZ) conj(a_n) a_j, j = n ^ maskXY
         o[1] = s * t.im;
     }
 };
+// the same sum visited one pair at a time: n with the top X/Y bit clear and
+// j = n ^ maskXY contribute sign(j) conj(a_n) a_j + sign(n) conj(a_j) a_n,
+// so every amplitude is read once (the plain form reads each twice)
+template <typename T>
+struct FExpecPauliPair {
+    const T* a;
+    InsertSpec topBit;
+    uint64_t maskXY, maskYZ;
+    __device__ void operator()(uint64_t w, double* o) const {
+        uint64_t n = insertBits(w, topBit);
+        uint64_t j = n ^ maskXY;
+        cd x = ldd(a, n), y = ldd(a, j);
+        cd t1 = cmul(cconj(x), y), t2 = cmul(cconj(y), x);
+        double s1 = parity64(j & maskYZ) ? -1.0 : 1.0;
+        double s2 = parity64(n & maskYZ) ? -1.0 : 1.0;
+        o[0] = s1 * t1.re + s2 * t2.re;
+        o[1] = s1 * t1.im + s2 * t2.im;
+    }
+};
 template <typename T>
 struct FDensExpecPauli {  // column n, row i = n ^ maskXY: sign(i & maskYZ) rho_{i,n}
     const T* a;
@@ -544,7 +563,16 @@ int qa_statevec_calcExpecPauliStr(const void* amps, int prec, int nq, uint64_t m
     QA_REQUIRE_INIT();
     if (!checkArgs(amps, prec, nq, __func__)) return QA_ERR_ARG;
     int rc;
-    QA_DISPATCH(prec, rc = runReduce<2>(FExpecPauli<T>{(const T*)amps, maskX | maskY, maskY | maskZ}, 1ull << nq, out));
+    uint64_t mxy = maskX | maskY, myz = maskY | maskZ;
+    if (mxy) {
+        InsertSpec top;
+        top.num = 1;
+        top.pos[0] = 63 - __builtin_clzll(mxy);
+        top.valueMask = 0;
+        QA_DISPATCH(prec, rc = runReduce<2>(FExpecPauliPair<T>{(const T*)amps, top, mxy, myz}, 1ull << (nq - 1), out));
+    } else {
+        QA_DISPATCH(prec, rc = runReduce<2>(FExpecPauli<T>{(const T*)amps, mxy, myz}, 1ull << nq, out));
+    }
     if (rc == QA_OK) timesPowerOfI(out, __builtin_popcountll(maskY));
     return rc;
 }

@@ -24,8 +24,17 @@ int getQuregPrecision(Qureg qureg);
 /* Bytes of device memory a qureg of this shape would take. */
 qindex getQuregDeviceBytes(int numQubits, int isDensMatr, int precision);
 
-/* Device-side handle of the library stream (a hipStream_t). */
+/* Device-side handle of the library stream (a hipStream_t).  Gates may be
+   queued for fusion: syncQuESTEnv() (or any amplitude read) applies them. */
 void* getQuESTStream(void);
+
+/* Gate fusion (no counterpart in the reference, which launches one kernel
+   per gate).  On by default with 2^12-amplitude tiles whose low 4 qubits are
+   always local and at most 512 queued gates; the environment variable
+   QUEST_AMD_FUSION=0 turns it off at initQuESTEnv, "tile,low,queue" sets
+   the shape.  Fused and unfused runs give identical amplitudes. */
+void setQuESTFusion(int enabled, int tileQubits, int lowQubits, int maxQueue);
+void getQuESTFusion(int* enabled, int* tileQubits, int* lowQubits, int* maxQueue);
 
 #ifdef __cplusplus
 }

@@ -100,6 +100,26 @@ int qa_eventElapsedMs(void* start, void* stop, double* elapsedMs);
 int qa_eventDestroy(void* event);
 
 /* ------------------------------------------------------------------ */
+/* gate fusion (no counterpart in QuEST, which runs one kernel per    */
+/* gate: gpu_subroutines.cpp:197-887)                                 */
+/* ------------------------------------------------------------------ */
+/* When enabled, the one/two-qubit dense, one-qubit diagonal, swap, Pauli
+   tensor/gadget, phase gadget, one-qubit dephasing and one-qubit
+   depolarising entry points queue their gate instead of launching it.
+   Queued gates are applied, in order and with arithmetic identical to the
+   unfused kernels, by LDS-tiled passes of 2^tileQubits amplitudes
+   (2^(tileQubits+1) in FP32) whose low `lowQubits` qubits are always in the
+   tile; every other entry point (reductions, copies, getters, setters,
+   sync, event records) first applies what is queued.  Queues are per state
+   and are flushed after `maxQueue` gates.  Changing the configuration
+   flushes every queue.  tileQubits in [2,13], lowQubits in
+   [0, tileQubits-2], maxQueue in [1,4096]; fusion is off after qa_init. */
+int qa_setFusion(int enabled, int tileQubits, int lowQubits, int maxQueue);
+int qa_getFusion(int* enabled, int* tileQubits, int* lowQubits, int* maxQueue);
+/* applies the gates queued on `amps` (NULL: on every state) */
+int qa_flush(void* amps);
+
+/* ------------------------------------------------------------------ */
 /* statevector gates             (gpu_subroutines.hpp:33-70)          */
 /* ------------------------------------------------------------------ */
 

@@ -252,6 +252,8 @@ _SIGS = {
     "_wrap_calcInnerProduct": (None, [Qureg, Qureg, C.POINTER(qcomp)]),
     "calcPartialTrace": (Qureg, [Qureg, IntP, cint]),
     "getQuESTStream": (ptr, []),
+    "setQuESTFusion": (None, [cint, cint, cint, cint]),
+    "getQuESTFusion": (None, [IntP, IntP, IntP, IntP]),
 }
 
 _raw = {}
@@ -420,6 +422,17 @@ def _gpu_last_error() -> str:
 def gpu():
     """The raw kernel-layer library (include/quest_amd_gpu.h)."""
     return _gpu
+
+
+def get_fusion() -> tuple:
+    """(enabled, tileQubits, lowQubits, maxQueue) of the gate-fusion queue."""
+    v = [C.c_int(0) for _ in range(4)]
+    getQuESTFusion(*[C.byref(x) for x in v])
+    return tuple(x.value for x in v)
+
+
+def set_fusion(enabled: bool, tile_qubits: int = 12, low_qubits: int = 4, max_queue: int = 512) -> None:
+    setQuESTFusion(int(bool(enabled)), int(tile_qubits), int(low_qubits), int(max_queue))
 
 
 def ensure_env() -> None:

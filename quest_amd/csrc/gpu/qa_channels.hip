@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "qa_common.hpp"
+#include "qa_fused.hpp"
 
 namespace qa {
 namespace {
@@ -92,8 +93,8 @@ __global__ void __launch_bounds__(BLOCK) k_depol1(T* __restrict__ amps, uint64_t
     for (int j = 0; j < 2; j++) {
         uint64_t n = base + (uint64_t)j * BLOCK;
         if (n < numWork) {
-            ct<T> n00 = {c1 * a00[j].re + c2 * a11[j].re, c1 * a00[j].im + c2 * a11[j].im};
-            ct<T> n11 = {c1 * a11[j].re + c2 * a00[j].re, c1 * a11[j].im + c2 * a00[j].im};
+            ct<T> n00 = {fmaT(c1, a00[j].re, c2 * a11[j].re), fmaT(c1, a00[j].im, c2 * a11[j].im)};
+            ct<T> n11 = {fmaT(c1, a11[j].re, c2 * a00[j].re), fmaT(c1, a11[j].im, c2 * a00[j].im)};
             stnt(amps, i00[j], n00);
             stnt(amps, i00[j] | ketBit, ct<T>{c3 * a01[j].re, c3 * a01[j].im});
             stnt(amps, i00[j] | braBit, ct<T>{c3 * a10[j].re, c3 * a10[j].im});
@@ -247,6 +248,14 @@ int qa_densmatr_oneQubitDephasing(void* amps, int prec, int N, int q, double pro
     InsertSpec ins = zerosAt({q, q + N});
     uint64_t work = 1ull << (2 * N - 2);
     double fac = 1 - 2 * prob;  // util_getOneQubitDephasingFactor
+    {
+        fuse::GOp op;
+        op.kind = fuse::F_DEPH1;
+        op.t1 = q;
+        op.t2 = q + N;
+        op.f[0] = fac;
+        QA_TRY_FUSE(amps, prec, 2 * N, op);
+    }
     QA_DISPATCH(prec, (k_dephase1<T><<<gridFor(work), BLOCK, 0, stream()>>>(
                           (T*)amps, work, ins, 1ull << q, 1ull << (q + N), (T)fac)));
     QA_LAUNCH_CHECK();
@@ -255,6 +264,7 @@ int qa_densmatr_oneQubitDephasing(void* amps, int prec, int N, int q, double pro
 
 int qa_densmatr_twoQubitDephasing(void* amps, int prec, int N, int q1, int q2, double prob) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     if (!checkDM(amps, prec, N, __func__)) return QA_ERR_ARG;
     QA_ARG_CHECK(q1 >= 0 && q1 < N && q2 >= 0 && q2 < N && q1 != q2, "%s: bad qubits", __func__);
     uint64_t num = 1ull << (2 * N);
@@ -273,6 +283,16 @@ int qa_densmatr_oneQubitDepolarising(void* amps, int prec, int N, int q, double 
     uint64_t work = 1ull << (2 * N - 2);
     // util_getOneQubitDepolarisingFactors (utilities.cpp:932-947)
     double c1 = 1 - (2 * prob / 3), c2 = 2 * prob / 3, c3 = 1 - (4 * prob / 3);
+    {
+        fuse::GOp op;
+        op.kind = fuse::F_DEPOL1;
+        op.t1 = q;
+        op.t2 = q + N;
+        op.f[0] = c1;
+        op.f[1] = c2;
+        op.f[2] = c3;
+        QA_TRY_FUSE(amps, prec, 2 * N, op);
+    }
     QA_DISPATCH(prec, (k_depol1<T><<<gridFor(work, 2), BLOCK, 0, stream()>>>(
                           (T*)amps, work, ins, 1ull << q, 1ull << (q + N), (T)c1, (T)c2, (T)c3)));
     QA_LAUNCH_CHECK();
@@ -281,6 +301,7 @@ int qa_densmatr_oneQubitDepolarising(void* amps, int prec, int N, int q, double 
 
 int qa_densmatr_twoQubitDepolarising(void* amps, int prec, int N, int q1, int q2, double prob) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     if (!checkDM(amps, prec, N, __func__)) return QA_ERR_ARG;
     QA_ARG_CHECK(q1 >= 0 && q1 < N && q2 >= 0 && q2 < N && q1 != q2, "%s: bad qubits", __func__);
     InsertSpec ins = zerosAt({q1, q2, q1 + N, q2 + N});
@@ -299,6 +320,7 @@ int qa_densmatr_twoQubitDepolarising(void* amps, int prec, int N, int q1, int q2
 int qa_densmatr_oneQubitPauliChannel(void* amps, int prec, int N, int q, double pI, double pX, double pY,
                                      double pZ) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     if (!checkDM(amps, prec, N, __func__)) return QA_ERR_ARG;
     QA_ARG_CHECK(q >= 0 && q < N, "%s: qubit out of range", __func__);
     InsertSpec ins = zerosAt({q, q + N});
@@ -313,6 +335,7 @@ int qa_densmatr_oneQubitPauliChannel(void* amps, int prec, int N, int q, double 
 
 int qa_densmatr_oneQubitDamping(void* amps, int prec, int N, int q, double prob) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     if (!checkDM(amps, prec, N, __func__)) return QA_ERR_ARG;
     QA_ARG_CHECK(q >= 0 && q < N, "%s: qubit out of range", __func__);
     InsertSpec ins = zerosAt({q, q + N});
@@ -326,6 +349,7 @@ int qa_densmatr_oneQubitDamping(void* amps, int prec, int N, int q, double prob)
 
 int qa_densmatr_mixQureg(double outProb, void* out, double inProb, const void* in, int prec, int N) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     if (!checkDM(out, prec, N, __func__) || !in) return QA_ERR_ARG;
     uint64_t num = 1ull << (2 * N);
     QA_DISPATCH(prec, (k_mixDM<T><<<gridFor(num, 1), BLOCK, 0, stream()>>>((T*)out, (const T*)in, num,
@@ -336,6 +360,7 @@ int qa_densmatr_mixQureg(double outProb, void* out, double inProb, const void* i
 
 int qa_densmatr_mixQuregWithStatevec(double outProb, void* out, double inProb, const void* psi, int prec, int N) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     if (!checkDM(out, prec, N, __func__) || !psi) return QA_ERR_ARG;
     uint64_t num = 1ull << (2 * N);
     QA_DISPATCH(prec, (k_mixSV<T><<<gridFor(num, 1), BLOCK, 0, stream()>>>((T*)out, (const T*)psi, num, N,
@@ -346,6 +371,7 @@ int qa_densmatr_mixQuregWithStatevec(double outProb, void* out, double inProb, c
 
 int qa_densmatr_initPureState(void* rho, const void* psi, int prec, int N) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     if (!checkDM(rho, prec, N, __func__) || !psi) return QA_ERR_ARG;
     uint64_t num = 1ull << (2 * N);
     QA_DISPATCH(prec, (k_pureDM<T><<<gridFor(num, 1), BLOCK, 0, stream()>>>((T*)rho, (const T*)psi, num, N)));
@@ -355,6 +381,7 @@ int qa_densmatr_initPureState(void* rho, const void* psi, int prec, int N) {
 
 int qa_densmatr_partialTrace(const void* in, int prec, int inN, void* out, const int* targs, int numTargs) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     if (!checkDM(in, prec, inN, __func__) || !out) return QA_ERR_ARG;
     QA_ARG_CHECK(numTargs >= 1 && numTargs < inN, "%s: bad number of traced qubits", __func__);
     std::vector<int> all;

@@ -43,9 +43,15 @@ template <typename T>
 struct ct {
     T re, im;
 };
+// Fused multiply-adds are written out so that every kernel that applies a
+// gate rounds it the same way (the compiler's own contraction choices may
+// differ between kernels, the fused-tile and single-gate paths must not).
+__device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 template <typename T>
 __device__ __forceinline__ ct<T> cmulT(ct<T> a, ct<T> b) {
-    return ct<T>{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+    T ii = a.im * b.im, ir = a.im * b.re;
+    return ct<T>{fmaT(a.re, b.re, -ii), fmaT(a.re, b.im, ir)};
 }
 template <typename T>
 __device__ __forceinline__ ct<T> caddT(ct<T> a, ct<T> b) {
@@ -53,8 +59,9 @@ __device__ __forceinline__ ct<T> caddT(ct<T> a, ct<T> b) {
 }
 template <typename T>
 __device__ __forceinline__ ct<T> cfmaT(ct<T> m, ct<T> a, ct<T> acc) {
-    // acc + m*a
-    return ct<T>{acc.re + m.re * a.re - m.im * a.im, acc.im + m.re * a.im + m.im * a.re};
+    // acc + m*a, accumulated term by term
+    T re = fmaT(m.re, a.re, acc.re), im = fmaT(m.re, a.im, acc.im);
+    return ct<T>{fmaT(-m.im, a.im, re), fmaT(m.im, a.re, im)};
 }
 
 template <typename T>

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "qa_common.hpp"
+#include "qa_fused.hpp"
 
 namespace qa {
 namespace {
@@ -481,6 +482,14 @@ int qa_statevec_anyCtrlSwap(void* amps, int prec, int numQubits, const int* ctrl
     int ex[2] = {targ1, targ2}, vals[2] = {1, 0};
     InsertSpec ins;
     if (!buildInsert(ins, numQubits, ctrls, ctrlStates, numCtrls, ex, vals, 2, __func__)) return QA_ERR_ARG;
+    {
+        fuse::GOp op;
+        op.kind = fuse::F_SWAP;
+        op.t1 = targ1;
+        op.t2 = targ2;
+        fuse::ctrlMasks(ctrls, ctrlStates, numCtrls, op.cMask, op.cVal);
+        QA_TRY_FUSE(amps, prec, numQubits, op);
+    }
     uint64_t work = 1ull << (numQubits - numCtrls - 2);
     uint64_t flip = (1ull << targ1) | (1ull << targ2);
     QA_DISPATCH(prec, QA_LAUNCH_STREAMING(k_swap, T, lowestBit(ins, 0), work, (T*)amps, work, ins, flip));
@@ -495,6 +504,14 @@ int qa_statevec_anyCtrlOneTargDenseMatr(void* amps, int prec, int numQubits, con
     InsertSpec ins;
     if (!buildInsert(ins, numQubits, ctrls, ctrlStates, numCtrls, &targ, nullptr, 1, __func__))
         return QA_ERR_ARG;
+    {
+        fuse::GOp op;
+        op.kind = fuse::F_DENSE1;
+        op.t1 = targ;
+        fuse::ctrlMasks(ctrls, ctrlStates, numCtrls, op.cMask, op.cVal);
+        for (int i = 0; i < 8; i++) op.f[i] = matr[i];
+        QA_TRY_FUSE(amps, prec, numQubits, op);
+    }
     uint64_t work = 1ull << (numQubits - numCtrls - 1);
     QA_DISPATCH(prec, launchOneTargDense<T>(amps, work, ins, targ, matr));
     QA_LAUNCH_CHECK();
@@ -509,6 +526,15 @@ int qa_statevec_anyCtrlTwoTargDenseMatr(void* amps, int prec, int numQubits, con
     int ex[2] = {targ1, targ2};
     InsertSpec ins;
     if (!buildInsert(ins, numQubits, ctrls, ctrlStates, numCtrls, ex, nullptr, 2, __func__)) return QA_ERR_ARG;
+    {
+        fuse::GOp op;
+        op.kind = fuse::F_DENSE2;
+        op.t1 = targ1;
+        op.t2 = targ2;
+        fuse::ctrlMasks(ctrls, ctrlStates, numCtrls, op.cMask, op.cVal);
+        for (int i = 0; i < 32; i++) op.f[i] = matr[i];
+        QA_TRY_FUSE(amps, prec, numQubits, op);
+    }
     uint64_t work = 1ull << (numQubits - numCtrls - 2);
     QA_DISPATCH(prec, launchTwoTargDense<T>(amps, work, ins, targ1, targ2, matr));
     QA_LAUNCH_CHECK();
@@ -539,6 +565,7 @@ int qa_statevec_anyCtrlAnyTargDenseMatr(void* amps, int prec, int numQubits, con
     InsertSpec ins;
     if (!buildInsert(ins, numQubits, ctrls, ctrlStates, numCtrls, targs, nullptr, numTargs, __func__))
         return QA_ERR_ARG;
+    QA_FLUSH_ALL();
     TargSpec ts;
     ts.num = numTargs;
     for (int i = 0; i < numTargs; i++) ts.targ[i] = targs[i];
@@ -566,6 +593,18 @@ int qa_statevec_anyCtrlOneTargDiagMatr(void* amps, int prec, int numQubits, cons
     if (!buildInsert(ins, numQubits, ctrls, ctrlStates, numCtrls, phaseOnly ? &targ : nullptr,
                      phaseOnly ? &one : nullptr, phaseOnly ? 1 : 0, __func__))
         return QA_ERR_ARG;
+    {
+        fuse::GOp op;
+        op.kind = fuse::F_DIAG1;
+        op.t1 = targ;
+        fuse::ctrlMasks(ctrls, ctrlStates, numCtrls, op.cMask, op.cVal);
+        if (phaseOnly) {
+            op.cMask |= 1ull << targ;
+            op.cVal |= 1ull << targ;
+        }
+        for (int i = 0; i < 4; i++) op.f[i] = e[i];
+        QA_TRY_FUSE(amps, prec, numQubits, op);
+    }
     uint64_t work = 1ull << (numQubits - numCtrls - (phaseOnly ? 1 : 0));
     int kt = phaseOnly ? -1 : targ;
     int low = lowestBit(ins, 0);  // runs of consecutive amplitudes break only at inserted bits
@@ -585,6 +624,7 @@ int qa_statevec_anyCtrlTwoTargDiagMatr(void* amps, int prec, int numQubits, cons
     InsertSpec ins;
     if (!buildInsert(ins, numQubits, ctrls, ctrlStates, numCtrls, nullptr, nullptr, 0, __func__))
         return QA_ERR_ARG;
+    QA_FLUSH_ALL();
     uint64_t work = 1ull << (numQubits - numCtrls);
     QA_DISPATCH(prec, (k_twoTargDiag<T><<<gridFor(work), BLOCK, 0, stream()>>>(
                           (T*)amps, work, ins, targ1, targ2, ct<T>{(T)e[0], (T)e[1]}, ct<T>{(T)e[2], (T)e[3]},
@@ -609,6 +649,7 @@ int qa_statevec_anyCtrlAnyTargDiagMatr(void* amps, int prec, int numQubits, cons
     InsertSpec ins;
     if (!buildInsert(ins, numQubits, ctrls, ctrlStates, numCtrls, nullptr, nullptr, 0, __func__))
         return QA_ERR_ARG;
+    QA_FLUSH_ALL();
     TargSpec ts;
     ts.num = numTargs;
     for (int i = 0; i < numTargs; i++) ts.targ[i] = targs[i];
@@ -623,6 +664,7 @@ int qa_statevec_allTargDiagMatr(void* amps, int prec, int numQubits, const void*
                                 double expRe, double expIm) {
     QA_REQUIRE_INIT();
     if (!checkQureg(amps, prec, numQubits, __func__)) return QA_ERR_ARG;
+    QA_FLUSH_ALL();
     uint64_t n = 1ull << numQubits;
     QA_DISPATCH(prec, (k_allTargDiag<T><<<gridFor(n, 1), BLOCK, 0, stream()>>>(
                           (T*)amps, n, (const double*)devElems, hasPower, expRe, expIm, -1, 0)));
@@ -634,6 +676,7 @@ int qa_densmatr_allTargDiagMatr(void* amps, int prec, int numQubitsN, const void
                                 double expRe, double expIm, int multiplyOnly) {
     QA_REQUIRE_INIT();
     if (!checkQureg(amps, prec, 2 * numQubitsN, __func__)) return QA_ERR_ARG;
+    QA_FLUSH_ALL();
     uint64_t n = 1ull << (2 * numQubitsN);
     QA_DISPATCH(prec, (k_allTargDiag<T><<<gridFor(n, 1), BLOCK, 0, stream()>>>(
                           (T*)amps, n, (const double*)devElems, hasPower, expRe, expIm, numQubitsN,
@@ -665,6 +708,18 @@ int qa_statevec_anyCtrlPauliTensorOrGadget(void* amps, int prec, int numQubits, 
     cd pf = {pairFacRe, pairFacIm};
     int ny = __builtin_popcountll(maskY) & 3;
     for (int k = 0; k < ny; k++) pf = cmul(pf, cd{0, 1});
+    {
+        fuse::GOp op;
+        op.kind = fuse::F_PAULI;
+        op.m1 = maskXY;
+        op.m2 = maskY | maskZ;
+        fuse::ctrlMasks(ctrls, ctrlStates, numCtrls, op.cMask, op.cVal);
+        op.f[0] = ampFacRe;
+        op.f[1] = ampFacIm;
+        op.f[2] = pf.re;
+        op.f[3] = pf.im;
+        QA_TRY_FUSE(amps, prec, numQubits, op);
+    }
     uint64_t work = 1ull << (numQubits - numCtrls - 1);
     QA_DISPATCH(prec, QA_LAUNCH_STREAMING(k_pauli, T, lowestBit(ins, maskXY), work, (T*)amps, work, ins, maskXY,
                                           maskY | maskZ, (ct<T>{(T)ampFacRe, (T)ampFacIm}), toT<T>(pf)));
@@ -680,6 +735,17 @@ int qa_statevec_anyCtrlAnyTargZOrPhaseGadget(void* amps, int prec, int numQubits
     InsertSpec ins;
     if (!buildInsert(ins, numQubits, ctrls, ctrlStates, numCtrls, nullptr, nullptr, 0, __func__))
         return QA_ERR_ARG;
+    {
+        fuse::GOp op;
+        op.kind = fuse::F_PARITY;
+        op.m1 = targMask;
+        fuse::ctrlMasks(ctrls, ctrlStates, numCtrls, op.cMask, op.cVal);
+        op.f[0] = fac0Re;
+        op.f[1] = fac0Im;
+        op.f[2] = fac1Re;
+        op.f[3] = fac1Im;
+        QA_TRY_FUSE(amps, prec, numQubits, op);
+    }
     uint64_t work = 1ull << (numQubits - numCtrls);
     QA_DISPATCH(prec, QA_LAUNCH_STREAMING(k_phaseGadget, T, lowestBit(ins, 0), work, (T*)amps, work, ins, targMask,
                                           (ct<T>{(T)fac0Re, (T)fac0Im}), (ct<T>{(T)fac1Re, (T)fac1Im})));
@@ -693,6 +759,7 @@ int qa_statevec_setQuregToSuperposition(int prec, int numQubits, double foRe, do
     QA_REQUIRE_INIT();
     if (!checkQureg(out, prec, numQubits, __func__)) return QA_ERR_ARG;
     QA_ARG_CHECK(in1 && in2, "%s: null input", __func__);
+    QA_FLUSH_ALL();
     uint64_t n = 1ull << numQubits;
     QA_DISPATCH(prec, (k_superpose<T><<<gridFor(n, 1), BLOCK, 0, stream()>>>(
                           (T*)out, (const T*)in1, (const T*)in2, n, ct<T>{(T)foRe, (T)foIm}, ct<T>{(T)f1Re, (T)f1Im},

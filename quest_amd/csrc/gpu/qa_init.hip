@@ -4,6 +4,7 @@
 #include <cmath>
 
 #include "qa_common.hpp"
+#include "qa_fused.hpp"
 
 namespace qa {
 namespace {
@@ -119,6 +120,7 @@ extern "C" {
 
 int qa_statevec_initUniformState(void* amps, int prec, int nq, double re, double im) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     if (!ok(amps, prec, nq, __func__)) return QA_ERR_ARG;
     uint64_t n = 1ull << nq;
     QA_DISPATCH(prec, (k_fill<T><<<gridFor(n), BLOCK, 0, stream()>>>((T*)amps, n, (T)re, (T)im)));
@@ -128,6 +130,7 @@ int qa_statevec_initUniformState(void* amps, int prec, int nq, double re, double
 
 int qa_statevec_initDebugState(void* amps, int prec, int nq) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     if (!ok(amps, prec, nq, __func__)) return QA_ERR_ARG;
     uint64_t n = 1ull << nq;
     QA_DISPATCH(prec, (k_debug<T><<<gridFor(n, 1), BLOCK, 0, stream()>>>((T*)amps, n)));
@@ -137,6 +140,7 @@ int qa_statevec_initDebugState(void* amps, int prec, int nq) {
 
 int qa_statevec_initUnnormalisedUniformlyRandomPureStateAmps(void* amps, int prec, int nq, uint64_t seed) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     if (!ok(amps, prec, nq, __func__)) return QA_ERR_ARG;
     uint64_t n = 1ull << nq;
     QA_DISPATCH(prec, (k_random<T><<<gridFor(n, 1), BLOCK, 0, stream()>>>((T*)amps, n, seed)));
@@ -147,6 +151,7 @@ int qa_statevec_initUnnormalisedUniformlyRandomPureStateAmps(void* amps, int pre
 int qa_statevec_multiQubitProjector(void* amps, int prec, int nq, const int* qubits, const int* outcomes, int num,
                                     double prob) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     if (!ok(amps, prec, nq, __func__)) return QA_ERR_ARG;
     uint64_t qm, om;
     if (!masks(nq, qubits, outcomes, num, qm, om, __func__)) return QA_ERR_ARG;
@@ -160,6 +165,7 @@ int qa_statevec_multiQubitProjector(void* amps, int prec, int nq, const int* qub
 int qa_densmatr_multiQubitProjector(void* amps, int prec, int N, const int* qubits, const int* outcomes, int num,
                                     double prob) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     if (!ok(amps, prec, 2 * N, __func__)) return QA_ERR_ARG;
     uint64_t qm, om;
     if (!masks(N, qubits, outcomes, num, qm, om, __func__)) return QA_ERR_ARG;

@@ -13,6 +13,7 @@
 // Amplitudes are widened to FP64 before squaring, so FP32 states reduce in
 // FP64 too.
 #include "qa_common.hpp"
+#include "qa_fused.hpp"
 
 namespace qa {
 namespace {
@@ -446,6 +447,7 @@ extern "C" {
 
 int qa_statevec_calcTotalProb(const void* amps, int prec, int nq, double* out) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     if (!checkArgs(amps, prec, nq, __func__)) return QA_ERR_ARG;
     int rc;
     QA_DISPATCH(prec, rc = runReduce<1>(FNormAll<T>{(const T*)amps}, 1ull << nq, out));
@@ -454,6 +456,7 @@ int qa_statevec_calcTotalProb(const void* amps, int prec, int nq, double* out) {
 
 int qa_densmatr_calcTotalProb(const void* amps, int prec, int N, double* out) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     if (!checkArgs(amps, prec, 2 * N, __func__)) return QA_ERR_ARG;
     InsertSpec none;
     none.num = 0;
@@ -466,6 +469,7 @@ int qa_densmatr_calcTotalProb(const void* amps, int prec, int N, double* out) {
 int qa_statevec_calcProbOfMultiQubitOutcome(const void* amps, int prec, int nq, const int* qubits,
                                             const int* outcomes, int num, double* out) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     if (!checkArgs(amps, prec, nq, __func__)) return QA_ERR_ARG;
     InsertSpec ins;
     if (!outcomeSpec(ins, nq, qubits, outcomes, num, __func__)) return QA_ERR_ARG;
@@ -477,6 +481,7 @@ int qa_statevec_calcProbOfMultiQubitOutcome(const void* amps, int prec, int nq, 
 int qa_densmatr_calcProbOfMultiQubitOutcome(const void* amps, int prec, int N, const int* qubits,
                                             const int* outcomes, int num, double* out) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     if (!checkArgs(amps, prec, 2 * N, __func__)) return QA_ERR_ARG;
     InsertSpec ins;
     if (!outcomeSpec(ins, N, qubits, outcomes, num, __func__)) return QA_ERR_ARG;
@@ -488,6 +493,7 @@ int qa_densmatr_calcProbOfMultiQubitOutcome(const void* amps, int prec, int N, c
 int qa_statevec_calcProbsOfAllMultiQubitOutcomes(double* out, const void* amps, int prec, int nq,
                                                  const int* qubits, int num) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     if (!checkArgs(amps, prec, nq, __func__)) return QA_ERR_ARG;
     InsertSpec chk;
     if (!outcomeSpec(chk, nq, qubits, nullptr, num, __func__)) return QA_ERR_ARG;
@@ -500,6 +506,7 @@ int qa_statevec_calcProbsOfAllMultiQubitOutcomes(double* out, const void* amps, 
 int qa_densmatr_calcProbsOfAllMultiQubitOutcomes(double* out, const void* amps, int prec, int N,
                                                  const int* qubits, int num) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     if (!checkArgs(amps, prec, 2 * N, __func__)) return QA_ERR_ARG;
     InsertSpec chk;
     if (!outcomeSpec(chk, N, qubits, nullptr, num, __func__)) return QA_ERR_ARG;
@@ -511,6 +518,7 @@ int qa_densmatr_calcProbsOfAllMultiQubitOutcomes(double* out, const void* amps, 
 
 int qa_statevec_calcInnerProduct(const void* a, const void* b, int prec, int nq, double* out) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     if (!checkArgs(a, prec, nq, __func__) || !b) return QA_ERR_ARG;
     int rc;
     QA_DISPATCH(prec, rc = runReduce<2>(FInner<T>{(const T*)a, (const T*)b}, 1ull << nq, out));
@@ -519,6 +527,7 @@ int qa_statevec_calcInnerProduct(const void* a, const void* b, int prec, int nq,
 
 int qa_densmatr_calcHilbertSchmidtDistance(const void* a, const void* b, int prec, int N, double* out) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     if (!checkArgs(a, prec, 2 * N, __func__) || !b) return QA_ERR_ARG;
     int rc;
     QA_DISPATCH(prec, rc = runReduce<1>(FHSDist<T>{(const T*)a, (const T*)b}, 1ull << (2 * N), out));
@@ -528,6 +537,7 @@ int qa_densmatr_calcHilbertSchmidtDistance(const void* a, const void* b, int pre
 int qa_densmatr_calcFidelityWithPureState(const void* rho, const void* psi, int prec, int N, int conj,
                                           double* out) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     if (!checkArgs(rho, prec, 2 * N, __func__) || !psi) return QA_ERR_ARG;
     int rc;
     QA_DISPATCH(prec, rc = runReduce<2>(FFidelity<T>{(const T*)rho, (const T*)psi, N, conj}, 1ull << (2 * N), out));
@@ -536,6 +546,7 @@ int qa_densmatr_calcFidelityWithPureState(const void* rho, const void* psi, int 
 
 int qa_statevec_calcExpecAnyTargZ(const void* amps, int prec, int nq, uint64_t mask, double* out) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     if (!checkArgs(amps, prec, nq, __func__)) return QA_ERR_ARG;
     int rc;
     QA_DISPATCH(prec, rc = runReduce<1>(FExpecZ<T>{(const T*)amps, mask}, 1ull << nq, out));
@@ -544,6 +555,7 @@ int qa_statevec_calcExpecAnyTargZ(const void* amps, int prec, int nq, uint64_t m
 
 int qa_densmatr_calcExpecAnyTargZ(const void* amps, int prec, int N, uint64_t mask, double* out) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     if (!checkArgs(amps, prec, 2 * N, __func__)) return QA_ERR_ARG;
     int rc;
     QA_DISPATCH(prec, rc = runReduce<2>(FDensExpecZ<T>{(const T*)amps, mask, (1ull << N) + 1}, 1ull << N, out));
@@ -561,6 +573,7 @@ static void timesPowerOfI(double* v, int k) {
 int qa_statevec_calcExpecPauliStr(const void* amps, int prec, int nq, uint64_t maskX, uint64_t maskY,
                                   uint64_t maskZ, double* out) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     if (!checkArgs(amps, prec, nq, __func__)) return QA_ERR_ARG;
     int rc;
     uint64_t mxy = maskX | maskY, myz = maskY | maskZ;
@@ -580,6 +593,7 @@ int qa_statevec_calcExpecPauliStr(const void* amps, int prec, int nq, uint64_t m
 int qa_densmatr_calcExpecPauliStr(const void* amps, int prec, int N, uint64_t maskX, uint64_t maskY,
                                   uint64_t maskZ, double* out) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     if (!checkArgs(amps, prec, 2 * N, __func__)) return QA_ERR_ARG;
     int rc;
     QA_DISPATCH(prec, rc = runReduce<2>(FDensExpecPauli<T>{(const T*)amps, maskX | maskY, maskY | maskZ, N},
@@ -591,6 +605,7 @@ int qa_densmatr_calcExpecPauliStr(const void* amps, int prec, int N, uint64_t ma
 int qa_statevec_calcExpecFullStateDiagMatr(const void* amps, int prec, int nq, const void* e, int powMode,
                                            double xr, double xi, double* out) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     if (!checkArgs(amps, prec, nq, __func__) || !e) return QA_ERR_ARG;
     int rc;
     QA_DISPATCH(prec, rc = runReduce<2>(FExpecDiag<T>{(const T*)amps, (const double*)e, powMode, xr, xi},
@@ -601,6 +616,7 @@ int qa_statevec_calcExpecFullStateDiagMatr(const void* amps, int prec, int nq, c
 int qa_densmatr_calcExpecFullStateDiagMatr(const void* amps, int prec, int N, const void* e, int powMode,
                                            double xr, double xi, double* out) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     if (!checkArgs(amps, prec, 2 * N, __func__) || !e) return QA_ERR_ARG;
     int rc;
     QA_DISPATCH(prec, rc = runReduce<2>(FDensExpecDiag<T>{(const T*)amps, (const double*)e, powMode, xr, xi,

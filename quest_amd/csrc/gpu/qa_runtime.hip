@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "qa_common.hpp"
+#include "qa_fused.hpp"
 
 namespace {
 
@@ -127,7 +128,9 @@ int qa_init(int device) {
 
 int qa_finalize(void) {
     if (!g_init) return QA_OK;
+    QA_FLUSH_ALL();
     QA_HIP(hipStreamSynchronize(g_stream));
+    qa::fuse::shutdown();
     if (g_devScratch) QA_HIP(hipFree(g_devScratch));
     if (g_hostScratch) QA_HIP(hipHostFree(g_hostScratch));
     if (g_devStage) QA_HIP(hipFree(g_devStage));
@@ -179,6 +182,7 @@ int qa_malloc(void** ptr, size_t numBytes) {
 int qa_free(void* ptr) {
     QA_REQUIRE_INIT();
     if (!ptr) return QA_OK;
+    qa::fuse::drop(ptr);  // gates still queued on a freed state are discarded
     QA_HIP(hipStreamSynchronize(g_stream));
     QA_HIP(hipFree(ptr));
     return QA_OK;
@@ -186,6 +190,7 @@ int qa_free(void* ptr) {
 
 int qa_memcpyHtoD(void* dst, const void* src, size_t n) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     QA_HIP(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, g_stream));
     QA_HIP(hipStreamSynchronize(g_stream));
     return QA_OK;
@@ -193,6 +198,7 @@ int qa_memcpyHtoD(void* dst, const void* src, size_t n) {
 
 int qa_memcpyDtoH(void* dst, const void* src, size_t n) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     QA_HIP(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, g_stream));
     QA_HIP(hipStreamSynchronize(g_stream));
     return QA_OK;
@@ -200,12 +206,14 @@ int qa_memcpyDtoH(void* dst, const void* src, size_t n) {
 
 int qa_memcpyDtoD(void* dst, const void* src, size_t n) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     QA_HIP(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToDevice, g_stream));
     return QA_OK;
 }
 
 int qa_sync(void) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     QA_HIP(hipStreamSynchronize(g_stream));
     return QA_OK;
 }
@@ -214,6 +222,7 @@ void* qa_getStream(void) { return (void*)g_stream; }
 
 int qa_getAmps(double* out, const void* amps, int prec, int64_t start, int64_t num) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     QA_ARG_CHECK(start >= 0 && num >= 0, "qa_getAmps: negative range");
     if (num == 0) return QA_OK;
     if (prec == QA_FP64) {
@@ -239,6 +248,7 @@ int qa_getAmps(double* out, const void* amps, int prec, int64_t start, int64_t n
 
 int qa_setAmps(void* amps, int prec, int64_t start, const double* in, int64_t num) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     QA_ARG_CHECK(start >= 0 && num >= 0, "qa_setAmps: negative range");
     if (num == 0) return QA_OK;
     if (prec == QA_FP64) {
@@ -265,6 +275,7 @@ int qa_setAmps(void* amps, int prec, int64_t start, const double* in, int64_t nu
 int qa_getDensityAmps(double* out, const void* amps, int prec, int numQubits, int64_t r0, int64_t c0,
                       int64_t nr, int64_t nc) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     int64_t dim = 1ll << numQubits;
     QA_ARG_CHECK(r0 >= 0 && c0 >= 0 && nr >= 0 && nc >= 0 && r0 + nr <= dim && c0 + nc <= dim,
                  "qa_getDensityAmps: block out of range");
@@ -308,6 +319,7 @@ int qa_getDensityAmps(double* out, const void* amps, int prec, int numQubits, in
 
 int qa_copyAmps(void* dst, int dstPrec, const void* src, int srcPrec, int64_t num) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     if (num <= 0) return QA_OK;
     uint64_t reals = 2ull * (uint64_t)num;
     if (dstPrec == srcPrec) {
@@ -327,6 +339,7 @@ int qa_copyAmps(void* dst, int dstPrec, const void* src, int srcPrec, int64_t nu
 
 int qa_timerStart(void) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     QA_HIP(hipEventRecord(g_ev0, g_stream));
     return QA_OK;
 }
@@ -341,6 +354,7 @@ int qa_eventCreate(void** ev) {
 
 int qa_eventRecord(void* ev) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     QA_HIP(hipEventRecord((hipEvent_t)ev, g_stream));
     return QA_OK;
 }
@@ -362,6 +376,7 @@ int qa_eventDestroy(void* ev) {
 
 int qa_timerStop(double* ms) {
     QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
     QA_HIP(hipEventRecord(g_ev1, g_stream));
     QA_HIP(hipEventSynchronize(g_ev1));
     float f = 0;

@@ -299,6 +299,20 @@ void initCustomQuESTEnv(int useDistrib, int useGpuAccel, int useMultithread) {
         const char* dev = std::getenv("QUEST_AMD_DEVICE");
         int rc = qa_init(dev ? std::atoi(dev) : 0);
         if (rc != QA_OK) fail(std::string("Could not bind a GPU: ") + qa_lastError());
+        // gate fusion is on unless QUEST_AMD_FUSION=0; "tile,low,queue" sets its shape
+        const char* fz = std::getenv("QUEST_AMD_FUSION");
+        int on = 1, tile = 12, low = 4, queue = 512;
+        if (fz && *fz) {
+            int t = 0, l = 0, q = 0;
+            if (std::sscanf(fz, "%d,%d,%d", &t, &l, &q) == 3) {
+                tile = t;
+                low = l;
+                queue = q;
+            } else {
+                on = std::atoi(fz) != 0;
+            }
+        }
+        check(qa_setFusion(on, tile, low, queue));
         std::random_device rd;
         setSeedsInternal({rd(), rd(), rd(), rd()});
         g_envInit = true;
@@ -402,6 +416,20 @@ void setValidationEpsilon(qreal eps) {
 qreal getValidationEpsilon(void) { return g_eps; }
 qindex getGpuCacheSize(void) { return 0; }
 void clearGpuCache(void) {}
+
+void setQuESTFusion(int enabled, int tileQubits, int lowQubits, int maxQueue) {
+    QAH_API({
+        requireEnv();
+        check(qa_setFusion(enabled, tileQubits, lowQubits, maxQueue));
+    });
+}
+
+void getQuESTFusion(int* enabled, int* tileQubits, int* lowQubits, int* maxQueue) {
+    QAH_API({
+        requireEnv();
+        check(qa_getFusion(enabled, tileQubits, lowQubits, maxQueue));
+    });
+}
 
 void* getQuESTStream(void) { return qa_getStream(); }
 

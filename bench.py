@@ -7,16 +7,21 @@ applyTwoQubitPhaseShift(U(0,2pi)) on 15 random disjoint pairs, one 3-control
 applyMultiControlledCompMatr1], then applyFullQuantumFourierTransform (30 H,
 435 phase shifts, 15 swaps), then 64 applyPauliGadget on 4-12 random qubits,
 then calcProbOfQubitOutcome on all 30 qubits and 16 calcExpecPauliStr.
-2184 gates and 46 reductions per step, every one a separate call of the
-quest.h API, i.e. one pass over the state each (no fusion).
+2384 gates and 46 reductions per step, every one a separate call of the
+quest.h API.  The library queues the gates and applies each maximal run
+whose dense targets fit one 2^12-amplitude tile (low 4 qubits always in the
+tile) as one LDS-tiled pass over the state (k_tilePass): one HBM read and
+write of the 16 GiB state per pass instead of per gate.
 
-Output: one JSON line (see DESIGN.md for the field definitions):
+Output: one JSON line:
   value      gates/s over the timed steps (HIP events on the library stream
              bracket them; wall clock agrees after syncQuESTEnv)
-  roofline   the dominant kernel (1-target dense, applyCompMatr1): algorithmic
-             bytes per launch 2 * 16 B * 2^30 = 32 GiB, divided by its average
-             duration measured with HIP events around each layer's block of 30
-             launches inside the timed steps
+  roofline   the dominant kernel (the fused tile pass k_tilePass<double>):
+             algorithmic bytes per launch 2 * 16 B * 2^30 = 32 GiB (every pass
+             reads and writes the whole state once), divided by its average
+             duration, measured with HIP events around every pass launch
+             inside the timed steps
+  unfused    the same step once more with fusion off (one kernel per gate)
   cpu_baseline  QuEST's own FP64 OpenMP CPU build (oracle/_ref) on a bounded
              sample of the same workload at the same size, on this host
 """
@@ -44,23 +49,14 @@ def log(msg: str) -> None:
 
 
 def build_step(Q, qc, circ, q):
-    """Prebuilt (callable, args) list for one step, plus the indices of the
-    calls that open/close each layer's block of applyCompMatr1 launches."""
+    """Prebuilt (callable, args) list for one step: each entry is one quest.h
+    call, with its matrices and Pauli strings built beforehand."""
     calls = []
-    u1_blocks = []
-    i = 0
-    ops = circ.ops
-    while i < len(ops):
-        op = ops[i]
-        if op[0] == "U1":
-            start = len(calls)
-            while i < len(ops) and ops[i][0] == "U1":
-                calls.append((Q.applyCompMatr1, (q, ops[i][1], Q.compmatr1(ops[i][2]))))
-                i += 1
-            u1_blocks.append((start, len(calls)))
-            continue
+    for op in circ.ops:
         k = op[0]
-        if k == "PS2":
+        if k == "U1":
+            calls.append((Q.applyCompMatr1, (q, op[1], Q.compmatr1(op[2]))))
+        elif k == "PS2":
             calls.append((Q.applyTwoQubitPhaseShift, (q, op[1], op[2], op[3])))
         elif k == "CU1":
             calls.append((Q.applyMultiControlledCompMatr1, (q, Q.ints(op[1]), len(op[1]), op[2], Q.compmatr1(op[3]))))
@@ -74,19 +70,28 @@ def build_step(Q, qc, circ, q):
             calls.append((Q.calcExpecPauliStr, (q, Q.pauli_str(op[1], op[2]))))
         else:
             raise ValueError(k)
-        i += 1
-    return calls, u1_blocks
+    return calls
 
 
-def cpu_baseline(circ, n: int) -> dict | None:
-    """Bounded sample of the same workload on the host cores: the first 8
-    applyCompMatr1 and 4 applyTwoQubitPhaseShift of layer 0 at n qubits,
-    through QuEST's own CPU build if present, else through the oracle port."""
+def cpu_baseline(circ, n: int, layers: int = 3) -> dict | None:
+    """Bounded sample of the same workload on the host cores: the first
+    `layers` layers of the step (each n applyCompMatr1, n/2
+    applyTwoQubitPhaseShift and one 3-control applyMultiControlledCompMatr1)
+    at n qubits, through QuEST's own CPU build if present, else through the
+    oracle's C restatement."""
     from quest_amd import circuits as qc
-    sample_ops = [op for op in circ.ops if op[0] == "U1"][:8] + [op for op in circ.ops if op[0] == "PS2"][:4]
+    sample_ops, seen = [], 0
+    for op in circ.ops:
+        if op[0] not in ("U1", "PS2", "CU1"):
+            break
+        sample_ops.append(op)
+        if op[0] == "CU1":
+            seen += 1
+            if seen == layers:
+                break
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     driver = ROOT / "oracle" / "_ref" / "circuit_driver"
-    desc = f"{len(sample_ops)} gates (8 applyCompMatr1 + 4 applyTwoQubitPhaseShift) on a {n}-qubit FP64 statevector"
+    desc = f"first {layers} layers ({len(sample_ops)} gates) of the step on a {n}-qubit FP64 statevector"
     if driver.exists():
         lines = [f"QUREG {n} 0", "INITPLUS", "TIMER_START"] + [qc.op_to_line(op) for op in sample_ops]
         lines.append("TIMER_STOP sample")
@@ -96,7 +101,7 @@ def cpu_baseline(circ, n: int) -> dict | None:
         env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="spread", OMP_PLACES="cores",
                    OMP_DYNAMIC="false")
         try:
-            out = subprocess.run([str(driver), path], capture_output=True, text=True, env=env, timeout=600)
+            out = subprocess.run([str(driver), path], capture_output=True, text=True, env=env, timeout=900)
         finally:
             os.unlink(path)
         if out.returncode == 0:
@@ -157,22 +162,48 @@ def kernel_sweep(Q, n: int, reps: int = 5) -> None:
     Q.destroyQureg(q)
 
 
+def timed_steps(Q, gpu, calls, steps: int) -> float:
+    """Milliseconds between HIP events recorded on the library stream before
+    the first and after the last call of `steps` steps (the stop record
+    applies any queued gates first)."""
+    import ctypes as C
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    gpu.qa_eventCreate(C.byref(e0))
+    gpu.qa_eventCreate(C.byref(e1))
+    Q.syncQuESTEnv()
+    gpu.qa_eventRecord(e0)
+    for s in range(steps):
+        for f, a in calls:
+            f(*a)
+        log(f"step {s} enqueued")
+    gpu.qa_eventRecord(e1)
+    Q.syncQuESTEnv()
+    ms = C.c_double()
+    gpu.qa_eventElapsedMs(e0, e1, C.byref(ms))
+    gpu.qa_eventDestroy(e0)
+    gpu.qa_eventDestroy(e1)
+    return ms.value
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--mode", choices=["circuit", "sweep"], default="circuit",
                     help="circuit: the headline workload; sweep: per-target dense-kernel timing + copy calibration")
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--depth", type=int, default=40)
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--fusion", default="12,4,512",
+                    help="tileQubits,lowQubits,maxQueue of the gate-fusion queue, or 'off'")
+    ap.add_argument("--no-unfused", action="store_true", help="skip the one-step unfused comparison")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported by this single-GPU build "
-              "(the statevector path would shard by prefix qubits; see DESIGN.md)", file=sys.stderr)
+              "(a multi-GPU statevector would shard by its highest qubits)", file=sys.stderr)
         return 2
 
     from quest_amd import api as Q, circuits as qc
@@ -180,29 +211,30 @@ def main() -> int:
     n = args.qubits
     if args.mode == "sweep":
         Q.ensure_env()
+        Q.set_fusion(False)
         kernel_sweep(Q, n)
         return 0
     circ = qc.layered(n, args.depth, args.seed, ctrl3=True, qft=True, gadgets=64, gadget_range=(4, 12),
                       num_expec=16, probs=True)
     gates = qc.count_gates(circ)
     reductions = sum(1 for op in circ.ops if op[0].startswith("OUT_"))
-    alg_bytes = sum(qc.gate_bytes(op, n, False) for op in circ.ops)
 
     Q.ensure_env()
+    fused = args.fusion != "off"
+    if fused:
+        tile, low, queue = (int(x) for x in args.fusion.split(","))
+        Q.set_fusion(True, tile, low, queue)
+    else:
+        Q.set_fusion(False)
+    fusion_cfg = Q.get_fusion()
     env = Q.getQuESTEnv()
-    log(f"bench: {n} qubits, {gates} gates + {reductions} reductions per step, GPU={env.isGpuAccelerated}")
+    log(f"bench: {n} qubits, {gates} gates + {reductions} reductions per step, GPU={env.isGpuAccelerated}, "
+        f"fusion={fusion_cfg}")
     q = Q.createQureg(n)
     Q.initRandomPureState(q)
-    calls, blocks = build_step(Q, qc, circ, q)
+    calls = build_step(Q, qc, circ, q)
     gpu = Q.gpu()
     import ctypes as C
-    P = C.c_void_p
-
-    def mk_event():
-        e = P()
-        if gpu.qa_eventCreate(C.byref(e)) != 0:
-            raise RuntimeError("event create failed")
-        return e
 
     for w in range(args.warmup):
         t = time.perf_counter()
@@ -211,53 +243,55 @@ def main() -> int:
         Q.syncQuESTEnv()
         log(f"warmup step {w}: {time.perf_counter() - t:.2f} s")
 
-    ev_start, ev_stop = mk_event(), mk_event()
-    blk_events = [(mk_event(), mk_event()) for _ in range(len(blocks) * args.steps)]
-    blk_sizes = []
-    Q.syncQuESTEnv()
+    # timed region: every tile pass is bracketed by its own pair of HIP
+    # events on the library stream (qa_setFusionTiming)
+    gpu.qa_setFusionTiming(1)
     t0 = time.perf_counter()
-    gpu.qa_eventRecord(ev_start)
-    bi = 0
-    for s in range(args.steps):
-        starts = {b[0]: j for j, b in enumerate(blocks)}
-        ends = {b[1] - 1: j for j, b in enumerate(blocks)}
-        for ci, (f, a) in enumerate(calls):
-            if ci in starts:
-                gpu.qa_eventRecord(blk_events[bi][0])
-            f(*a)
-            if ci in ends:
-                gpu.qa_eventRecord(blk_events[bi][1])
-                b = blocks[ends[ci]]
-                blk_sizes.append(b[1] - b[0])
-                bi += 1
-        log(f"step {s} enqueued")
-    gpu.qa_eventRecord(ev_stop)
-    Q.syncQuESTEnv()
+    total_ms = timed_steps(Q, gpu, calls, args.steps)
     wall = time.perf_counter() - t0
-    ms = C.c_double()
-    gpu.qa_eventElapsedMs(ev_start, ev_stop, C.byref(ms))
-    total_ms = ms.value
-    blk_ms = 0.0
-    for (a, b) in blk_events:
-        gpu.qa_eventElapsedMs(a, b, C.byref(ms))
-        blk_ms += ms.value
-    launches = sum(blk_sizes)
-    dense_avg_s = blk_ms / 1e3 / launches
-    dense_bytes = 2 * 16 * (1 << n)
-    achieved = dense_bytes / dense_avg_s
+    passes, fgates, pass_ms = C.c_int64(), C.c_int64(), C.c_double()
+    gpu.qa_getFusionStats(C.byref(passes), C.byref(fgates), C.byref(pass_ms))
+    gpu.qa_setFusionTiming(0)
     value = args.steps * gates / (total_ms / 1e3)
-    log(f"timed: {total_ms:.1f} ms (wall {wall * 1e3:.1f} ms) for {args.steps} steps; "
-        f"applyCompMatr1 avg {dense_avg_s * 1e3:.3f} ms over {launches} launches = {achieved / 1e9:.0f} GB/s")
+    state = 16 * (1 << n)
+    pass_bytes = 2 * state
+    log(f"timed: {total_ms:.1f} ms (wall {wall * 1e3:.1f} ms) for {args.steps} steps = {value:.1f} gates/s")
+
+    roofline = None
+    if passes.value:
+        avg_s = pass_ms.value / 1e3 / passes.value
+        achieved = pass_bytes / avg_s
+        log(f"tile passes: {passes.value} ({fgates.value} gates, {fgates.value / passes.value:.2f} per pass), "
+            f"avg {avg_s * 1e3:.3f} ms = {achieved / 1e9:.0f} GB/s; passes are "
+            f"{100 * pass_ms.value / total_ms:.1f}% of the timed region")
+        traffic = None
+        tfile = ROOT / "profiles" / f"k_tilePass_{n}q_fp64_traffic.json"
+        if tfile.exists():
+            traffic = json.loads(tfile.read_text()).get("traffic_bytes_per_dispatch")
+        roofline = {
+            "bound": "hbm",
+            "achieved": achieved / 1e9,
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK,
+            "traffic": traffic,
+            "kernel": "k_tilePass<double> (fused gate pass)",
+            "bytes_per_launch": pass_bytes,
+            "avg_launch_ms": avg_s * 1e3,
+            "launches": passes.value,
+            "share_of_step_time": pass_ms.value / total_ms,
+        }
+
+    unfused = None
+    if fused and not args.no_unfused:
+        Q.set_fusion(False)
+        ums = timed_steps(Q, gpu, calls, 1)
+        unfused = {"value": gates / (ums / 1e3), "ms_per_step": ums}
+        log(f"unfused: {ums:.1f} ms per step = {unfused['value']:.1f} gates/s")
+        Q.set_fusion(True, *fusion_cfg[1:])
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(circ, n)
     Q.destroyQureg(q)
-
-    # HBM bytes per launch of the dominant kernel, from the calibrated
-    # rocprofv3 FETCH_SIZE / WRITE_SIZE runs committed under profiles/
-    traffic = None
-    tfile = ROOT / "profiles" / f"k_oneTargDense_{n}q_fp64_traffic.json"
-    if tfile.exists():
-        traffic = json.loads(tfile.read_text()).get("traffic_bytes_per_dispatch")
 
     out = {
         "metric": METRIC,
@@ -279,21 +313,15 @@ def main() -> int:
                          f"{n} calcProbOfQubitOutcome + 16 calcExpecPauliStr"),
             "gates_per_step": gates,
             "reductions_per_step": reductions,
-            "state_bytes": 16 * (1 << n),
+            "state_bytes": state,
             "parallelism": "single GPU",
+            "fusion": ({"tile_qubits": fusion_cfg[1], "low_qubits": fusion_cfg[2], "max_queue": fusion_cfg[3],
+                        "passes_per_step": passes.value / args.steps,
+                        "gates_per_pass": (fgates.value / passes.value) if passes.value else None}
+                       if fused else "off"),
         },
-        "hbm_fraction_all_ops": (args.steps * alg_bytes / (total_ms / 1e3)) / HBM_PEAK,
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved / 1e9,
-            "peak": HBM_PEAK / 1e9,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK,
-            "traffic": traffic,
-            "kernel": "k_oneTargDense<double> (applyCompMatr1)",
-            "bytes_per_launch": dense_bytes,
-            "avg_launch_ms": dense_avg_s * 1e3,
-        },
+        "roofline": roofline,
+        "unfused": unfused,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

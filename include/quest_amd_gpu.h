@@ -107,17 +107,26 @@ int qa_eventDestroy(void* event);
    tensor/gadget, phase gadget, one-qubit dephasing and one-qubit
    depolarising entry points queue their gate instead of launching it.
    Queued gates are applied, in order and with arithmetic identical to the
-   unfused kernels, by LDS-tiled passes of 2^tileQubits amplitudes
-   (2^(tileQubits+1) in FP32) whose low `lowQubits` qubits are always in the
-   tile; every other entry point (reductions, copies, getters, setters,
-   sync, event records) first applies what is queued.  Queues are per state
-   and are flushed after `maxQueue` gates.  Changing the configuration
-   flushes every queue.  tileQubits in [2,13], lowQubits in
-   [0, tileQubits-2], maxQueue in [1,4096]; fusion is off after qa_init. */
+   unfused kernels, by tiled passes over the state: each workgroup holds a
+   tile of 2^tileQubits amplitudes (whose low `lowQubits` qubits are always
+   in the tile) and applies runs of gates whose dense targets lie in 4 of
+   the tile's qubits in registers.  A gate is queued when its dense targets
+   (Pauli X/Y bits included) number at most 4; every other entry point
+   (reductions, copies, getters, setters, sync, event records) first applies
+   what is queued.  Queues are per state, states of fewer than 4 qubits are
+   never queued, and a queue is flushed after `maxQueue` gates.  Changing
+   the configuration flushes every queue.  tileQubits in [4,13], lowQubits
+   in [0, tileQubits-2], maxQueue in [1,4096]; fusion is off after qa_init. */
 int qa_setFusion(int enabled, int tileQubits, int lowQubits, int maxQueue);
 int qa_getFusion(int* enabled, int* tileQubits, int* lowQubits, int* maxQueue);
 /* applies the gates queued on `amps` (NULL: on every state) */
 int qa_flush(void* amps);
+/* counters of tile passes launched and gates they applied since the last
+   qa_getFusionStats (which resets them); with timing on, each pass launch is
+   bracketed by HIP events on the library stream and passMs sums their
+   elapsed times (qa_getFusionStats waits for them) */
+int qa_setFusionTiming(int enabled);
+int qa_getFusionStats(int64_t* passes, int64_t* gates, double* passMs);
 
 /* ------------------------------------------------------------------ */
 /* statevector gates             (gpu_subroutines.hpp:33-70)          */

@@ -22,6 +22,9 @@ Output: one JSON line:
              duration, measured with HIP events around every pass launch
              inside the timed steps
   unfused    the same step once more with fusion off (one kernel per gate)
+  hbm_fraction_unfused_bytes  the step's algorithmic bytes counted as if
+             every gate and reduction were its own pass, per second, over
+             8 TB/s (above 1 when fusion saves passes)
   cpu_baseline  QuEST's own FP64 OpenMP CPU build (oracle/_ref) on a bounded
              sample of the same workload at the same size, on this host
 """
@@ -64,10 +67,20 @@ def build_step(Q, qc, circ, q):
             calls.append((Q.applyFullQuantumFourierTransform, (q,)))
         elif k == "PG":
             calls.append((Q.applyPauliGadget, (q, Q.pauli_str(op[1], op[2]), op[3])))
+        elif k == "CZ":
+            calls.append((Q.applyControlledPauliZ, (q, op[1], op[2])))
+        elif k == "DEPOL":
+            calls.append((Q.mixDepolarising, (q, op[1], op[2])))
+        elif k == "KRAUS":
+            import numpy as np
+            km = Q.make_krausmap([np.array(m, dtype=complex) for m in op[2]])
+            calls.append((Q.mixKrausMap, (q, Q.ints(op[1]), len(op[1]), km)))
         elif k == "OUT_PROB":
             calls.append((Q.calcProbOfQubitOutcome, (q, op[1], op[2])))
         elif k == "OUT_EXPEC":
             calls.append((Q.calcExpecPauliStr, (q, Q.pauli_str(op[1], op[2]))))
+        elif k == "OUT_PURITY":
+            calls.append((Q.calcPurity, (q,)))
         else:
             raise ValueError(k)
     return calls
@@ -187,8 +200,9 @@ def timed_steps(Q, gpu, calls, steps: int) -> float:
 
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--mode", choices=["circuit", "sweep"], default="circuit",
-                    help="circuit: the headline workload; sweep: per-target dense-kernel timing + copy calibration")
+    ap.add_argument("--mode", choices=["circuit", "density", "sweep"], default="circuit",
+                    help="circuit: the headline 30-qubit statevector workload; density: the 14-qubit noisy "
+                         "density-matrix workload; sweep: per-target dense-kernel timing + copy calibration")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
@@ -214,10 +228,24 @@ def main() -> int:
         Q.set_fusion(False)
         kernel_sweep(Q, n)
         return 0
-    circ = qc.layered(n, args.depth, args.seed, ctrl3=True, qft=True, gadgets=64, gadget_range=(4, 12),
-                      num_expec=16, probs=True)
+    dens = args.mode == "density"
+    if dens:
+        # 14-qubit density matrix: depth 20 of [Haar applyCompMatr1 + mixDepolarising on every qubit, CZ +
+        # mixDepolarising on both qubits of 7 random disjoint pairs], a 2-qubit mixKrausMap every 5 layers
+        n = args.qubits if args.qubits != 30 else 14
+        depth = args.depth if args.depth != 40 else 20
+        circ = qc.noisy_density(n, depth, args.seed, kraus_every=5, num_expec=8)
+    else:
+        circ = qc.layered(n, args.depth, args.seed, ctrl3=True, qft=True, gadgets=64, gadget_range=(4, 12),
+                          num_expec=16, probs=True)
     gates = qc.count_gates(circ)
+    if dens:
+        # gates and channels, each one quest.h call
+        gates += sum(1 for op in circ.ops if op[0] in ("DEPOL", "KRAUS"))
     reductions = sum(1 for op in circ.ops if op[0].startswith("OUT_"))
+    # algorithmic HBM bytes of the step with every gate and reduction as its
+    # own pass (the unfused formula per op)
+    alg_bytes = sum(qc.gate_bytes(op, n, dens) for op in circ.ops)
 
     Q.ensure_env()
     fused = args.fusion != "off"
@@ -230,7 +258,7 @@ def main() -> int:
     env = Q.getQuESTEnv()
     log(f"bench: {n} qubits, {gates} gates + {reductions} reductions per step, GPU={env.isGpuAccelerated}, "
         f"fusion={fusion_cfg}")
-    q = Q.createQureg(n)
+    q = Q.createDensityQureg(n) if dens else Q.createQureg(n)
     Q.initRandomPureState(q)
     calls = build_step(Q, qc, circ, q)
     gpu = Q.gpu()
@@ -253,7 +281,7 @@ def main() -> int:
     gpu.qa_getFusionStats(C.byref(passes), C.byref(fgates), C.byref(pass_ms))
     gpu.qa_setFusionTiming(0)
     value = args.steps * gates / (total_ms / 1e3)
-    state = 16 * (1 << n)
+    state = 16 * (1 << (2 * n if dens else n))
     pass_bytes = 2 * state
     log(f"timed: {total_ms:.1f} ms (wall {wall * 1e3:.1f} ms) for {args.steps} steps = {value:.1f} gates/s")
 
@@ -283,20 +311,26 @@ def main() -> int:
         }
 
     unfused = None
-    if fused and not args.no_unfused:
+    if fused and not args.no_unfused and not dens:
         Q.set_fusion(False)
         ums = timed_steps(Q, gpu, calls, 1)
         unfused = {"value": gates / (ums / 1e3), "ms_per_step": ums}
         log(f"unfused: {ums:.1f} ms per step = {unfused['value']:.1f} gates/s")
         Q.set_fusion(True, *fusion_cfg[1:])
 
-    cpu = None if args.no_cpu_baseline else cpu_baseline(circ, n)
+    if fused and not args.no_unfused and dens:
+        Q.set_fusion(False)
+        ums = timed_steps(Q, gpu, calls, args.steps)
+        unfused = {"value": args.steps * gates / (ums / 1e3), "ms_per_step": ums / args.steps}
+        log(f"unfused: {ums / args.steps:.1f} ms per step = {unfused['value']:.1f} ops/s")
+        Q.set_fusion(True, *fusion_cfg[1:])
+    cpu = None if (args.no_cpu_baseline or dens) else cpu_baseline(circ, n)
     Q.destroyQureg(q)
 
     out = {
-        "metric": METRIC,
+        "metric": METRIC if not dens else "14-qubit FP64 density-matrix ops/s (gates and channels)",
         "value": value,
-        "unit": "gates/s",
+        "unit": "gates/s" if not dens else "ops/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
@@ -307,11 +341,15 @@ def main() -> int:
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": (f"{n}-qubit complex128 statevector: {args.depth} layers x ({n} Haar applyCompMatr1 + "
-                         f"{n // 2} applyTwoQubitPhaseShift + 1 3-control applyMultiControlledCompMatr1), "
-                         "applyFullQuantumFourierTransform, 64 applyPauliGadget (4-12 qubits); "
-                         f"{n} calcProbOfQubitOutcome + 16 calcExpecPauliStr"),
-            "gates_per_step": gates,
+            "workload": ((f"{n}-qubit complex128 density matrix: {depth} layers x ({n} Haar applyCompMatr1 and "
+                          f"{n} mixDepolarising, {n // 2} CZ and {2 * (n // 2)} mixDepolarising), a 2-qubit "
+                          "4-operator mixKrausMap every 5 layers; calcPurity + 8 calcExpecPauliStr + "
+                          f"{n} calcProbOfQubitOutcome") if dens else
+                         (f"{n}-qubit complex128 statevector: {args.depth} layers x ({n} Haar applyCompMatr1 + "
+                          f"{n // 2} applyTwoQubitPhaseShift + 1 3-control applyMultiControlledCompMatr1), "
+                          "applyFullQuantumFourierTransform, 64 applyPauliGadget (4-12 qubits); "
+                          f"{n} calcProbOfQubitOutcome + 16 calcExpecPauliStr")),
+            ("ops_per_step" if dens else "gates_per_step"): gates,
             "reductions_per_step": reductions,
             "state_bytes": state,
             "parallelism": "single GPU",
@@ -320,6 +358,7 @@ def main() -> int:
                         "gates_per_pass": (fgates.value / passes.value) if passes.value else None}
                        if fused else "off"),
         },
+        "hbm_fraction_unfused_bytes": (args.steps * alg_bytes / (total_ms / 1e3)) / HBM_PEAK,
         "roofline": roofline,
         "unfused": unfused,
         "cpu_baseline": cpu,

@@ -21,347 +21,47 @@
 // order, so fused and unfused runs give identical amplitudes.
 #include <algorithm>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <unordered_map>
 #include <vector>
 
 #include "qa_common.hpp"
 #include "qa_fused.hpp"
+#include "qa_tile.hpp"
 
 namespace qa {
 namespace fuse {
 
-// ---- tile-space program -----------------------------------------------------
-//
-// Inside a pass each thread holds RA = 2^RB amplitudes of the tile in
-// registers: those whose tile indices differ only in the RB "register
-// slots" of the current group.  A run of gates whose dense targets all lie
-// in those slots is applied in registers; a GROUP op changes the register
-// slots by writing the registers back to the LDS copy of the tile and
-// gathering the new layout.  LDS is therefore read and written once per
-// group instead of once per gate.
-
-constexpr int RB = 4;        // register slots per group
-constexpr int RA = 1 << RB;  // amplitudes per thread
-enum { F_GROUP = 100 };
-
-struct TOp {
-    int kind;
-    int rb1, rb2;            // register bits of the dense targets
-    int slot[RB];            // GROUP: tile slots of register bits 0..RB-1 (ascending)
-    uint32_t rcMask, rcVal;  // control condition, register part (over v)
-    uint32_t rm1, rm2;       // register parts of the masks below
-    uint64_t jcMask, jcVal;  // control condition, tile part outside the register slots
-    uint64_t gcMask, gcVal;  // control condition, non-local part (tested against the tile base)
-    uint64_t jm1, jm2;       // DIAG1 target / PARITY mask / PAULI maskXY (m1), maskYZ (m2): tile part
-    uint64_t gm1, gm2;       // ... non-local part
-    double f[32];
-};
-
-struct TileArgs {
-    int k;          // tile bits
-    int low;        // contiguous low bits (always local)
-    int numNonLocal;
-    int numOps;
-    int localPos[MAXQ];     // sorted global positions of the tile bits
-    int nonLocalPos[MAXQ];  // sorted global positions of the other bits
-};
-
-__device__ __forceinline__ uint64_t deposit(uint64_t v, const int* pos, int num) {
-    uint64_t r = 0;
-    for (int i = 0; i < num; i++) r |= ((v >> i) & 1ull) << pos[i];
-    return r;
-}
-
-// LDS position of tile amplitude j: the low 4 index bits are XORed with the
-// next 4, so threads whose amplitudes are 16 apart (register slots 0..3)
-// still spread over all banks
-__device__ __forceinline__ uint32_t swz(uint32_t j) { return j ^ ((j >> 4) & 15u); }
-
-template <int V>
-__device__ __forceinline__ uint32_t offOf(const uint32_t* sb) {
-    uint32_t o = 0;
-#pragma unroll
-    for (int b = 0; b < RB; b++)
-        if (V & (1 << b)) o |= sb[b];
-    return o;
-}
-
-template <typename T, int V>
-struct RegIO {
-    __device__ __forceinline__ static void load(ct<T>* r, const ct<T>* tile, uint32_t jb, const uint32_t* sb) {
-        r[V] = tile[swz(jb | offOf<V>(sb))];
-        RegIO<T, V + 1>::load(r, tile, jb, sb);
-    }
-    __device__ __forceinline__ static void store(const ct<T>* r, ct<T>* tile, uint32_t jb, const uint32_t* sb) {
-        tile[swz(jb | offOf<V>(sb))] = r[V];
-        RegIO<T, V + 1>::store(r, tile, jb, sb);
-    }
-};
-template <typename T>
-struct RegIO<T, RA> {
-    __device__ __forceinline__ static void load(ct<T>*, const ct<T>*, uint32_t, const uint32_t*) {}
-    __device__ __forceinline__ static void store(const ct<T>*, ct<T>*, uint32_t, const uint32_t*) {}
-};
-
-__device__ __forceinline__ bool regCond(int v, const TOp& op) { return ((uint32_t)v & op.rcMask) == op.rcVal; }
-
-template <typename T>
-__device__ __forceinline__ ct<T> cf(const TOp& op, int i) {
-    return ct<T>{(T)op.f[2 * i], (T)op.f[2 * i + 1]};
-}
-
-// the arithmetic of each gate below is the expression of its single-gate
-// kernel (qa_gates.hip / qa_channels.hip), term for term
-
-template <typename T, int B>
-__device__ __forceinline__ void regDense1(ct<T>* r, const TOp& op) {
-    const ct<T> m00 = cf<T>(op, 0), m01 = cf<T>(op, 1), m10 = cf<T>(op, 2), m11 = cf<T>(op, 3);
-#pragma unroll
-    for (int v = 0; v < RA; v++) {
-        if (v & (1 << B)) continue;
-        if (!regCond(v, op)) continue;
-        const ct<T> a0 = r[v], a1 = r[v | (1 << B)];
-        ct<T> b0 = cmulT(m00, a0);
-        b0 = cfmaT(m01, a1, b0);
-        ct<T> b1 = cmulT(m10, a0);
-        b1 = cfmaT(m11, a1, b1);
-        r[v] = b0;
-        r[v | (1 << B)] = b1;
-    }
-}
-
-template <typename T, int B1, int B2>
-__device__ __forceinline__ void regDense2(ct<T>* r, const TOp& op) {
-    constexpr int off[4] = {0, 1 << B1, 1 << B2, (1 << B1) | (1 << B2)};
-#pragma unroll
-    for (int v = 0; v < RA; v++) {
-        if (v & ((1 << B1) | (1 << B2))) continue;
-        if (!regCond(v, op)) continue;
-        ct<T> x[4];
-#pragma unroll
-        for (int c = 0; c < 4; c++) x[c] = r[v | off[c]];
-#pragma unroll
-        for (int row = 0; row < 4; row++) {
-            ct<T> acc = cmulT(cf<T>(op, 4 * row), x[0]);
-#pragma unroll
-            for (int c = 1; c < 4; c++) acc = cfmaT(cf<T>(op, 4 * row + c), x[c], acc);
-            r[v | off[row]] = acc;
-        }
-    }
-}
-
-template <typename T, int B1, int B2>
-__device__ __forceinline__ void regSwap(ct<T>* r, const TOp& op) {
-#pragma unroll
-    for (int v = 0; v < RA; v++) {
-        if (!(v & (1 << B1)) || (v & (1 << B2))) continue;
-        if (!regCond(v, op)) continue;
-        constexpr int flip = (1 << B1) | (1 << B2);
-        const ct<T> x = r[v];
-        r[v] = r[v ^ flip];
-        r[v ^ flip] = x;
-    }
-}
-
-// one-qubit depolarising / dephasing on (ket, bra): both formulas are
-// symmetric in the two bits, so B1 < B2 covers either order
-template <typename T, int B1, int B2>
-__device__ __forceinline__ void regDepol1(ct<T>* r, const TOp& op) {
-    const T c1 = (T)op.f[0], c2 = (T)op.f[1], c3 = (T)op.f[2];
-#pragma unroll
-    for (int v = 0; v < RA; v++) {
-        if (v & ((1 << B1) | (1 << B2))) continue;
-        constexpr int k1 = 1 << B1, k2 = 1 << B2;
-        const ct<T> a00 = r[v], a01 = r[v | k1], a10 = r[v | k2], a11 = r[v | k1 | k2];
-        r[v] = ct<T>{fmaT(c1, a00.re, c2 * a11.re), fmaT(c1, a00.im, c2 * a11.im)};
-        r[v | k1] = ct<T>{c3 * a01.re, c3 * a01.im};
-        r[v | k2] = ct<T>{c3 * a10.re, c3 * a10.im};
-        r[v | k1 | k2] = ct<T>{fmaT(c1, a11.re, c2 * a00.re), fmaT(c1, a11.im, c2 * a00.im)};
-    }
-}
-
-template <typename T, int B1, int B2>
-__device__ __forceinline__ void regDeph1(ct<T>* r, const TOp& op) {
-    const T fac = (T)op.f[0];
-#pragma unroll
-    for (int v = 0; v < RA; v++) {
-        if (v & ((1 << B1) | (1 << B2))) continue;
-        constexpr int k1 = 1 << B1, k2 = 1 << B2;
-        const ct<T> x = r[v | k1], y = r[v | k2];
-        r[v | k1] = ct<T>{x.re * fac, x.im * fac};
-        r[v | k2] = ct<T>{y.re * fac, y.im * fac};
-    }
-}
-
-// Pauli tensor / gadget with X/Y bits MXY (all in registers); thrPar is the
-// parity of the maskYZ bits outside the registers for this thread
-template <typename T, int MXY>
-__device__ __forceinline__ void regPauli(ct<T>* r, const TOp& op, int thrPar) {
-    constexpr int top = 31 - __builtin_clz((unsigned)MXY);
-    const ct<T> ampFac = cf<T>(op, 0), pairFac = cf<T>(op, 1);
-#pragma unroll
-    for (int v = 0; v < RA; v++) {
-        if (v & (1 << top)) continue;
-        if (!regCond(v, op)) continue;
-        const int w = v ^ MXY;
-        const ct<T> va = r[v], vb = r[w];
-        const T sA = ((__popc((uint32_t)v & op.rm2) & 1) ^ thrPar) ? (T)-1 : (T)1;
-        const T sB = ((__popc((uint32_t)w & op.rm2) & 1) ^ thrPar) ? (T)-1 : (T)1;
-        const ct<T> pB = {pairFac.re * sB, pairFac.im * sB};
-        const ct<T> pA = {pairFac.re * sA, pairFac.im * sA};
-        r[v] = cfmaT(pB, vb, cmulT(ampFac, va));
-        r[w] = cfmaT(pA, va, cmulT(ampFac, vb));
-    }
-}
-
-template <typename T>
-__device__ __forceinline__ void regDiag1(ct<T>* r, const TOp& op, bool thrBit) {
-    const ct<T> d0 = cf<T>(op, 0), d1 = cf<T>(op, 1);
-#pragma unroll
-    for (int v = 0; v < RA; v++) {
-        if (!regCond(v, op)) continue;
-        const bool bit = op.rm1 ? (((uint32_t)v & op.rm1) != 0) : thrBit;
-        r[v] = cmulT(r[v], bit ? d1 : d0);
-    }
-}
-
-template <typename T>
-__device__ __forceinline__ void regParity(ct<T>* r, const TOp& op, int thrPar) {
-    const ct<T> f0 = cf<T>(op, 0), f1 = cf<T>(op, 1);
-#pragma unroll
-    for (int v = 0; v < RA; v++) {
-        if (!regCond(v, op)) continue;
-        r[v] = cmulT(r[v], ((__popc((uint32_t)v & op.rm1) & 1) ^ thrPar) ? f1 : f0);
-    }
-}
-
-#define QA_PAIR_CASES(FN)                                             \
-    case 0 * RB + 1: FN<T, 0, 1>(r, op); break;                       \
-    case 0 * RB + 2: FN<T, 0, 2>(r, op); break;                       \
-    case 0 * RB + 3: FN<T, 0, 3>(r, op); break;                       \
-    case 1 * RB + 2: FN<T, 1, 2>(r, op); break;                       \
-    case 1 * RB + 3: FN<T, 1, 3>(r, op); break;                       \
-    case 2 * RB + 3: FN<T, 2, 3>(r, op); break;
-#define QA_PAIR_CASES_REV(FN)                                         \
-    case 1 * RB + 0: FN<T, 1, 0>(r, op); break;                       \
-    case 2 * RB + 0: FN<T, 2, 0>(r, op); break;                       \
-    case 3 * RB + 0: FN<T, 3, 0>(r, op); break;                       \
-    case 2 * RB + 1: FN<T, 2, 1>(r, op); break;                       \
-    case 3 * RB + 1: FN<T, 3, 1>(r, op); break;                       \
-    case 3 * RB + 2: FN<T, 3, 2>(r, op); break;
-
 // NT: non-temporal HBM loads/stores, used when the contiguous low run of a
-// tile row is at least 2^STREAM_LOW_BIT amplitudes
+// tile row is at least 2^STREAM_LOW_BIT amplitudes.  Persistent workgroups:
+// each prefetches its next tile into registers before working on the
+// current one, so HBM traffic overlaps the register and LDS work.
 template <typename T, int NT>
 __global__ void __launch_bounds__(512) k_tilePass(T* __restrict__ amps, TileArgs a, const TOp* __restrict__ ops) {
     extern __shared__ unsigned char smem[];
-    const int k = a.k;
     ct<T>* tile = reinterpret_cast<ct<T>*>(smem);
-    uint64_t* table = reinterpret_cast<uint64_t*>(smem + (sizeof(ct<T>) << k));
-    const uint32_t nt = blockDim.x;
     const uint32_t tid = threadIdx.x;
-    const uint64_t base = deposit((uint64_t)blockIdx.x, a.nonLocalPos, a.numNonLocal);
-    // global offset of tile row h (the k-low tile bits above the contiguous
-    // low run) = tlo[low half of h] | thi[high half of h]
-    const int hb = k - a.low, b0 = hb >> 1, b1 = hb - b0;
-    uint64_t* tlo = table;
-    uint64_t* thi = table + (1ull << b0);
-    const uint32_t lowMask = (1u << a.low) - 1u, b0Mask = (1u << b0) - 1u;
-    const int s0 = a.low + b0;
-    for (uint32_t h = tid; h < (1u << b0); h += nt) tlo[h] = deposit(h, a.localPos + a.low, b0);
-    for (uint32_t h = tid; h < (1u << b1); h += nt) thi[h] = deposit(h, a.localPos + a.low + b0, b1);
-    __syncthreads();
-    // every thread moves exactly RA = tileSize / nt amplitudes: all RA
-    // loads are issued before the first LDS write
-    ct<T> r[RA];
-#pragma unroll
-    for (int i = 0; i < RA; i++) {
-        const uint32_t j = tid + (uint32_t)i * nt;
-        r[i] = ldx<NT>(amps, base | (j & lowMask) | tlo[(j >> a.low) & b0Mask] | thi[j >> s0]);
-    }
-#pragma unroll
-    for (int i = 0; i < RA; i++) tile[swz(tid + (uint32_t)i * nt)] = r[i];
+    RegLayout row, first;
+    passLayouts(row, first, a, ops, tid);
+    const RegLayout& loadL = a.firstDirect ? first : row;
 
-    uint32_t sb[RB] = {0, 0, 0, 0};
-    uint32_t jb = 0;
-    bool have = false;
-    for (int o = 0; o < a.numOps; o++) {
-        const TOp& op = ops[o];
-        if (op.kind == F_GROUP) {
-            if (have) RegIO<T, 0>::store(r, tile, jb, sb);
-            __syncthreads();
-            uint32_t gm = 0;
+    ct<T> rn[RA];
+    uint64_t t = blockIdx.x;
+    if (t < a.numTiles)
+        RegIO<T, NT, 0>::hbmLoad(rn, amps, deposit(t, a.nonLocalPos, a.numNonLocal) | loadL.tg, loadL);
+    for (; t < a.numTiles; t += gridDim.x) {
+        const uint64_t base = deposit(t, a.nonLocalPos, a.numNonLocal);
+        ct<T> r[RA];
 #pragma unroll
-            for (int b = 0; b < RB; b++) {
-                sb[b] = 1u << op.slot[b];
-                gm |= sb[b];
-            }
-            // this thread's tile index with the register slots zero: its id
-            // deposited over the other slots, ascending
-            jb = 0;
-            for (int s = 0, i = 0; s < k; s++) {
-                if ((gm >> s) & 1u) continue;
-                jb |= ((tid >> i) & 1u) << s;
-                i++;
-            }
-            RegIO<T, 0>::load(r, tile, jb, sb);
-            have = true;
-            continue;
-        }
-        if ((base & op.gcMask) != op.gcVal) continue;      // uniform over the workgroup
-        if ((jb & op.jcMask) != op.jcVal) continue;        // uniform over the thread's registers
-        switch (op.kind) {
-            case F_DENSE1:
-                switch (op.rb1) {
-                    case 0: regDense1<T, 0>(r, op); break;
-                    case 1: regDense1<T, 1>(r, op); break;
-                    case 2: regDense1<T, 2>(r, op); break;
-                    case 3: regDense1<T, 3>(r, op); break;
-                }
-                break;
-            case F_DENSE2:
-                switch (op.rb1 * RB + op.rb2) {
-                    QA_PAIR_CASES(regDense2)
-                    QA_PAIR_CASES_REV(regDense2)
-                }
-                break;
-            case F_SWAP:
-                switch (op.rb1 * RB + op.rb2) { QA_PAIR_CASES(regSwap) }
-                break;
-            case F_DEPOL1:
-                switch (op.rb1 * RB + op.rb2) { QA_PAIR_CASES(regDepol1) }
-                break;
-            case F_DEPH1:
-                switch (op.rb1 * RB + op.rb2) { QA_PAIR_CASES(regDeph1) }
-                break;
-            case F_DIAG1:
-                regDiag1<T>(r, op, ((jb & op.jm1) | (base & op.gm1)) != 0);
-                break;
-            case F_PARITY:
-                regParity<T>(r, op, parity64((jb & op.jm1) ^ (base & op.gm1)));
-                break;
-            case F_PAULI: {
-                const int thrPar = parity64((jb & op.jm2) ^ (base & op.gm2));
-                switch (op.rm1) {
-#define QA_PAULI_CASE(M) case M: regPauli<T, M>(r, op, thrPar); break;
-                    QA_PAULI_CASE(1) QA_PAULI_CASE(2) QA_PAULI_CASE(3) QA_PAULI_CASE(4) QA_PAULI_CASE(5)
-                    QA_PAULI_CASE(6) QA_PAULI_CASE(7) QA_PAULI_CASE(8) QA_PAULI_CASE(9) QA_PAULI_CASE(10)
-                    QA_PAULI_CASE(11) QA_PAULI_CASE(12) QA_PAULI_CASE(13) QA_PAULI_CASE(14) QA_PAULI_CASE(15)
-#undef QA_PAULI_CASE
-                }
-                break;
-            }
-        }
-    }
-    if (have) RegIO<T, 0>::store(r, tile, jb, sb);
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < RA; i++) r[i] = tile[swz(tid + (uint32_t)i * nt)];
-#pragma unroll
-    for (int i = 0; i < RA; i++) {
-        const uint32_t j = tid + (uint32_t)i * nt;
-        stx<NT>(amps, base | (j & lowMask) | tlo[(j >> a.low) & b0Mask] | thi[j >> s0], r[i]);
+        for (int v = 0; v < RA; v++) r[v] = rn[v];
+        const uint64_t tn = t + gridDim.x;
+        if (tn < a.numTiles)
+            RegIO<T, NT, 0>::hbmLoad(rn, amps, deposit(tn, a.nonLocalPos, a.numNonLocal) | loadL.tg, loadL);
+        tileBody<T, NT>(amps, a, ops, base, r, tile, row, first, tid);
+        // the next tile's first LDS writes must not overtake this tile's reads
+        if (a.usesLds) __syncthreads();
     }
 }
 
@@ -412,10 +112,26 @@ uint64_t denseMask(const GOp& g) {
 }
 
 // dynamic LDS of one tile pass: the tile plus the two row-offset tables
-size_t ldsBytes(int prec, int k, int low) {
+size_t ldsBytes(int prec, int k) {
     size_t amp = prec == QA_FP64 ? 16 : 8;
-    int hb = k - low, b0 = hb >> 1, b1 = hb - b0;
-    return (amp << k) + sizeof(uint64_t) * ((1ull << b0) + (1ull << b1));
+    return amp << k;
+}
+
+unsigned residentBlocks(const void* fn, unsigned block, size_t lds) {
+    static std::mutex mu;
+    static std::map<std::tuple<const void*, unsigned, size_t>, unsigned> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    const auto key = std::make_tuple(fn, block, lds);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int perCU = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, fn, (int)block, lds) != hipSuccess || perCU < 1)
+        perCU = 1;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;
+    unsigned r = (unsigned)(perCU * cus);
+    cache[key] = r;
+    return r;
 }
 
 std::vector<TOp> g_prog;  // program of the pass being launched (under g_mu)
@@ -444,20 +160,21 @@ void splitMask(uint64_t gm, const Layout& L, uint32_t& reg, uint64_t& tile, uint
 TOp toTile(const GOp& g, const Layout& L) {
     TOp t;
     std::memset(&t, 0, sizeof(t));
-    t.kind = g.kind;
-    t.rb1 = g.t1 >= 0 ? L.regOf[L.slotOf[g.t1]] : -1;
-    t.rb2 = g.t2 >= 0 ? L.regOf[L.slotOf[g.t2]] : -1;
-    if ((g.kind == F_SWAP || g.kind == F_DEPOL1 || g.kind == F_DEPH1) && t.rb1 > t.rb2) std::swap(t.rb1, t.rb2);
+    TOpHead& h = t.h;
+    h.kind = g.kind;
+    h.rb1 = g.t1 >= 0 ? L.regOf[L.slotOf[g.t1]] : -1;
+    h.rb2 = g.t2 >= 0 ? L.regOf[L.slotOf[g.t2]] : -1;
+    if ((g.kind == F_SWAP || g.kind == F_DEPOL1 || g.kind == F_DEPH1) && h.rb1 > h.rb2) std::swap(h.rb1, h.rb2);
     uint32_t rv;
     uint64_t tv, gv;
-    splitMask(g.cMask, L, t.rcMask, t.jcMask, t.gcMask);
+    splitMask(g.cMask, L, h.rcMask, h.jcMask, h.gcMask);
     splitMask(g.cVal, L, rv, tv, gv);
-    t.rcVal = rv;
-    t.jcVal = tv;
-    t.gcVal = gv;
-    if (g.kind == F_DIAG1) splitMask(1ull << g.t1, L, t.rm1, t.jm1, t.gm1);
-    else splitMask(g.m1, L, t.rm1, t.jm1, t.gm1);
-    splitMask(g.m2, L, t.rm2, t.jm2, t.gm2);
+    h.rcVal = rv;
+    h.jcVal = tv;
+    h.gcVal = gv;
+    if (g.kind == F_DIAG1) splitMask(1ull << g.t1, L, h.rm1, t.jm1, t.gm1);
+    else splitMask(g.m1, L, h.rm1, t.jm1, t.gm1);
+    splitMask(g.m2, L, h.rm2, t.jm2, t.gm2);
     std::memcpy(t.f, g.f, sizeof(t.f));
     return t;
 }
@@ -474,16 +191,28 @@ uint32_t regNeed(const GOp& g, const Layout& L) {
     return r;
 }
 
-// ops [begin, end) of the queue, with every dense target local, as one
-// pass: groups of consecutive ops whose dense targets fit RB register slots
-int launchPass(void* amps, const Queue& qu, const std::vector<GOp>& ops, size_t begin, size_t end, uint64_t localMask) {
-    const int n = qu.numQubits;
-    TileArgs a;
+}  // namespace fuse (planner helpers)
+
+size_t fuse::takeRun(const GOp* ops, size_t num, int numQubits, int kMax, int low, uint64_t& localMask) {
+    uint64_t L = (low >= 64) ? ~0ull : ((1ull << low) - 1ull);
+    size_t i = 0;
+    for (; i < num; i++) {
+        uint64_t u = L | denseMask(ops[i]);
+        if (__builtin_popcountll(u) > kMax) break;
+        L = u;
+    }
+    for (int q = 0; q < numQubits && __builtin_popcountll(L) < kMax; q++) L |= 1ull << q;
+    localMask = L;
+    return i;
+}
+
+void fuse::planPass(const GOp* ops, size_t num, int numQubits, uint64_t localMask, int lowQubits, TileArgs& a,
+                    std::vector<TOp>& prog) {
     std::memset(&a, 0, sizeof(a));
     Layout L;
     for (int q = 0; q < 64; q++) L.slotOf[q] = -1;
     int k = 0, nl = 0;
-    for (int q = 0; q < n; q++) {
+    for (int q = 0; q < numQubits; q++) {
         if ((localMask >> q) & 1ull) {
             L.slotOf[q] = k;
             a.localPos[k++] = q;
@@ -492,26 +221,32 @@ int launchPass(void* amps, const Queue& qu, const std::vector<GOp>& ops, size_t 
         }
     }
     a.k = k;
-    a.low = std::min(g_cfg.lowQubits, k);
     a.numNonLocal = nl;
+    a.numTiles = 1ull << nl;
+    // slots of the contiguous low qubits: a layout whose register slots
+    // avoid them reads and writes HBM in runs as long as the row layout's
+    const int low = std::min(lowQubits, k);
+    const uint32_t lowSlots = (1u << low) - 1u;
 
-    // plan the groups
-    std::vector<TOp>& prog = g_prog;
-    prog.clear();
-    size_t i = begin;
-    while (i < end) {
+    // groups of consecutive ops whose register needs fit RB slots; padding
+    // takes the highest free slots outside the low run first
+    const size_t start = prog.size();
+    int numGroups = 0;
+    uint32_t firstG = 0, lastG = 0;
+    size_t i = 0;
+    while (i < num) {
         uint32_t G = 0;
         size_t j = i;
-        for (; j < end; j++) {
+        for (; j < num; j++) {
             uint32_t u = G | regNeed(ops[j], L);
             if (__builtin_popcount(u) > RB) break;
             G = u;
         }
-        // pad the register slots with the lowest free ones
-        for (int sl = 0; sl < k && __builtin_popcount(G) < RB; sl++) G |= 1u << sl;
+        for (int sl = k - 1; sl >= low && __builtin_popcount(G) < RB; sl--) G |= 1u << sl;
+        for (int sl = 0; sl < low && __builtin_popcount(G) < RB; sl++) G |= 1u << sl;
         TOp grp;
         std::memset(&grp, 0, sizeof(grp));
-        grp.kind = F_GROUP;
+        grp.h.kind = F_GROUP;
         for (int sl = 0; sl < k; sl++) L.regOf[sl] = -1;
         for (int sl = 0, b = 0; sl < k; sl++)
             if ((G >> sl) & 1u) {
@@ -519,8 +254,26 @@ int launchPass(void* amps, const Queue& qu, const std::vector<GOp>& ops, size_t 
                 L.regOf[sl] = b++;
             }
         prog.push_back(grp);
+        if (numGroups == 0) firstG = G;
+        lastG = G;
+        numGroups++;
         for (; i < j; i++) prog.push_back(toTile(ops[i], L));
     }
+    a.firstDirect = (firstG & lowSlots) == 0;
+    a.lastDirect = (lastG & lowSlots) == 0;
+    a.usesLds = !a.firstDirect || !a.lastDirect || numGroups > 1;
+    a.numOps = (int)(prog.size() - start);
+}
+
+namespace fuse {
+
+// launches one pass applying ops [begin, end) of the queue
+int launchPass(void* amps, const Queue& qu, const std::vector<GOp>& ops, size_t begin, size_t end, uint64_t localMask) {
+    TileArgs a;
+    std::vector<TOp>& prog = g_prog;
+    prog.clear();
+    planPass(ops.data() + begin, end - begin, qu.numQubits, localMask, g_cfg.lowQubits, a, prog);
+    const int k = a.k;
 
     size_t num = prog.size();
     if (g_ringPos + num > RING_OPS) {
@@ -532,10 +285,12 @@ int launchPass(void* amps, const Queue& qu, const std::vector<GOp>& ops, size_t 
     TOp* d = g_ringDev + g_ringPos;
     g_ringPos += num;
     QA_HIP(hipMemcpyAsync(d, h, num * sizeof(TOp), hipMemcpyHostToDevice, stream()));
-    a.numOps = (int)num;
-    size_t lds = ldsBytes(qu.prec, k, a.low);
-    unsigned grid = (unsigned)(1ull << nl);
-    unsigned block = 1u << (k - RB);
+    const size_t lds = a.usesLds ? ldsBytes(qu.prec, k) : 0;
+    const unsigned block = 1u << (k - RB);
+    const bool nt = std::min(g_cfg.lowQubits, k) >= STREAM_LOW_BIT;
+    const void* fn = qu.prec == QA_FP64 ? (nt ? (const void*)k_tilePass<double, 1> : (const void*)k_tilePass<double, 0>)
+                                        : (nt ? (const void*)k_tilePass<float, 1> : (const void*)k_tilePass<float, 0>);
+    const unsigned grid = (unsigned)std::min<uint64_t>(a.numTiles, residentBlocks(fn, block, lds));
     std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
     if (g_stats.timing) {
         if (g_stats.used == g_stats.pool.size()) {
@@ -547,7 +302,6 @@ int launchPass(void* amps, const Queue& qu, const std::vector<GOp>& ops, size_t 
         ev = &g_stats.pool[g_stats.used++];
         QA_HIP(hipEventRecord(ev->first, stream()));
     }
-    const bool nt = a.low >= STREAM_LOW_BIT;
     if (qu.prec == QA_FP64) {
         if (nt) k_tilePass<double, 1><<<grid, block, lds, stream()>>>((double*)amps, a, d);
         else k_tilePass<double, 0><<<grid, block, lds, stream()>>>((double*)amps, a, d);
@@ -562,8 +316,8 @@ int launchPass(void* amps, const Queue& qu, const std::vector<GOp>& ops, size_t 
     return QA_OK;
 }
 
-// greedy planner: extend the local set while the next gate's non-diagonal
-// targets still fit in k qubits, then launch the run as one pass
+// greedy planner: extend the local set while the next gate's dense targets
+// still fit the tile, then launch the run as one pass
 int flushQueue(void* amps, Queue& qu) {
     if (qu.ops.empty()) return QA_OK;
     if (!ensureRing()) {
@@ -573,28 +327,19 @@ int flushQueue(void* amps, Queue& qu) {
     const int n = qu.numQubits;
     const int kMax = std::min(n, g_cfg.tileQubits);
     const int low = std::min(g_cfg.lowQubits, kMax);
-    const uint64_t lowMask = (low >= 64) ? ~0ull : ((1ull << low) - 1ull);
     std::vector<GOp>& ops = qu.ops;
     size_t i = 0;
     int rc = QA_OK;
     while (i < ops.size() && rc == QA_OK) {
-        uint64_t L = lowMask;
-        size_t start = i;
-        while (i < ops.size()) {
-            uint64_t need = denseMask(ops[i]);
-            uint64_t u = L | need;
-            if (__builtin_popcountll(u) > kMax) break;
-            L = u;
-            i++;
-        }
-        if (i == start) {
+        uint64_t L = 0;
+        size_t run = takeRun(ops.data() + i, ops.size() - i, n, kMax, low, L);
+        if (run == 0) {
             setError("fusion: a gate needs more local qubits than the tile holds");
             rc = QA_ERR_ARG;
             break;
         }
-        // pad the local set to exactly kMax qubits with the lowest free qubits
-        for (int q = 0; q < n && __builtin_popcountll(L) < kMax; q++) L |= 1ull << q;
-        rc = launchPass(amps, qu, ops, start, i, L);
+        rc = launchPass(amps, qu, ops, i, i + run, L);
+        i += run;
     }
     ops.clear();
     return rc;
@@ -693,8 +438,8 @@ int qa_setFusion(int enabled, int tileQubits, int lowQubits, int maxQueue) {
         int dev = 0, maxLds = 0;
         QA_HIP(hipGetDevice(&dev));
         QA_HIP(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-        // FP32 tiles hold one more qubit and the same bytes as FP64 tiles
-        size_t need = qa::fuse::ldsBytes(QA_FP64, tileQubits, lowQubits);
+        // FP64 tiles are the larger ones
+        size_t need = qa::fuse::ldsBytes(QA_FP64, tileQubits);
         QA_ARG_CHECK(need <= (size_t)maxLds, "qa_setFusion: a %d-qubit tile needs %zu bytes of LDS, the device allows %d",
                      tileQubits, need, maxLds);
         if (need > 65536) {

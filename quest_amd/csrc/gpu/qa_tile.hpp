@@ -1,0 +1,422 @@
+// Tile-pass programs shared by the gate-fusion queue (qa_fused.hip) and the
+// batched serving engine (qa_serve.hip): the device side that applies one
+// pass to one tile, and the host-side planner that turns a run of queued
+// gates into such a pass.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "qa_common.hpp"
+#include "qa_fused.hpp"
+
+namespace qa {
+namespace fuse {
+
+// ---- tile-space program -----------------------------------------------------
+//
+// One pass = one launch of k_tilePass over all 2^(n-k) tiles of the state.
+// A tile is the 2^k amplitudes that share the values of the n-k non-local
+// qubits; its k local qubits are numbered by "slots" 0..k-1 in ascending
+// qubit order, and slots 0..low-1 are the contiguous low qubits.
+//
+// Inside a tile each thread holds RA = 2^RB amplitudes in registers: those
+// whose tile indices differ only in the RB "register slots" of the current
+// group (the "layout").  A run of gates whose dense targets all lie in those
+// slots is applied in registers; a GROUP op changes the layout by writing
+// the registers to an LDS copy of the tile and gathering the new layout.
+// When the first (last) group's register slots avoid the low slots, its
+// layout is coalesced enough to load (store) HBM directly; otherwise the
+// tile goes through LDS in the "row" layout whose register slots are the
+// top RB slots.  Workgroups are persistent and prefetch their next tile into
+// registers before working on the current one, so HBM traffic overlaps the
+// register and LDS work.
+
+constexpr int RB = 4;        // register slots per group
+constexpr int RA = 1 << RB;  // amplitudes per thread
+enum { F_GROUP = 100 };
+
+// the first 64 bytes of an op, fetched with one scalar load
+struct TOpHead {
+    int kind;
+    int rb1, rb2;            // register bits of the dense targets
+    uint32_t rcMask, rcVal;  // control condition, register part (over v)
+    uint32_t rm1, rm2;       // register parts of m1 / m2
+    uint32_t pad;
+    uint64_t gcMask, gcVal;  // control condition, non-local part (tested against the tile base)
+    uint64_t jcMask, jcVal;  // control condition, tile part outside the register slots
+};
+static_assert(sizeof(TOpHead) == 64, "op header is one 64-byte scalar load");
+
+struct TOp {
+    TOpHead h;
+    uint64_t jm1, jm2;  // DIAG1 target / PARITY mask / PAULI maskXY (m1), maskYZ (m2): tile part
+    uint64_t gm1, gm2;  // ... non-local part
+    int slot[RB];       // GROUP: tile slots of register bits 0..RB-1 (ascending)
+    int pad[4];
+    double f[32];
+};
+
+struct TileArgs {
+    int k;            // tile bits
+    int numNonLocal;
+    int numOps;
+    int firstDirect;  // the first group loads its layout from HBM
+    int lastDirect;   // the last group stores its layout to HBM
+    int usesLds;      // some layout change goes through LDS
+    uint64_t numTiles;
+    int localPos[MAXQ];     // global positions of the tile slots (ascending)
+    int nonLocalPos[MAXQ];  // global positions of the other qubits (ascending)
+};
+
+__device__ __forceinline__ uint64_t deposit(uint64_t v, const int* pos, int num) {
+    uint64_t r = 0;
+    for (int i = 0; i < num; i++) r |= ((v >> i) & 1ull) << pos[i];
+    return r;
+}
+
+// LDS position of tile amplitude j: the low 4 index bits are XORed with the
+// next 4, so threads whose amplitudes are 16 apart (register slots 0..3)
+// still spread over all banks
+__device__ __forceinline__ uint32_t swz(uint32_t j) { return j ^ ((j >> 4) & 15u); }
+
+// a register layout: tile slots (sb) and global bits (qb) of the register
+// bits, and this thread's tile index (jb) and global offset (tg) with the
+// register bits zero
+struct RegLayout {
+    uint32_t sb[RB];
+    uint64_t qb[RB];
+    uint32_t jb;
+    uint64_t tg;
+};
+
+__device__ __forceinline__ void makeLayout(RegLayout& L, const int* slots, const int* localPos, int k, uint32_t tid) {
+    uint32_t gm = 0;
+#pragma unroll
+    for (int b = 0; b < RB; b++) {
+        L.sb[b] = 1u << slots[b];
+        L.qb[b] = 1ull << localPos[slots[b]];
+        gm |= L.sb[b];
+    }
+    // the thread id deposited over the other slots, ascending
+    L.jb = 0;
+    L.tg = 0;
+    for (int s = 0, i = 0; s < k; s++) {
+        if ((gm >> s) & 1u) continue;
+        const uint32_t bit = (tid >> i) & 1u;
+        L.jb |= bit << s;
+        L.tg |= (uint64_t)bit << localPos[s];
+        i++;
+    }
+}
+
+template <int V>
+__device__ __forceinline__ uint32_t offJ(const RegLayout& L) {
+    uint32_t o = 0;
+#pragma unroll
+    for (int b = 0; b < RB; b++)
+        if (V & (1 << b)) o |= L.sb[b];
+    return o;
+}
+template <int V>
+__device__ __forceinline__ uint64_t offQ(const RegLayout& L) {
+    uint64_t o = 0;
+#pragma unroll
+    for (int b = 0; b < RB; b++)
+        if (V & (1 << b)) o |= L.qb[b];
+    return o;
+}
+
+template <typename T, int NT, int V>
+struct RegIO {
+    __device__ __forceinline__ static void ldsLoad(ct<T>* r, const ct<T>* tile, const RegLayout& L) {
+        r[V] = tile[swz(L.jb | offJ<V>(L))];
+        RegIO<T, NT, V + 1>::ldsLoad(r, tile, L);
+    }
+    __device__ __forceinline__ static void ldsStore(const ct<T>* r, ct<T>* tile, const RegLayout& L) {
+        tile[swz(L.jb | offJ<V>(L))] = r[V];
+        RegIO<T, NT, V + 1>::ldsStore(r, tile, L);
+    }
+    __device__ __forceinline__ static void hbmLoad(ct<T>* r, const T* amps, uint64_t at, const RegLayout& L) {
+        r[V] = ldx<NT>(amps, at | offQ<V>(L));
+        RegIO<T, NT, V + 1>::hbmLoad(r, amps, at, L);
+    }
+    __device__ __forceinline__ static void hbmStore(const ct<T>* r, T* amps, uint64_t at, const RegLayout& L) {
+        stx<NT>(amps, at | offQ<V>(L), r[V]);
+        RegIO<T, NT, V + 1>::hbmStore(r, amps, at, L);
+    }
+};
+template <typename T, int NT>
+struct RegIO<T, NT, RA> {
+    __device__ __forceinline__ static void ldsLoad(ct<T>*, const ct<T>*, const RegLayout&) {}
+    __device__ __forceinline__ static void ldsStore(const ct<T>*, ct<T>*, const RegLayout&) {}
+    __device__ __forceinline__ static void hbmLoad(ct<T>*, const T*, uint64_t, const RegLayout&) {}
+    __device__ __forceinline__ static void hbmStore(const ct<T>*, T*, uint64_t, const RegLayout&) {}
+};
+
+__device__ __forceinline__ bool regCond(int v, const TOpHead& h) { return ((uint32_t)v & h.rcMask) == h.rcVal; }
+
+template <typename T>
+__device__ __forceinline__ ct<T> cf(const double* f, int i) {
+    return ct<T>{(T)f[2 * i], (T)f[2 * i + 1]};
+}
+
+// the arithmetic of each gate below is the expression of its single-gate
+// kernel (qa_gates.hip / qa_channels.hip), term for term
+
+template <typename T, int B>
+__device__ __forceinline__ void regDense1(ct<T>* r, const TOpHead& h, const double* f) {
+    const ct<T> m00 = cf<T>(f, 0), m01 = cf<T>(f, 1), m10 = cf<T>(f, 2), m11 = cf<T>(f, 3);
+#pragma unroll
+    for (int v = 0; v < RA; v++) {
+        if (v & (1 << B)) continue;
+        if (!regCond(v, h)) continue;
+        const ct<T> a0 = r[v], a1 = r[v | (1 << B)];
+        ct<T> b0 = cmulT(m00, a0);
+        b0 = cfmaT(m01, a1, b0);
+        ct<T> b1 = cmulT(m10, a0);
+        b1 = cfmaT(m11, a1, b1);
+        r[v] = b0;
+        r[v | (1 << B)] = b1;
+    }
+}
+
+template <typename T, int B1, int B2>
+__device__ __forceinline__ void regDense2(ct<T>* r, const TOpHead& h, const double* f) {
+    constexpr int off[4] = {0, 1 << B1, 1 << B2, (1 << B1) | (1 << B2)};
+#pragma unroll
+    for (int v = 0; v < RA; v++) {
+        if (v & ((1 << B1) | (1 << B2))) continue;
+        if (!regCond(v, h)) continue;
+        ct<T> x[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) x[c] = r[v | off[c]];
+#pragma unroll
+        for (int row = 0; row < 4; row++) {
+            ct<T> acc = cmulT(cf<T>(f, 4 * row), x[0]);
+#pragma unroll
+            for (int c = 1; c < 4; c++) acc = cfmaT(cf<T>(f, 4 * row + c), x[c], acc);
+            r[v | off[row]] = acc;
+        }
+    }
+}
+
+template <typename T, int B1, int B2>
+__device__ __forceinline__ void regSwap(ct<T>* r, const TOpHead& h, const double*) {
+#pragma unroll
+    for (int v = 0; v < RA; v++) {
+        if (!(v & (1 << B1)) || (v & (1 << B2))) continue;
+        if (!regCond(v, h)) continue;
+        constexpr int flip = (1 << B1) | (1 << B2);
+        const ct<T> x = r[v];
+        r[v] = r[v ^ flip];
+        r[v ^ flip] = x;
+    }
+}
+
+// one-qubit depolarising / dephasing on (ket, bra): both formulas are
+// symmetric in the two bits, so B1 < B2 covers either order
+template <typename T, int B1, int B2>
+__device__ __forceinline__ void regDepol1(ct<T>* r, const TOpHead&, const double* f) {
+    const T c1 = (T)f[0], c2 = (T)f[1], c3 = (T)f[2];
+#pragma unroll
+    for (int v = 0; v < RA; v++) {
+        if (v & ((1 << B1) | (1 << B2))) continue;
+        constexpr int k1 = 1 << B1, k2 = 1 << B2;
+        const ct<T> a00 = r[v], a01 = r[v | k1], a10 = r[v | k2], a11 = r[v | k1 | k2];
+        r[v] = ct<T>{fmaT(c1, a00.re, c2 * a11.re), fmaT(c1, a00.im, c2 * a11.im)};
+        r[v | k1] = ct<T>{c3 * a01.re, c3 * a01.im};
+        r[v | k2] = ct<T>{c3 * a10.re, c3 * a10.im};
+        r[v | k1 | k2] = ct<T>{fmaT(c1, a11.re, c2 * a00.re), fmaT(c1, a11.im, c2 * a00.im)};
+    }
+}
+
+template <typename T, int B1, int B2>
+__device__ __forceinline__ void regDeph1(ct<T>* r, const TOpHead&, const double* f) {
+    const T fac = (T)f[0];
+#pragma unroll
+    for (int v = 0; v < RA; v++) {
+        if (v & ((1 << B1) | (1 << B2))) continue;
+        constexpr int k1 = 1 << B1, k2 = 1 << B2;
+        const ct<T> x = r[v | k1], y = r[v | k2];
+        r[v | k1] = ct<T>{x.re * fac, x.im * fac};
+        r[v | k2] = ct<T>{y.re * fac, y.im * fac};
+    }
+}
+
+// Pauli tensor / gadget with X/Y bits MXY (all in registers); thrPar is the
+// parity of the maskYZ bits outside the registers for this thread
+template <typename T, int MXY>
+__device__ __forceinline__ void regPauli(ct<T>* r, const TOpHead& h, const double* f, int thrPar) {
+    constexpr int top = 31 - __builtin_clz((unsigned)MXY);
+    const ct<T> ampFac = cf<T>(f, 0), pairFac = cf<T>(f, 1);
+#pragma unroll
+    for (int v = 0; v < RA; v++) {
+        if (v & (1 << top)) continue;
+        if (!regCond(v, h)) continue;
+        const int w = v ^ MXY;
+        const ct<T> va = r[v], vb = r[w];
+        const T sA = ((__popc((uint32_t)v & h.rm2) & 1) ^ thrPar) ? (T)-1 : (T)1;
+        const T sB = ((__popc((uint32_t)w & h.rm2) & 1) ^ thrPar) ? (T)-1 : (T)1;
+        const ct<T> pB = {pairFac.re * sB, pairFac.im * sB};
+        const ct<T> pA = {pairFac.re * sA, pairFac.im * sA};
+        r[v] = cfmaT(pB, vb, cmulT(ampFac, va));
+        r[w] = cfmaT(pA, va, cmulT(ampFac, vb));
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void regDiag1(ct<T>* r, const TOpHead& h, const double* f, bool thrBit) {
+    const ct<T> d0 = cf<T>(f, 0), d1 = cf<T>(f, 1);
+#pragma unroll
+    for (int v = 0; v < RA; v++) {
+        if (!regCond(v, h)) continue;
+        const bool bit = h.rm1 ? (((uint32_t)v & h.rm1) != 0) : thrBit;
+        r[v] = cmulT(r[v], bit ? d1 : d0);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void regParity(ct<T>* r, const TOpHead& h, const double* f, int thrPar) {
+    const ct<T> f0 = cf<T>(f, 0), f1 = cf<T>(f, 1);
+#pragma unroll
+    for (int v = 0; v < RA; v++) {
+        if (!regCond(v, h)) continue;
+        r[v] = cmulT(r[v], ((__popc((uint32_t)v & h.rm1) & 1) ^ thrPar) ? f1 : f0);
+    }
+}
+
+#define QA_PAIR_CASES(FN)                                    \
+    case 0 * RB + 1: FN<T, 0, 1>(r, h, f); break;            \
+    case 0 * RB + 2: FN<T, 0, 2>(r, h, f); break;            \
+    case 0 * RB + 3: FN<T, 0, 3>(r, h, f); break;            \
+    case 1 * RB + 2: FN<T, 1, 2>(r, h, f); break;            \
+    case 1 * RB + 3: FN<T, 1, 3>(r, h, f); break;            \
+    case 2 * RB + 3: FN<T, 2, 3>(r, h, f); break;
+#define QA_PAIR_CASES_REV(FN)                                \
+    case 1 * RB + 0: FN<T, 1, 0>(r, h, f); break;            \
+    case 2 * RB + 0: FN<T, 2, 0>(r, h, f); break;            \
+    case 3 * RB + 0: FN<T, 3, 0>(r, h, f); break;            \
+    case 2 * RB + 1: FN<T, 2, 1>(r, h, f); break;            \
+    case 3 * RB + 1: FN<T, 3, 1>(r, h, f); break;            \
+    case 3 * RB + 2: FN<T, 3, 2>(r, h, f); break;
+
+// applies one non-GROUP op to the registers of one thread
+template <typename T>
+__device__ __forceinline__ void applyOp(ct<T>* r, const TOp* __restrict__ op, const TOpHead& h, uint32_t jb,
+                                        uint64_t base) {
+    const double* f = op->f;
+    switch (h.kind) {
+        case F_DENSE1:
+            switch (h.rb1) {
+                case 0: regDense1<T, 0>(r, h, f); break;
+                case 1: regDense1<T, 1>(r, h, f); break;
+                case 2: regDense1<T, 2>(r, h, f); break;
+                case 3: regDense1<T, 3>(r, h, f); break;
+            }
+            break;
+        case F_DENSE2:
+            switch (h.rb1 * RB + h.rb2) {
+                QA_PAIR_CASES(regDense2)
+                QA_PAIR_CASES_REV(regDense2)
+            }
+            break;
+        case F_SWAP:
+            switch (h.rb1 * RB + h.rb2) { QA_PAIR_CASES(regSwap) }
+            break;
+        case F_DEPOL1:
+            switch (h.rb1 * RB + h.rb2) { QA_PAIR_CASES(regDepol1) }
+            break;
+        case F_DEPH1:
+            switch (h.rb1 * RB + h.rb2) { QA_PAIR_CASES(regDeph1) }
+            break;
+        case F_DIAG1:
+            regDiag1<T>(r, h, f, ((jb & op->jm1) | (base & op->gm1)) != 0);
+            break;
+        case F_PARITY:
+            regParity<T>(r, h, f, parity64((jb & op->jm1) ^ (base & op->gm1)));
+            break;
+        case F_PAULI: {
+            const int thrPar = parity64((jb & op->jm2) ^ (base & op->gm2));
+            switch (h.rm1) {
+#define QA_PAULI_CASE(M) case M: regPauli<T, M>(r, h, f, thrPar); break;
+                QA_PAULI_CASE(1) QA_PAULI_CASE(2) QA_PAULI_CASE(3) QA_PAULI_CASE(4) QA_PAULI_CASE(5)
+                QA_PAULI_CASE(6) QA_PAULI_CASE(7) QA_PAULI_CASE(8) QA_PAULI_CASE(9) QA_PAULI_CASE(10)
+                QA_PAULI_CASE(11) QA_PAULI_CASE(12) QA_PAULI_CASE(13) QA_PAULI_CASE(14) QA_PAULI_CASE(15)
+#undef QA_PAULI_CASE
+            }
+            break;
+        }
+    }
+}
+
+// NT: non-temporal HBM loads/stores, used when the contiguous low run of a
+// tile row is at least 2^STREAM_LOW_BIT amplitudes
+// the tile's load layout: the first group's when it loads HBM directly,
+// else the row layout whose register slots are the top RB slots
+__device__ __forceinline__ void passLayouts(RegLayout& row, RegLayout& first, const TileArgs& a,
+                                            const TOp* __restrict__ ops, uint32_t tid) {
+    int rowSlots[RB];
+#pragma unroll
+    for (int b = 0; b < RB; b++) rowSlots[b] = a.k - RB + b;
+    makeLayout(row, rowSlots, a.localPos, a.k, tid);
+    makeLayout(first, ops[0].slot, a.localPos, a.k, tid);
+}
+
+// one tile of a pass: r holds the tile in its load layout; applies the
+// pass's ops in order and stores the tile
+template <typename T, int NT>
+__device__ __forceinline__ void tileBody(T* __restrict__ amps, const TileArgs& a, const TOp* __restrict__ ops,
+                                         uint64_t base, ct<T>* r, ct<T>* tile, const RegLayout& row,
+                                         const RegLayout& first, uint32_t tid) {
+    RegLayout cur = first;
+    bool inRegs = a.firstDirect;  // registers hold `cur` (else the row layout, not yet in LDS)
+    if (!a.firstDirect) RegIO<T, NT, 0>::ldsStore(r, tile, row);
+    for (int o = a.firstDirect ? 1 : 0; o < a.numOps; o++) {
+        const TOp* op = ops + o;
+        const TOpHead h = op->h;
+        if (h.kind == F_GROUP) {
+            if (inRegs) RegIO<T, NT, 0>::ldsStore(r, tile, cur);
+            __syncthreads();
+            makeLayout(cur, op->slot, a.localPos, a.k, tid);
+            RegIO<T, NT, 0>::ldsLoad(r, tile, cur);
+            inRegs = true;
+            continue;
+        }
+        if ((base & h.gcMask) != h.gcVal) continue;    // uniform over the workgroup
+        if ((cur.jb & h.jcMask) != h.jcVal) continue;  // uniform over the thread's registers
+        applyOp<T>(r, op, h, cur.jb, base);
+    }
+    if (a.lastDirect) {
+        RegIO<T, NT, 0>::hbmStore(r, amps, base | cur.tg, cur);
+    } else {
+        RegIO<T, NT, 0>::ldsStore(r, tile, cur);
+        __syncthreads();
+        RegIO<T, NT, 0>::ldsLoad(r, tile, row);
+        RegIO<T, NT, 0>::hbmStore(r, amps, base | row.tg, row);
+    }
+}
+
+// ---- host-side planner -------------------------------------------------------
+
+// resident workgroups of a kernel on the whole current device, for
+// persistent grids (cached per kernel, block size and LDS bytes)
+unsigned residentBlocks(const void* fn, unsigned block, size_t lds);
+
+// qubits whose amplitude pairs an op mixes (all must be local in its pass;
+// diagonal ops may involve any qubit)
+uint64_t denseMask(const GOp& g);
+
+// the longest prefix of ops[0..num) whose dense targets, with the `low`
+// lowest qubits, fit kMax local qubits; localMask is set to that local set
+// padded to exactly kMax qubits with the lowest free ones
+size_t takeRun(const GOp* ops, size_t num, int numQubits, int kMax, int low, uint64_t& localMask);
+
+// the program of one pass applying ops[0..num) (dense targets all inside
+// localMask) to a state of numQubits index bits: fills every field of `a`
+// and appends the ops, with their GROUP ops, to `prog`
+void planPass(const GOp* ops, size_t num, int numQubits, uint64_t localMask, int lowQubits, TileArgs& a,
+              std::vector<TOp>& prog);
+
+}  // namespace fuse
+}  // namespace qa

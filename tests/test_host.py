@@ -39,7 +39,7 @@ def declared_functions(header: Path) -> list[str]:
 
 
 @pytest.mark.parametrize("header,lib", [("quest_amd_gpu.h", GPU_LIB), ("quest.h", HOST_LIB),
-                                        ("quest_amd_ext.h", HOST_LIB)])
+                                        ("quest_amd_ext.h", HOST_LIB), ("quest_amd_serve.h", GPU_LIB)])
 def test_library_exports_every_declared_symbol(header, lib):
     assert lib.exists(), f"{lib} is not built (python -m quest_amd.build)"
     handle = ctypes.CDLL(str(lib))

@@ -172,10 +172,19 @@ TOp toTile(const GOp& g, const Layout& L) {
     h.rcVal = rv;
     h.jcVal = tv;
     h.gcVal = gv;
-    if (g.kind == F_DIAG1) splitMask(1ull << g.t1, L, h.rm1, t.jm1, t.gm1);
-    else splitMask(g.m1, L, h.rm1, t.jm1, t.gm1);
-    splitMask(g.m2, L, h.rm2, t.jm2, t.gm2);
-    std::memcpy(t.f, g.f, sizeof(t.f));
+    uint64_t jm1, gm1, jm2, gm2;
+    if (g.kind == F_DIAG1) splitMask(1ull << g.t1, L, h.rm1, jm1, gm1);
+    else splitMask(g.m1, L, h.rm1, jm1, gm1);
+    splitMask(g.m2, L, h.rm2, jm2, gm2);
+    std::memcpy(t.b, g.f, 8 * sizeof(double));
+    if (g.kind == F_DIAG1 || g.kind == F_PARITY) {
+        t.b[4] = jm1;
+        t.b[5] = gm1;
+    } else if (g.kind == F_PAULI) {
+        t.b[4] = jm2;
+        t.b[5] = gm2;
+    }
+    if (g.kind == F_DENSE2) std::memcpy(t.fx, g.f + 8, sizeof(t.fx));
     return t;
 }
 
@@ -248,9 +257,10 @@ void fuse::planPass(const GOp* ops, size_t num, int numQubits, uint64_t localMas
         std::memset(&grp, 0, sizeof(grp));
         grp.h.kind = F_GROUP;
         for (int sl = 0; sl < k; sl++) L.regOf[sl] = -1;
+        int* slots = reinterpret_cast<int*>(grp.b);
         for (int sl = 0, b = 0; sl < k; sl++)
             if ((G >> sl) & 1u) {
-                grp.slot[b] = sl;
+                slots[b] = sl;
                 L.regOf[sl] = b++;
             }
         prog.push_back(grp);

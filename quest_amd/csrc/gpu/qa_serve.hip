@@ -429,7 +429,7 @@ bool launchStep(Engine& E, StepBuf& sb, std::string& err) {
         std::memcpy(sb.host + at, kv.second.data(), kv.second.size() * sizeof(PassDesc));
         at += kv.second.size() * sizeof(PassDesc);
     }
-    const size_t opsAt = at;
+    const size_t opsAt = (at + 63) & ~size_t(63);  // ops are 64-byte aligned
     if (opBytes) std::memcpy(sb.host + opsAt, prog.data(), opBytes);
     const size_t jobsAt = (opsAt + opBytes + 63) & ~size_t(63);
     if (jobBytes) std::memcpy(sb.host + jobsAt, jobs.data(), jobBytes);

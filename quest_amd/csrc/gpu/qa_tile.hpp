@@ -48,14 +48,69 @@ struct TOpHead {
 };
 static_assert(sizeof(TOpHead) == 64, "op header is one 64-byte scalar load");
 
-struct TOp {
+// One op: a 64-byte header, a 64-byte body, both fetched with one 64-byte
+// scalar load each, and (for two-qubit dense matrices only) the rest of
+// the matrix.  Body by kind:
+//   DENSE1, DENSE2: b = matrix elements 0..3 (re, im interleaved)
+//   DIAG1, PARITY, PAULI: b[0..3] = two complex factors (d0, d1 / f0, f1 /
+//     ampFac, pairFac), b[4] = the tile part of the target bit (DIAG1), the
+//     parity mask (PARITY) or the maskYZ bits (PAULI) outside the
+//     registers, b[5] = its non-local part
+//   DEPOL1: b[0..2] = c1, c2, c3;  DEPH1: b[0] = factor
+//   GROUP: b[0..1] = the RB tile slots of the register bits (4 ints)
+struct alignas(64) TOp {
     TOpHead h;
-    uint64_t jm1, jm2;  // DIAG1 target / PARITY mask / PAULI maskXY (m1), maskYZ (m2): tile part
-    uint64_t gm1, gm2;  // ... non-local part
-    int slot[RB];       // GROUP: tile slots of register bits 0..RB-1 (ascending)
-    int pad[4];
-    double f[32];
+    uint64_t b[8];
+    double fx[24];  // DENSE2: matrix elements 4..15
 };
+static_assert(sizeof(TOp) % 64 == 0, "ops stay 64-byte aligned in arrays");
+
+typedef uint32_t u32x16 __attribute__((ext_vector_type(16)));
+
+template <int I>
+__device__ __forceinline__ uint64_t u64At(const u32x16& v) {
+    return ((uint64_t)v[2 * I + 1] << 32) | (uint64_t)v[2 * I];
+}
+template <int I>
+__device__ __forceinline__ double dAt(const u32x16& v) {
+    return __longlong_as_double((long long)u64At<I>(v));
+}
+
+// an op as fetched: header fields and body in scalar registers
+struct OpRegs {
+    TOpHead h;
+    double f[8];
+    uint64_t jm, gm;
+};
+
+__device__ __forceinline__ void fetchOp(const TOp* __restrict__ op, u32x16& h, u32x16& b) {
+    h = reinterpret_cast<const u32x16*>(op)[0];
+    b = reinterpret_cast<const u32x16*>(op)[1];
+}
+
+__device__ __forceinline__ void unpackOp(const u32x16& hv, const u32x16& bv, OpRegs& o) {
+    o.h.kind = (int)hv[0];
+    o.h.rb1 = (int)hv[1];
+    o.h.rb2 = (int)hv[2];
+    o.h.rcMask = hv[3];
+    o.h.rcVal = hv[4];
+    o.h.rm1 = hv[5];
+    o.h.rm2 = hv[6];
+    o.h.gcMask = u64At<4>(hv);
+    o.h.gcVal = u64At<5>(hv);
+    o.h.jcMask = u64At<6>(hv);
+    o.h.jcVal = u64At<7>(hv);
+    o.f[0] = dAt<0>(bv);
+    o.f[1] = dAt<1>(bv);
+    o.f[2] = dAt<2>(bv);
+    o.f[3] = dAt<3>(bv);
+    o.f[4] = dAt<4>(bv);
+    o.f[5] = dAt<5>(bv);
+    o.f[6] = dAt<6>(bv);
+    o.f[7] = dAt<7>(bv);
+    o.jm = u64At<4>(bv);
+    o.gm = u64At<5>(bv);
+}
 
 struct TileArgs {
     int k;            // tile bits
@@ -181,8 +236,25 @@ __device__ __forceinline__ void regDense1(ct<T>* r, const TOpHead& h, const doub
     }
 }
 
+// element i (0..15) of a 4x4 matrix whose elements 0..3 are in f and 4..15
+// in fx (memory)
+template <typename T, int I>
+__device__ __forceinline__ ct<T> m4(const double* f, const double* __restrict__ fx) {
+    if (I < 4) return ct<T>{(T)f[2 * I], (T)f[2 * I + 1]};
+    return ct<T>{(T)fx[2 * (I - 4)], (T)fx[2 * (I - 4) + 1]};
+}
+
+template <typename T, int B1, int B2, int ROW>
+__device__ __forceinline__ ct<T> dense2Row(const ct<T>* x, const double* f, const double* __restrict__ fx) {
+    ct<T> acc = cmulT(m4<T, 4 * ROW>(f, fx), x[0]);
+    acc = cfmaT(m4<T, 4 * ROW + 1>(f, fx), x[1], acc);
+    acc = cfmaT(m4<T, 4 * ROW + 2>(f, fx), x[2], acc);
+    acc = cfmaT(m4<T, 4 * ROW + 3>(f, fx), x[3], acc);
+    return acc;
+}
+
 template <typename T, int B1, int B2>
-__device__ __forceinline__ void regDense2(ct<T>* r, const TOpHead& h, const double* f) {
+__device__ __forceinline__ void regDense2(ct<T>* r, const TOpHead& h, const double* f, const double* __restrict__ fx) {
     constexpr int off[4] = {0, 1 << B1, 1 << B2, (1 << B1) | (1 << B2)};
 #pragma unroll
     for (int v = 0; v < RA; v++) {
@@ -191,13 +263,10 @@ __device__ __forceinline__ void regDense2(ct<T>* r, const TOpHead& h, const doub
         ct<T> x[4];
 #pragma unroll
         for (int c = 0; c < 4; c++) x[c] = r[v | off[c]];
-#pragma unroll
-        for (int row = 0; row < 4; row++) {
-            ct<T> acc = cmulT(cf<T>(f, 4 * row), x[0]);
-#pragma unroll
-            for (int c = 1; c < 4; c++) acc = cfmaT(cf<T>(f, 4 * row + c), x[c], acc);
-            r[v | off[row]] = acc;
-        }
+        r[v | off[0]] = dense2Row<T, B1, B2, 0>(x, f, fx);
+        r[v | off[1]] = dense2Row<T, B1, B2, 1>(x, f, fx);
+        r[v | off[2]] = dense2Row<T, B1, B2, 2>(x, f, fx);
+        r[v | off[3]] = dense2Row<T, B1, B2, 3>(x, f, fx);
     }
 }
 
@@ -293,19 +362,13 @@ __device__ __forceinline__ void regParity(ct<T>* r, const TOpHead& h, const doub
     case 1 * RB + 2: FN<T, 1, 2>(r, h, f); break;            \
     case 1 * RB + 3: FN<T, 1, 3>(r, h, f); break;            \
     case 2 * RB + 3: FN<T, 2, 3>(r, h, f); break;
-#define QA_PAIR_CASES_REV(FN)                                \
-    case 1 * RB + 0: FN<T, 1, 0>(r, h, f); break;            \
-    case 2 * RB + 0: FN<T, 2, 0>(r, h, f); break;            \
-    case 3 * RB + 0: FN<T, 3, 0>(r, h, f); break;            \
-    case 2 * RB + 1: FN<T, 2, 1>(r, h, f); break;            \
-    case 3 * RB + 1: FN<T, 3, 1>(r, h, f); break;            \
-    case 3 * RB + 2: FN<T, 3, 2>(r, h, f); break;
 
 // applies one non-GROUP op to the registers of one thread
 template <typename T>
-__device__ __forceinline__ void applyOp(ct<T>* r, const TOp* __restrict__ op, const TOpHead& h, uint32_t jb,
+__device__ __forceinline__ void applyOp(ct<T>* r, const TOp* __restrict__ op, const OpRegs& o, uint32_t jb,
                                         uint64_t base) {
-    const double* f = op->f;
+    const TOpHead& h = o.h;
+    const double* f = o.f;
     switch (h.kind) {
         case F_DENSE1:
             switch (h.rb1) {
@@ -315,12 +378,16 @@ __device__ __forceinline__ void applyOp(ct<T>* r, const TOp* __restrict__ op, co
                 case 3: regDense1<T, 3>(r, h, f); break;
             }
             break;
-        case F_DENSE2:
+        case F_DENSE2: {
+            const double* fx = op->fx;
             switch (h.rb1 * RB + h.rb2) {
-                QA_PAIR_CASES(regDense2)
-                QA_PAIR_CASES_REV(regDense2)
+#define QA_D2(B1, B2) case B1 * RB + B2: regDense2<T, B1, B2>(r, h, f, fx); break;
+                QA_D2(0, 1) QA_D2(0, 2) QA_D2(0, 3) QA_D2(1, 2) QA_D2(1, 3) QA_D2(2, 3)
+                QA_D2(1, 0) QA_D2(2, 0) QA_D2(3, 0) QA_D2(2, 1) QA_D2(3, 1) QA_D2(3, 2)
+#undef QA_D2
             }
             break;
+        }
         case F_SWAP:
             switch (h.rb1 * RB + h.rb2) { QA_PAIR_CASES(regSwap) }
             break;
@@ -331,13 +398,13 @@ __device__ __forceinline__ void applyOp(ct<T>* r, const TOp* __restrict__ op, co
             switch (h.rb1 * RB + h.rb2) { QA_PAIR_CASES(regDeph1) }
             break;
         case F_DIAG1:
-            regDiag1<T>(r, h, f, ((jb & op->jm1) | (base & op->gm1)) != 0);
+            regDiag1<T>(r, h, f, ((jb & o.jm) | (base & o.gm)) != 0);
             break;
         case F_PARITY:
-            regParity<T>(r, h, f, parity64((jb & op->jm1) ^ (base & op->gm1)));
+            regParity<T>(r, h, f, parity64((jb & o.jm) ^ (base & o.gm)));
             break;
         case F_PAULI: {
-            const int thrPar = parity64((jb & op->jm2) ^ (base & op->gm2));
+            const int thrPar = parity64((jb & o.jm) ^ (base & o.gm));
             switch (h.rm1) {
 #define QA_PAULI_CASE(M) case M: regPauli<T, M>(r, h, f, thrPar); break;
                 QA_PAULI_CASE(1) QA_PAULI_CASE(2) QA_PAULI_CASE(3) QA_PAULI_CASE(4) QA_PAULI_CASE(5)
@@ -350,8 +417,8 @@ __device__ __forceinline__ void applyOp(ct<T>* r, const TOp* __restrict__ op, co
     }
 }
 
-// NT: non-temporal HBM loads/stores, used when the contiguous low run of a
-// tile row is at least 2^STREAM_LOW_BIT amplitudes
+__device__ __forceinline__ const int* groupSlots(const TOp& op) { return reinterpret_cast<const int*>(op.b); }
+
 // the tile's load layout: the first group's when it loads HBM directly,
 // else the row layout whose register slots are the top RB slots
 __device__ __forceinline__ void passLayouts(RegLayout& row, RegLayout& first, const TileArgs& a,
@@ -360,7 +427,7 @@ __device__ __forceinline__ void passLayouts(RegLayout& row, RegLayout& first, co
 #pragma unroll
     for (int b = 0; b < RB; b++) rowSlots[b] = a.k - RB + b;
     makeLayout(row, rowSlots, a.localPos, a.k, tid);
-    makeLayout(first, ops[0].slot, a.localPos, a.k, tid);
+    makeLayout(first, groupSlots(ops[0]), a.localPos, a.k, tid);
 }
 
 // one tile of a pass: r holds the tile in its load layout; applies the
@@ -372,20 +439,30 @@ __device__ __forceinline__ void tileBody(T* __restrict__ amps, const TileArgs& a
     RegLayout cur = first;
     bool inRegs = a.firstDirect;  // registers hold `cur` (else the row layout, not yet in LDS)
     if (!a.firstDirect) RegIO<T, NT, 0>::ldsStore(r, tile, row);
-    for (int o = a.firstDirect ? 1 : 0; o < a.numOps; o++) {
-        const TOp* op = ops + o;
-        const TOpHead h = op->h;
-        if (h.kind == F_GROUP) {
+    // each op's two 64-byte records are fetched one op ahead, so the scalar
+    // load latency overlaps the previous op's arithmetic
+    int o = a.firstDirect ? 1 : 0;
+    u32x16 ch, cb, nh, nb;
+    if (o < a.numOps) fetchOp(ops + o, ch, cb);
+    for (; o < a.numOps; o++) {
+        if (o + 1 < a.numOps) fetchOp(ops + o + 1, nh, nb);
+        OpRegs op;
+        unpackOp(ch, cb, op);
+        if (op.h.kind == F_GROUP) {
+            int slots[RB];
+#pragma unroll
+            for (int b = 0; b < RB; b++) slots[b] = (int)cb[b];
             if (inRegs) RegIO<T, NT, 0>::ldsStore(r, tile, cur);
             __syncthreads();
-            makeLayout(cur, op->slot, a.localPos, a.k, tid);
+            makeLayout(cur, slots, a.localPos, a.k, tid);
             RegIO<T, NT, 0>::ldsLoad(r, tile, cur);
             inRegs = true;
-            continue;
+        } else if ((base & op.h.gcMask) == op.h.gcVal &&   // uniform over the workgroup
+                   (cur.jb & op.h.jcMask) == op.h.jcVal) {  // uniform over the thread's registers
+            applyOp<T>(r, ops + o, op, cur.jb, base);
         }
-        if ((base & h.gcMask) != h.gcVal) continue;    // uniform over the workgroup
-        if ((cur.jb & h.jcMask) != h.jcVal) continue;  // uniform over the thread's registers
-        applyOp<T>(r, op, h, cur.jb, base);
+        ch = nh;
+        cb = nb;
     }
     if (a.lastDirect) {
         RegIO<T, NT, 0>::hbmStore(r, amps, base | cur.tg, cur);

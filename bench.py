@@ -198,9 +198,212 @@ def timed_steps(Q, gpu, calls, steps: int) -> float:
     return ms.value
 
 
+def serve_bench(args) -> int:
+    """Serving workload: `args.circuits` variational requests (qubits
+    U{10..24}, depth U{4..32}, RY+RZ on every qubit then a CZ ladder per
+    layer, 8 Pauli expectations each) arriving as a Poisson stream at
+    `args.rate` per second, served by the batched complex64 engine;
+    compared with running a sample of the same requests one at a time as
+    complex128 quregs through the quest.h API, one kernel per gate."""
+    import ctypes as C
+    import threading
+
+    import numpy as np
+
+    from quest_amd import api as Q, circuits as qc, serve as S
+
+    Q.ensure_env()
+    reqs = S.serving_requests(args.circuits, args.seed)
+    packed = []
+    for c in reqs:
+        gates, ng = S.pack_gates(c.ops)
+        masks = [S.pauli_masks(ch, t) for ch, t in S.paulis_of(c)]
+        xs = (C.c_uint64 * len(masks))(*[m[0] for m in masks])
+        ys = (C.c_uint64 * len(masks))(*[m[1] for m in masks])
+        zs = (C.c_uint64 * len(masks))(*[m[2] for m in masks])
+        packed.append((S.ServeCircuit(c.num_qubits, ng, gates, len(masks), xs, ys, zs), gates, xs, ys, zs))
+    rng = np.random.default_rng(args.seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, size=len(reqs)))
+    total_gates = sum(qc.count_gates(c) for c in reqs)
+    log(f"serve: {len(reqs)} requests, {total_gates} gates, Poisson rate {args.rate}/s "
+        f"(last arrival at {arrivals[-1]:.2f} s)")
+    lib = S._lib()
+    eng = S.Engine(args.arena_gib << 30, max_active=args.max_active, tile_qubits=args.tile, low_qubits=4,
+                   precision=1)
+    # warm the engine's kernels on a few requests outside the measurement
+    for c in reqs[:4]:
+        eng.wait(eng.submit(c.num_qubits, c.ops, S.paulis_of(c)))
+    ids = [0] * len(reqs)
+    t0 = time.perf_counter()
+
+    def feeder():
+        for i, (circ, *_keep) in enumerate(packed):
+            delay = t0 + arrivals[i] - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            rid = C.c_int64()
+            rc = lib.qa_serveSubmit(eng._h, C.byref(circ), C.byref(rid))
+            if rc != 0:
+                raise RuntimeError(lib.qa_lastError().decode())
+            ids[i] = rid.value
+
+    th = threading.Thread(target=feeder)
+    th.start()
+    th.join()
+    results, lat = [], []
+    out = np.zeros(8)
+    L = C.c_double()
+    for rid in ids:
+        rc = lib.qa_serveWait(eng._h, C.c_int64(rid), out.ctypes.data_as(C.POINTER(C.c_double)), C.byref(L))
+        if rc != 0:
+            raise RuntimeError(lib.qa_lastError().decode())
+        results.append(out.copy())
+        lat.append(L.value)
+    wall = time.perf_counter() - t0
+    st = eng.stats()
+    eng.close()
+    lat = np.array(lat)
+    value = len(reqs) / wall
+    log(f"served {len(reqs)} in {wall:.2f} s = {value:.1f} circuits/s; latency p50 {np.percentile(lat, 50):.2f} ms, "
+        f"p99 {np.percentile(lat, 99):.2f} ms; engine {st}")
+
+    # baseline: a stratified sample of the same requests, one at a time,
+    # complex128 quregs through quest.h with fusion off (one kernel per gate)
+    order = sorted(range(len(reqs)), key=lambda i: reqs[i].num_qubits)
+    sample = [order[int(j * (len(order) - 1) / (args.baseline_sample - 1))] for j in range(args.baseline_sample)]
+    fusion = Q.get_fusion()
+    Q.set_fusion(False)
+    base_t, max_err = 0.0, 0.0
+    for i in sample:
+        c = reqs[i]
+        q = Q.createQureg(c.num_qubits)
+        Q.syncQuESTEnv()
+        t = time.perf_counter()
+        _, res = qc.run(c, q)
+        Q.syncQuESTEnv()
+        base_t += time.perf_counter() - t
+        Q.destroyQureg(q)
+        want = np.array([v for (lab, v) in res if lab == "expec"])
+        max_err = max(max_err, float(np.max(np.abs(results[i] - want))))
+    Q.set_fusion(bool(fusion[0]), *fusion[1:])
+    # the sample's mean time per request, weighted to the full stream's mix:
+    # every request of the stream is represented by its nearest-size sample
+    mean_sample = base_t / len(sample)
+    baseline = 1.0 / mean_sample
+    log(f"baseline (sequential complex128, one kernel per gate): {len(sample)} requests in {base_t:.2f} s "
+        f"= {baseline:.2f} circuits/s; max |expectation error| complex64 engine vs complex128: {max_err:.2e}")
+    print(json.dumps({
+        "metric": "batched circuits/s + p99 latency",
+        "value": value,
+        "unit": "circuits/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "c64 amplitudes, f64 reductions",
+        "data": "synthetic",
+        "config": {"workload": (f"{len(reqs)} variational requests, qubits U{{10..24}}, depth U{{4..32}}, "
+                                "RY+RZ on every qubit then a CZ ladder per layer, 8 Pauli expectations each, "
+                                f"Poisson arrivals at {args.rate}/s"),
+                   "arena_gib": args.arena_gib, "max_active": args.max_active, "tile_qubits": args.tile},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "max": float(lat.max())},
+        "gates_per_s": total_gates / wall,
+        "engine": st,
+        "baseline": {"value": baseline, "unit": "circuits/s",
+                     "kind": "sequential complex128 quregs through quest.h, one kernel per gate",
+                     "sample": f"{len(sample)} requests spread over the qubit-count range, {base_t:.2f} s"},
+        "speedup_vs_baseline": value / baseline,
+        "max_abs_expectation_error": max_err,
+    }), flush=True)
+    return 0
+
+
+def precision_bench(args) -> int:
+    """Precision form: the plain-method circuit family (20 layers of a Haar
+    applyCompMatr1 on every qubit then applyTwoQubitPhaseShift on n/2
+    disjoint random pairs; probabilities of every qubit and one Pauli
+    expectation) on a complex64 statevector at n qubits (default 33, 64 GiB)
+    against the same circuit on complex128 at n qubits (128 GiB); then the
+    error of complex64 against complex128 at 30 qubits on the same initial
+    state: state infidelity and the largest absolute error of the outputs."""
+    from quest_amd import api as Q, circuits as qc
+    import numpy as np
+
+    Q.ensure_env()
+    n = args.qubits if args.qubits != 30 else 33
+    depth = args.depth if args.depth != 40 else 20
+
+    def family(m):
+        return qc.layered(m, depth, args.seed, ctrl3=False, qft=False, gadgets=0, num_expec=1, probs=True)
+
+    def run_calls(calls):
+        return [f(*a) for f, a in calls]
+
+    gpu = Q.gpu()
+    import ctypes as C
+    circ = family(n)
+    gates = qc.count_gates(circ)
+    rates = {}
+    for prec in (1, 2):
+        q = Q.createCustomPrecisionQureg(n, 0, prec)
+        Q.initRandomPureState(q)
+        calls = build_step(Q, qc, circ, q)
+        run_calls(calls)
+        Q.syncQuESTEnv()
+        gpu.qa_setFusionTiming(1)
+        ms = timed_steps(Q, gpu, calls, 1)
+        passes, fg, pms = C.c_int64(), C.c_int64(), C.c_double()
+        gpu.qa_getFusionStats(C.byref(passes), C.byref(fg), C.byref(pms))
+        gpu.qa_setFusionTiming(0)
+        Q.destroyQureg(q)
+        amp = 8 if prec == 1 else 16
+        avg = pms.value / max(1, passes.value)
+        rates[prec] = {"gates_per_s": gates / (ms / 1e3), "ms_per_step": ms, "passes": passes.value,
+                       "pass_avg_ms": avg, "pass_GBps": 2 * amp * (1 << n) / (avg * 1e-3) / 1e9 if passes.value else None}
+        log(f"{n} qubits {'complex64' if prec == 1 else 'complex128'}: {ms:.1f} ms = "
+            f"{rates[prec]['gates_per_s']:.1f} gates/s, {passes.value} passes, {avg:.2f} ms each")
+
+    # error at 30 qubits: same initial state, same circuit, both precisions
+    m = 30
+    c30 = family(m)
+    q64 = Q.createCustomPrecisionQureg(m, 0, 2)
+    q32 = Q.createCustomPrecisionQureg(m, 0, 1)
+    Q.initRandomPureState(q64)
+    Q.setQuregToClone(q32, q64)
+    out64 = [v for v in run_calls(build_step(Q, qc, c30, q64)) if v is not None]
+    out32 = [v for v in run_calls(build_step(Q, qc, c30, q32)) if v is not None]
+    # the complex64 result widened to complex128, then <psi64|psi32> on the GPU
+    q32w = Q.createCustomPrecisionQureg(m, 0, 2)
+    Q.setQuregToClone(q32w, q32)
+    Q.destroyQureg(q32)
+    ip = Q.inner_product(q64, q32w)
+    infidelity = 1.0 - abs(ip) ** 2
+    Q.destroyQureg(q64)
+    Q.destroyQureg(q32w)
+    err_out = float(np.max(np.abs(np.array(out32) - np.array(out64))))
+    log(f"30 qubits: infidelity {infidelity:.2e}, max |output error| complex64 vs complex128 = {err_out:.2e}")
+    print(json.dumps({
+        "metric": f"{n}-qubit complex64 gates/s (fused), against complex128",
+        "value": rates[1]["gates_per_s"], "unit": "gates/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "c64 amplitudes, f64 reductions", "data": "synthetic",
+        "config": {"workload": (f"{n}-qubit statevector, {depth} layers x ({n} Haar applyCompMatr1 + {n // 2} "
+                                f"applyTwoQubitPhaseShift); {n} calcProbOfQubitOutcome + 1 calcExpecPauliStr"),
+                   "gates_per_step": gates},
+        "complex64": rates[1], "complex128": rates[2],
+        "error_at_30_qubits": {"state_infidelity": infidelity, "max_abs_output_error": err_out,
+                               "reference": "the same circuit on a complex128 qureg from the same initial state"},
+    }), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--mode", choices=["circuit", "density", "sweep"], default="circuit",
+    ap.add_argument("--rate", type=float, default=2500.0, help="serve mode: Poisson arrival rate (1/s)")
+    ap.add_argument("--circuits", type=int, default=4096, help="serve mode: number of requests")
+    ap.add_argument("--arena-gib", type=int, default=48, help="serve mode: HBM arena of the engine")
+    ap.add_argument("--max-active", type=int, default=256, help="serve mode: requests running at once")
+    ap.add_argument("--tile", type=int, default=12, help="serve mode: tile qubits")
+    ap.add_argument("--baseline-sample", type=int, default=24, help="serve mode: requests run sequentially")
+    ap.add_argument("--mode", choices=["circuit", "density", "serve", "precision", "sweep"], default="circuit",
                     help="circuit: the headline 30-qubit statevector workload; density: the 14-qubit noisy "
                          "density-matrix workload; sweep: per-target dense-kernel timing + copy calibration")
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,6 +422,11 @@ def main() -> int:
         print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported by this single-GPU build "
               "(a multi-GPU statevector would shard by its highest qubits)", file=sys.stderr)
         return 2
+
+    if args.mode == "serve":
+        return serve_bench(args)
+    if args.mode == "precision":
+        return precision_bench(args)
 
     from quest_amd import api as Q, circuits as qc
 

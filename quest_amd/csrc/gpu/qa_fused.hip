@@ -162,8 +162,9 @@ TOp toTile(const GOp& g, const Layout& L) {
     std::memset(&t, 0, sizeof(t));
     TOpHead& h = t.h;
     h.kind = g.kind;
-    h.rb1 = g.t1 >= 0 ? L.regOf[L.slotOf[g.t1]] : -1;
-    h.rb2 = g.t2 >= 0 ? L.regOf[L.slotOf[g.t2]] : -1;
+    auto regBit = [&](int q) { return (q >= 0 && L.slotOf[q] >= 0) ? L.regOf[L.slotOf[q]] : -1; };
+    h.rb1 = regBit(g.t1);
+    h.rb2 = regBit(g.t2);
     if ((g.kind == F_SWAP || g.kind == F_DEPOL1 || g.kind == F_DEPH1) && h.rb1 > h.rb2) std::swap(h.rb1, h.rb2);
     uint32_t rv;
     uint64_t tv, gv;
@@ -185,6 +186,13 @@ TOp toTile(const GOp& g, const Layout& L) {
         t.b[5] = gm2;
     }
     if (g.kind == F_DENSE2) std::memcpy(t.fx, g.f + 8, sizeof(t.fx));
+    if (g.kind == F_DIAG1 && h.rb1 >= 0 && ((h.rcMask >> h.rb1) & 1u)) {
+        // phase-only diagonal (its target is one of its controls, set to 1):
+        // the register kernel skips the amplitudes with the bit clear itself
+        h.rcMask &= ~(1u << h.rb1);
+        h.rcVal &= ~(1u << h.rb1);
+        h.pad = 1;
+    }
     return t;
 }
 

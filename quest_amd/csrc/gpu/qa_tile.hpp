@@ -42,7 +42,7 @@ struct TOpHead {
     int rb1, rb2;            // register bits of the dense targets
     uint32_t rcMask, rcVal;  // control condition, register part (over v)
     uint32_t rm1, rm2;       // register parts of m1 / m2
-    uint32_t pad;
+    uint32_t pad;            // DIAG1 with its target in the registers: 1 = phase-only
     uint64_t gcMask, gcVal;  // control condition, non-local part (tested against the tile base)
     uint64_t jcMask, jcVal;  // control condition, tile part outside the register slots
 };
@@ -96,6 +96,7 @@ __device__ __forceinline__ void unpackOp(const u32x16& hv, const u32x16& bv, OpR
     o.h.rcVal = hv[4];
     o.h.rm1 = hv[5];
     o.h.rm2 = hv[6];
+    o.h.pad = hv[7];
     o.h.gcMask = u64At<4>(hv);
     o.h.gcVal = u64At<5>(hv);
     o.h.jcMask = u64At<6>(hv);
@@ -209,7 +210,13 @@ struct RegIO<T, NT, RA> {
     __device__ __forceinline__ static void hbmStore(const ct<T>*, T*, uint64_t, const RegLayout&) {}
 };
 
-__device__ __forceinline__ bool regCond(int v, const TOpHead& h) { return ((uint32_t)v & h.rcMask) == h.rcVal; }
+// register part of an op's control condition; C = false when the op has no
+// control among the register bits (no per-amplitude test is compiled in,
+// so updates happen in place)
+template <bool C>
+__device__ __forceinline__ bool regCond(int v, const TOpHead& h) {
+    return !C || (((uint32_t)v & h.rcMask) == h.rcVal);
+}
 
 template <typename T>
 __device__ __forceinline__ ct<T> cf(const double* f, int i) {
@@ -219,13 +226,13 @@ __device__ __forceinline__ ct<T> cf(const double* f, int i) {
 // the arithmetic of each gate below is the expression of its single-gate
 // kernel (qa_gates.hip / qa_channels.hip), term for term
 
-template <typename T, int B>
+template <typename T, int B, bool C>
 __device__ __forceinline__ void regDense1(ct<T>* r, const TOpHead& h, const double* f) {
     const ct<T> m00 = cf<T>(f, 0), m01 = cf<T>(f, 1), m10 = cf<T>(f, 2), m11 = cf<T>(f, 3);
 #pragma unroll
     for (int v = 0; v < RA; v++) {
         if (v & (1 << B)) continue;
-        if (!regCond(v, h)) continue;
+        if (!regCond<C>(v, h)) continue;
         const ct<T> a0 = r[v], a1 = r[v | (1 << B)];
         ct<T> b0 = cmulT(m00, a0);
         b0 = cfmaT(m01, a1, b0);
@@ -244,7 +251,7 @@ __device__ __forceinline__ ct<T> m4(const double* f, const double* __restrict__ 
     return ct<T>{(T)fx[2 * (I - 4)], (T)fx[2 * (I - 4) + 1]};
 }
 
-template <typename T, int B1, int B2, int ROW>
+template <typename T, int ROW>
 __device__ __forceinline__ ct<T> dense2Row(const ct<T>* x, const double* f, const double* __restrict__ fx) {
     ct<T> acc = cmulT(m4<T, 4 * ROW>(f, fx), x[0]);
     acc = cfmaT(m4<T, 4 * ROW + 1>(f, fx), x[1], acc);
@@ -253,29 +260,29 @@ __device__ __forceinline__ ct<T> dense2Row(const ct<T>* x, const double* f, cons
     return acc;
 }
 
-template <typename T, int B1, int B2>
+template <typename T, int B1, int B2, bool C>
 __device__ __forceinline__ void regDense2(ct<T>* r, const TOpHead& h, const double* f, const double* __restrict__ fx) {
     constexpr int off[4] = {0, 1 << B1, 1 << B2, (1 << B1) | (1 << B2)};
 #pragma unroll
     for (int v = 0; v < RA; v++) {
         if (v & ((1 << B1) | (1 << B2))) continue;
-        if (!regCond(v, h)) continue;
+        if (!regCond<C>(v, h)) continue;
         ct<T> x[4];
 #pragma unroll
         for (int c = 0; c < 4; c++) x[c] = r[v | off[c]];
-        r[v | off[0]] = dense2Row<T, B1, B2, 0>(x, f, fx);
-        r[v | off[1]] = dense2Row<T, B1, B2, 1>(x, f, fx);
-        r[v | off[2]] = dense2Row<T, B1, B2, 2>(x, f, fx);
-        r[v | off[3]] = dense2Row<T, B1, B2, 3>(x, f, fx);
+        r[v | off[0]] = dense2Row<T, 0>(x, f, fx);
+        r[v | off[1]] = dense2Row<T, 1>(x, f, fx);
+        r[v | off[2]] = dense2Row<T, 2>(x, f, fx);
+        r[v | off[3]] = dense2Row<T, 3>(x, f, fx);
     }
 }
 
-template <typename T, int B1, int B2>
+template <typename T, int B1, int B2, bool C>
 __device__ __forceinline__ void regSwap(ct<T>* r, const TOpHead& h, const double*) {
 #pragma unroll
     for (int v = 0; v < RA; v++) {
         if (!(v & (1 << B1)) || (v & (1 << B2))) continue;
-        if (!regCond(v, h)) continue;
+        if (!regCond<C>(v, h)) continue;
         constexpr int flip = (1 << B1) | (1 << B2);
         const ct<T> x = r[v];
         r[v] = r[v ^ flip];
@@ -284,8 +291,9 @@ __device__ __forceinline__ void regSwap(ct<T>* r, const TOpHead& h, const double
 }
 
 // one-qubit depolarising / dephasing on (ket, bra): both formulas are
-// symmetric in the two bits, so B1 < B2 covers either order
-template <typename T, int B1, int B2>
+// symmetric in the two bits, so B1 < B2 covers either order (never
+// controlled)
+template <typename T, int B1, int B2, bool C>
 __device__ __forceinline__ void regDepol1(ct<T>* r, const TOpHead&, const double* f) {
     const T c1 = (T)f[0], c2 = (T)f[1], c3 = (T)f[2];
 #pragma unroll
@@ -300,7 +308,7 @@ __device__ __forceinline__ void regDepol1(ct<T>* r, const TOpHead&, const double
     }
 }
 
-template <typename T, int B1, int B2>
+template <typename T, int B1, int B2, bool C>
 __device__ __forceinline__ void regDeph1(ct<T>* r, const TOpHead&, const double* f) {
     const T fac = (T)f[0];
 #pragma unroll
@@ -315,14 +323,14 @@ __device__ __forceinline__ void regDeph1(ct<T>* r, const TOpHead&, const double*
 
 // Pauli tensor / gadget with X/Y bits MXY (all in registers); thrPar is the
 // parity of the maskYZ bits outside the registers for this thread
-template <typename T, int MXY>
+template <typename T, int MXY, bool C>
 __device__ __forceinline__ void regPauli(ct<T>* r, const TOpHead& h, const double* f, int thrPar) {
     constexpr int top = 31 - __builtin_clz((unsigned)MXY);
     const ct<T> ampFac = cf<T>(f, 0), pairFac = cf<T>(f, 1);
 #pragma unroll
     for (int v = 0; v < RA; v++) {
         if (v & (1 << top)) continue;
-        if (!regCond(v, h)) continue;
+        if (!regCond<C>(v, h)) continue;
         const int w = v ^ MXY;
         const ct<T> va = r[v], vb = r[w];
         const T sA = ((__popc((uint32_t)v & h.rm2) & 1) ^ thrPar) ? (T)-1 : (T)1;
@@ -334,54 +342,70 @@ __device__ __forceinline__ void regPauli(ct<T>* r, const TOpHead& h, const doubl
     }
 }
 
-template <typename T>
-__device__ __forceinline__ void regDiag1(ct<T>* r, const TOpHead& h, const double* f, bool thrBit) {
+// one-target diagonal whose target is register bit B; PH: phase-only (the
+// amplitudes with bit B clear are left as they are, as in the single-gate
+// kernel, which never touches them)
+template <typename T, int B, bool C, bool PH>
+__device__ __forceinline__ void regDiag1R(ct<T>* r, const TOpHead& h, const double* f) {
     const ct<T> d0 = cf<T>(f, 0), d1 = cf<T>(f, 1);
 #pragma unroll
     for (int v = 0; v < RA; v++) {
-        if (!regCond(v, h)) continue;
-        const bool bit = h.rm1 ? (((uint32_t)v & h.rm1) != 0) : thrBit;
-        r[v] = cmulT(r[v], bit ? d1 : d0);
+        if (PH && !(v & (1 << B))) continue;
+        if (!regCond<C>(v, h)) continue;
+        r[v] = cmulT(r[v], (v & (1 << B)) ? d1 : d0);
     }
 }
 
-template <typename T>
+// one-target diagonal whose target is outside the registers: one factor
+// for all of this thread's amplitudes
+template <typename T, bool C>
+__device__ __forceinline__ void regDiag1T(ct<T>* r, const TOpHead& h, const double* f, bool thrBit) {
+    const ct<T> d = thrBit ? cf<T>(f, 1) : cf<T>(f, 0);
+#pragma unroll
+    for (int v = 0; v < RA; v++) {
+        if (!regCond<C>(v, h)) continue;
+        r[v] = cmulT(r[v], d);
+    }
+}
+
+template <typename T, bool C>
 __device__ __forceinline__ void regParity(ct<T>* r, const TOpHead& h, const double* f, int thrPar) {
     const ct<T> f0 = cf<T>(f, 0), f1 = cf<T>(f, 1);
 #pragma unroll
     for (int v = 0; v < RA; v++) {
-        if (!regCond(v, h)) continue;
+        if (!regCond<C>(v, h)) continue;
         r[v] = cmulT(r[v], ((__popc((uint32_t)v & h.rm1) & 1) ^ thrPar) ? f1 : f0);
     }
 }
 
-#define QA_PAIR_CASES(FN)                                    \
-    case 0 * RB + 1: FN<T, 0, 1>(r, h, f); break;            \
-    case 0 * RB + 2: FN<T, 0, 2>(r, h, f); break;            \
-    case 0 * RB + 3: FN<T, 0, 3>(r, h, f); break;            \
-    case 1 * RB + 2: FN<T, 1, 2>(r, h, f); break;            \
-    case 1 * RB + 3: FN<T, 1, 3>(r, h, f); break;            \
-    case 2 * RB + 3: FN<T, 2, 3>(r, h, f); break;
+#define QA_PAIR_CASES(FN)                                      \
+    case 0 * RB + 1: FN<T, 0, 1, C>(r, h, f); break;           \
+    case 0 * RB + 2: FN<T, 0, 2, C>(r, h, f); break;           \
+    case 0 * RB + 3: FN<T, 0, 3, C>(r, h, f); break;           \
+    case 1 * RB + 2: FN<T, 1, 2, C>(r, h, f); break;           \
+    case 1 * RB + 3: FN<T, 1, 3, C>(r, h, f); break;           \
+    case 2 * RB + 3: FN<T, 2, 3, C>(r, h, f); break;
 
-// applies one non-GROUP op to the registers of one thread
-template <typename T>
-__device__ __forceinline__ void applyOp(ct<T>* r, const TOp* __restrict__ op, const OpRegs& o, uint32_t jb,
-                                        uint64_t base) {
+// applies one non-GROUP op to the registers of one thread; C = the op has
+// controls among the register bits
+template <typename T, bool C>
+__device__ __forceinline__ void applyOpC(ct<T>* r, const TOp* __restrict__ op, const OpRegs& o, uint32_t jb,
+                                         uint64_t base) {
     const TOpHead& h = o.h;
     const double* f = o.f;
     switch (h.kind) {
         case F_DENSE1:
             switch (h.rb1) {
-                case 0: regDense1<T, 0>(r, h, f); break;
-                case 1: regDense1<T, 1>(r, h, f); break;
-                case 2: regDense1<T, 2>(r, h, f); break;
-                case 3: regDense1<T, 3>(r, h, f); break;
+                case 0: regDense1<T, 0, C>(r, h, f); break;
+                case 1: regDense1<T, 1, C>(r, h, f); break;
+                case 2: regDense1<T, 2, C>(r, h, f); break;
+                case 3: regDense1<T, 3, C>(r, h, f); break;
             }
             break;
         case F_DENSE2: {
             const double* fx = op->fx;
             switch (h.rb1 * RB + h.rb2) {
-#define QA_D2(B1, B2) case B1 * RB + B2: regDense2<T, B1, B2>(r, h, f, fx); break;
+#define QA_D2(B1, B2) case B1 * RB + B2: regDense2<T, B1, B2, C>(r, h, f, fx); break;
                 QA_D2(0, 1) QA_D2(0, 2) QA_D2(0, 3) QA_D2(1, 2) QA_D2(1, 3) QA_D2(2, 3)
                 QA_D2(1, 0) QA_D2(2, 0) QA_D2(3, 0) QA_D2(2, 1) QA_D2(3, 1) QA_D2(3, 2)
 #undef QA_D2
@@ -398,15 +422,31 @@ __device__ __forceinline__ void applyOp(ct<T>* r, const TOp* __restrict__ op, co
             switch (h.rb1 * RB + h.rb2) { QA_PAIR_CASES(regDeph1) }
             break;
         case F_DIAG1:
-            regDiag1<T>(r, h, f, ((jb & o.jm) | (base & o.gm)) != 0);
+            if (h.rb1 < 0) {
+                regDiag1T<T, C>(r, h, f, ((jb & o.jm) | (base & o.gm)) != 0);
+            } else if (h.pad) {
+                switch (h.rb1) {
+                    case 0: regDiag1R<T, 0, C, true>(r, h, f); break;
+                    case 1: regDiag1R<T, 1, C, true>(r, h, f); break;
+                    case 2: regDiag1R<T, 2, C, true>(r, h, f); break;
+                    case 3: regDiag1R<T, 3, C, true>(r, h, f); break;
+                }
+            } else {
+                switch (h.rb1) {
+                    case 0: regDiag1R<T, 0, C, false>(r, h, f); break;
+                    case 1: regDiag1R<T, 1, C, false>(r, h, f); break;
+                    case 2: regDiag1R<T, 2, C, false>(r, h, f); break;
+                    case 3: regDiag1R<T, 3, C, false>(r, h, f); break;
+                }
+            }
             break;
         case F_PARITY:
-            regParity<T>(r, h, f, parity64((jb & o.jm) ^ (base & o.gm)));
+            regParity<T, C>(r, h, f, parity64((jb & o.jm) ^ (base & o.gm)));
             break;
         case F_PAULI: {
             const int thrPar = parity64((jb & o.jm) ^ (base & o.gm));
             switch (h.rm1) {
-#define QA_PAULI_CASE(M) case M: regPauli<T, M>(r, h, f, thrPar); break;
+#define QA_PAULI_CASE(M) case M: regPauli<T, M, C>(r, h, f, thrPar); break;
                 QA_PAULI_CASE(1) QA_PAULI_CASE(2) QA_PAULI_CASE(3) QA_PAULI_CASE(4) QA_PAULI_CASE(5)
                 QA_PAULI_CASE(6) QA_PAULI_CASE(7) QA_PAULI_CASE(8) QA_PAULI_CASE(9) QA_PAULI_CASE(10)
                 QA_PAULI_CASE(11) QA_PAULI_CASE(12) QA_PAULI_CASE(13) QA_PAULI_CASE(14) QA_PAULI_CASE(15)
@@ -415,6 +455,13 @@ __device__ __forceinline__ void applyOp(ct<T>* r, const TOp* __restrict__ op, co
             break;
         }
     }
+}
+
+template <typename T>
+__device__ __forceinline__ void applyOp(ct<T>* r, const TOp* __restrict__ op, const OpRegs& o, uint32_t jb,
+                                        uint64_t base) {
+    if (o.h.rcMask) applyOpC<T, true>(r, op, o, jb, base);
+    else applyOpC<T, false>(r, op, o, jb, base);
 }
 
 __device__ __forceinline__ const int* groupSlots(const TOp& op) { return reinterpret_cast<const int*>(op.b); }
@@ -439,13 +486,9 @@ __device__ __forceinline__ void tileBody(T* __restrict__ amps, const TileArgs& a
     RegLayout cur = first;
     bool inRegs = a.firstDirect;  // registers hold `cur` (else the row layout, not yet in LDS)
     if (!a.firstDirect) RegIO<T, NT, 0>::ldsStore(r, tile, row);
-    // each op's two 64-byte records are fetched one op ahead, so the scalar
-    // load latency overlaps the previous op's arithmetic
-    int o = a.firstDirect ? 1 : 0;
-    u32x16 ch, cb, nh, nb;
-    if (o < a.numOps) fetchOp(ops + o, ch, cb);
-    for (; o < a.numOps; o++) {
-        if (o + 1 < a.numOps) fetchOp(ops + o + 1, nh, nb);
+    for (int o = a.firstDirect ? 1 : 0; o < a.numOps; o++) {
+        u32x16 ch, cb;
+        fetchOp(ops + o, ch, cb);
         OpRegs op;
         unpackOp(ch, cb, op);
         if (op.h.kind == F_GROUP) {
@@ -461,8 +504,6 @@ __device__ __forceinline__ void tileBody(T* __restrict__ amps, const TileArgs& a
                    (cur.jb & op.h.jcMask) == op.h.jcVal) {  // uniform over the thread's registers
             applyOp<T>(r, ops + o, op, cur.jb, base);
         }
-        ch = nh;
-        cb = nb;
     }
     if (a.lastDirect) {
         RegIO<T, NT, 0>::hbmStore(r, amps, base | cur.tg, cur);

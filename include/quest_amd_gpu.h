@@ -121,12 +121,24 @@ int qa_setFusion(int enabled, int tileQubits, int lowQubits, int maxQueue);
 int qa_getFusion(int* enabled, int* tileQubits, int* lowQubits, int* maxQueue);
 /* applies the gates queued on `amps` (NULL: on every state) */
 int qa_flush(void* amps);
-/* counters of tile passes launched and gates they applied since the last
-   qa_getFusionStats (which resets them); with timing on, each pass launch is
-   bracketed by HIP events on the library stream and passMs sums their
-   elapsed times (qa_getFusionStats waits for them) */
+/* complex64 states only: within a pass, every run of at least `minGates`
+   queued gates whose qubits (targets, controls, masks) fit 5 of the tile's
+   qubits is multiplied on the host (FP64) into one 32x32 complex block and
+   applied with FP32 MFMA instead of gate by gate (tiles of at least 2^10
+   amplitudes).  Results then differ from the unfused kernels by rounding.
+   0 (the default) turns blocks off. */
+int qa_setFusionBlocks(int minGates);
+/* tile passes with (1, the default) or without (0) each workgroup's next
+   tile prefetched into registers while it works on the current one */
+int qa_setFusionPrefetch(int enabled);
+/* counters of tile passes launched, the gates they applied and those
+   gates' algorithmic real flops (a complex multiply counts 6, a complex
+   multiply-add 8) since the last qa_getFusionStats (which resets them); with
+   timing on, each pass launch is bracketed by HIP events on the library
+   stream and passMs sums their elapsed times (qa_getFusionStats waits for
+   them) */
 int qa_setFusionTiming(int enabled);
-int qa_getFusionStats(int64_t* passes, int64_t* gates, double* passMs);
+int qa_getFusionStats(int64_t* passes, int64_t* gates, double* passMs, double* flops);
 
 /* ------------------------------------------------------------------ */
 /* statevector gates             (gpu_subroutines.hpp:33-70)          */

@@ -39,7 +39,7 @@ InsertSpec zerosAt(std::vector<int> qs) {
 template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_dephase1(T* __restrict__ amps, uint64_t numWork, InsertSpec ins,
                                                    uint64_t ketBit, uint64_t braBit, T fac) {
-    const uint64_t base = (uint64_t)blockIdx.x * (BLOCK * ITEMS) + threadIdx.x;
+    const uint64_t base = blockId() * (BLOCK * ITEMS) + threadIdx.x;
 #pragma unroll
     for (int j = 0; j < ITEMS; j++) {
         uint64_t n = base + (uint64_t)j * BLOCK;
@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(BLOCK) k_dephase1(T* __restrict__ amps, uint64
 template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_dephase2(T* __restrict__ amps, uint64_t numAmps, int kA, int bA,
                                                    int kB, int bB, T term) {
-    const uint64_t base = (uint64_t)blockIdx.x * (BLOCK * ITEMS) + threadIdx.x;
+    const uint64_t base = blockId() * (BLOCK * ITEMS) + threadIdx.x;
 #pragma unroll
     for (int j = 0; j < ITEMS; j++) {
         uint64_t i = base + (uint64_t)j * BLOCK;
@@ -75,7 +75,7 @@ __global__ void __launch_bounds__(BLOCK) k_dephase2(T* __restrict__ amps, uint64
 template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_depol1(T* __restrict__ amps, uint64_t numWork, InsertSpec ins,
                                                  uint64_t ketBit, uint64_t braBit, T c1, T c2, T c3) {
-    const uint64_t base = (uint64_t)blockIdx.x * (BLOCK * 2) + threadIdx.x;
+    const uint64_t base = blockId() * (BLOCK * 2) + threadIdx.x;
     uint64_t i00[2];
     ct<T> a00[2], a01[2], a10[2], a11[2];
 #pragma unroll
@@ -108,7 +108,7 @@ template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_pauliChannel(T* __restrict__ amps, uint64_t numWork,
                                                        InsertSpec ins, uint64_t ketBit, uint64_t braBit,
                                                        T fAA, T fBB, T fAB, T fBA) {
-    const uint64_t n = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const uint64_t n = blockId() * BLOCK + threadIdx.x;
     if (n >= numWork) return;
     uint64_t i00 = insertBits(n, ins);
     uint64_t i01 = i00 | ketBit, i10 = i00 | braBit, i11 = i01 | braBit;
@@ -123,7 +123,7 @@ __global__ void __launch_bounds__(BLOCK) k_pauliChannel(T* __restrict__ amps, ui
 template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_damp(T* __restrict__ amps, uint64_t numWork, InsertSpec ins,
                                                uint64_t ketBit, uint64_t braBit, T prob, T c1, T c2) {
-    const uint64_t n = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const uint64_t n = blockId() * BLOCK + threadIdx.x;
     if (n >= numWork) return;
     uint64_t i00 = insertBits(n, ins);
     uint64_t i01 = i00 | ketBit, i10 = i00 | braBit, i11 = i01 | braBit;
@@ -143,7 +143,7 @@ template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_depol2(T* __restrict__ amps, uint64_t numWork, InsertSpec ins,
                                                  uint64_t k1, uint64_t k2, uint64_t b1, uint64_t b2, T c1mc2,
                                                  T c2, T c3) {
-    const uint64_t n = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const uint64_t n = blockId() * BLOCK + threadIdx.x;
     if (n >= numWork) return;
     const uint64_t i0 = insertBits(n, ins);
     ct<T> diag[4];
@@ -173,7 +173,7 @@ __global__ void __launch_bounds__(BLOCK) k_depol2(T* __restrict__ amps, uint64_t
 template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_mixDM(T* __restrict__ out, const T* __restrict__ in, uint64_t num,
                                                 T po, T pi) {
-    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const uint64_t i = blockId() * BLOCK + threadIdx.x;
     if (i >= num) return;
     ct<T> a = ld(out, i), b = ld(in, i);
     st(out, i, ct<T>{po * a.re + pi * b.re, po * a.im + pi * b.im});
@@ -183,7 +183,7 @@ __global__ void __launch_bounds__(BLOCK) k_mixDM(T* __restrict__ out, const T* _
 template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_mixSV(T* __restrict__ out, const T* __restrict__ psi, uint64_t num,
                                                 int N, T po, T pi) {
-    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const uint64_t i = blockId() * BLOCK + threadIdx.x;
     if (i >= num) return;
     uint64_t r = i & ((1ull << N) - 1), c = i >> N;
     ct<T> a = ld(out, i), pr = ld(psi, r), pc = ld(psi, c);
@@ -195,7 +195,7 @@ __global__ void __launch_bounds__(BLOCK) k_mixSV(T* __restrict__ out, const T* _
 template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_pureDM(T* __restrict__ rho, const T* __restrict__ psi, uint64_t num,
                                                  int N) {
-    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const uint64_t i = blockId() * BLOCK + threadIdx.x;
     if (i >= num) return;
     uint64_t r = i & ((1ull << N) - 1), c = i >> N;
     ct<T> pr = ld(psi, r), pc = ld(psi, c);
@@ -209,7 +209,7 @@ template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_partialTrace(const T* __restrict__ in, T* __restrict__ out,
                                                        uint64_t numOut, InsertSpec ins, TargSpec targs,
                                                        TargSpec pairs) {
-    const uint64_t n = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const uint64_t n = blockId() * BLOCK + threadIdx.x;
     if (n >= numOut) return;
     uint64_t k = insertBits(n, ins);
     uint64_t numInner = 1ull << targs.num;

@@ -166,11 +166,20 @@ __device__ __forceinline__ int parity64(uint64_t x) { return __popcll(x) & 1; }
 constexpr int BLOCK = 256;   // 4 wave64s per workgroup
 constexpr int ITEMS = 4;     // work items per thread, strided by BLOCK
 
-inline unsigned gridFor(uint64_t work, int items = ITEMS) {
-    uint64_t per = (uint64_t)BLOCK * (uint64_t)items;
+// Workgroups covering `work` items, `items` per thread.  AMD dispatch
+// packets count work-items in 32 bits, so a grid of more than 2^30
+// work-items folds into a second dimension (2^22 workgroups per row);
+// kernels number their workgroups with blockId() and skip items past the
+// end (states of 33-34 qubits need the fold).
+inline dim3 gridFor(uint64_t work, int items = ITEMS) {
+    const uint64_t per = (uint64_t)BLOCK * (uint64_t)items;
     uint64_t g = (work + per - 1) / per;
-    return (unsigned)(g ? g : 1);
+    if (!g) g = 1;
+    const uint64_t maxX = 1ull << 22;
+    if (g <= maxX) return dim3((unsigned)g, 1, 1);
+    return dim3((unsigned)maxX, (unsigned)((g + maxX - 1) / maxX), 1);
 }
+__device__ __forceinline__ uint64_t blockId() { return (uint64_t)blockIdx.y * gridDim.x + blockIdx.x; }
 
 // ---- errors ------------------------------------------------------------
 void setError(const char* fmt, ...);

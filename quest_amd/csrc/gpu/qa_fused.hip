@@ -20,6 +20,8 @@
 // single-gate kernels (qa_gates.hip, qa_channels.hip), applied in the same
 // order, so fused and unfused runs give identical amplitudes.
 #include <algorithm>
+#include <cmath>
+#include <complex>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -35,11 +37,12 @@ namespace qa {
 namespace fuse {
 
 // NT: non-temporal HBM loads/stores, used when the contiguous low run of a
-// tile row is at least 2^STREAM_LOW_BIT amplitudes.  Persistent workgroups:
-// each prefetches its next tile into registers before working on the
-// current one, so HBM traffic overlaps the register and LDS work.
-template <typename T, int NT>
-__global__ void __launch_bounds__(512) k_tilePass(T* __restrict__ amps, TileArgs a, const TOp* __restrict__ ops) {
+// tile row is at least 2^STREAM_LOW_BIT amplitudes.  Persistent workgroups;
+// PF = 1: each prefetches its next tile into registers before working on
+// the current one, so HBM traffic overlaps the register and LDS work (16
+// more registers of amplitudes per thread); PF = 0: load, work, store.
+template <typename T, int NT, int PF>
+__device__ __forceinline__ void tilePass(T* __restrict__ amps, const TileArgs& a, const TOp* __restrict__ ops) {
     extern __shared__ unsigned char smem[];
     ct<T>* tile = reinterpret_cast<ct<T>*>(smem);
     const uint32_t tid = threadIdx.x;
@@ -47,21 +50,69 @@ __global__ void __launch_bounds__(512) k_tilePass(T* __restrict__ amps, TileArgs
     passLayouts(row, first, a, ops, tid);
     const RegLayout& loadL = a.firstDirect ? first : row;
 
-    ct<T> rn[RA];
-    uint64_t t = blockIdx.x;
-    if (t < a.numTiles)
-        RegIO<T, NT, 0>::hbmLoad(rn, amps, deposit(t, a.nonLocalPos, a.numNonLocal) | loadL.tg, loadL);
-    for (; t < a.numTiles; t += gridDim.x) {
-        const uint64_t base = deposit(t, a.nonLocalPos, a.numNonLocal);
-        ct<T> r[RA];
+    if constexpr (PF) {
+        ct<T> rn[RA];
+        uint64_t t = blockIdx.x;
+        if (t < a.numTiles)
+            RegIO<T, NT, 0>::hbmLoad(rn, amps, deposit(t, a.nonLocalPos, a.numNonLocal) | loadL.tg, loadL);
+        for (; t < a.numTiles; t += gridDim.x) {
+            const uint64_t base = deposit(t, a.nonLocalPos, a.numNonLocal);
+            ct<T> r[RA];
 #pragma unroll
-        for (int v = 0; v < RA; v++) r[v] = rn[v];
-        const uint64_t tn = t + gridDim.x;
-        if (tn < a.numTiles)
-            RegIO<T, NT, 0>::hbmLoad(rn, amps, deposit(tn, a.nonLocalPos, a.numNonLocal) | loadL.tg, loadL);
-        tileBody<T, NT>(amps, a, ops, base, r, tile, row, first, tid);
-        // the next tile's first LDS writes must not overtake this tile's reads
-        if (a.usesLds) __syncthreads();
+            for (int v = 0; v < RA; v++) r[v] = rn[v];
+            const uint64_t tn = t + gridDim.x;
+            if (tn < a.numTiles)
+                RegIO<T, NT, 0>::hbmLoad(rn, amps, deposit(tn, a.nonLocalPos, a.numNonLocal) | loadL.tg, loadL);
+            tileBody<T, NT>(amps, a, ops, base, r, tile, row, first, tid);
+            // the next tile's first LDS writes must not overtake this tile's reads
+            if (a.usesLds) __syncthreads();
+        }
+    } else {
+        for (uint64_t t = blockIdx.x; t < a.numTiles; t += gridDim.x) {
+            const uint64_t base = deposit(t, a.nonLocalPos, a.numNonLocal);
+            ct<T> r[RA];
+            RegIO<T, NT, 0>::hbmLoad(r, amps, base | loadL.tg, loadL);
+            tileBody<T, NT>(amps, a, ops, base, r, tile, row, first, tid);
+            if (a.usesLds) __syncthreads();
+        }
+    }
+}
+
+// the prefetching form needs 16 more registers of amplitudes per thread;
+// the lean form is held to a register budget of three waves per SIMD
+template <typename T, int NT>
+__global__ void __launch_bounds__(512) k_tilePass(T* __restrict__ amps, TileArgs a, const TOp* __restrict__ ops) {
+    tilePass<T, NT, 1>(amps, a, ops);
+}
+template <typename T, int NT>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3)))
+k_tilePassLean(T* __restrict__ amps, TileArgs a, const TOp* __restrict__ ops) {
+    tilePass<T, NT, 0>(amps, a, ops);
+}
+
+// [precision FP32/FP64][non-temporal][prefetch]
+template <typename T, int NT, int PF>
+void launchTile(unsigned grid, unsigned block, size_t lds, void* amps, const TileArgs& a, const TOp* ops) {
+    if constexpr (PF) k_tilePass<T, NT><<<grid, block, lds, stream()>>>((T*)amps, a, ops);
+    else k_tilePassLean<T, NT><<<grid, block, lds, stream()>>>((T*)amps, a, ops);
+}
+
+const void* tileKernel(int prec, bool nt, bool pf) {
+    if (prec == QA_FP64)
+        return nt ? (pf ? (const void*)k_tilePass<double, 1> : (const void*)k_tilePassLean<double, 1>)
+                  : (pf ? (const void*)k_tilePass<double, 0> : (const void*)k_tilePassLean<double, 0>);
+    return nt ? (pf ? (const void*)k_tilePass<float, 1> : (const void*)k_tilePassLean<float, 1>)
+              : (pf ? (const void*)k_tilePass<float, 0> : (const void*)k_tilePassLean<float, 0>);
+}
+
+void launchTileAny(int prec, bool nt, bool pf, unsigned grid, unsigned block, size_t lds, void* amps,
+                   const TileArgs& a, const TOp* ops) {
+    if (prec == QA_FP64) {
+        if (nt) pf ? launchTile<double, 1, 1>(grid, block, lds, amps, a, ops) : launchTile<double, 1, 0>(grid, block, lds, amps, a, ops);
+        else pf ? launchTile<double, 0, 1>(grid, block, lds, amps, a, ops) : launchTile<double, 0, 0>(grid, block, lds, amps, a, ops);
+    } else {
+        if (nt) pf ? launchTile<float, 1, 1>(grid, block, lds, amps, a, ops) : launchTile<float, 1, 0>(grid, block, lds, amps, a, ops);
+        else pf ? launchTile<float, 0, 1>(grid, block, lds, amps, a, ops) : launchTile<float, 0, 0>(grid, block, lds, amps, a, ops);
     }
 }
 
@@ -81,6 +132,7 @@ Config g_cfg;  // disabled by default
 // around each tile-pass launch (for measuring the pass kernel alone)
 struct Stats {
     int64_t passes = 0, gates = 0;
+    double flops = 0;  // algorithmic real floating-point operations of the passes
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
     size_t used = 0;
@@ -135,6 +187,14 @@ unsigned residentBlocks(const void* fn, unsigned block, size_t lds) {
 }
 
 std::vector<TOp> g_prog;  // program of the pass being launched (under g_mu)
+
+// packed MFMA block matrices: staged per pass, uploaded through a ring like
+// the op ring (a wrap waits for the stream)
+constexpr size_t BLOCK_RING_FLOATS = 4u << 20;  // 1024 blocks of 16 KiB
+std::vector<float> g_blockData;
+float* g_blockHost = nullptr;
+float* g_blockDev = nullptr;
+size_t g_blockPos = 0;
 
 // where each qubit of the state sits during one group of a pass
 struct Layout {
@@ -223,11 +283,13 @@ size_t fuse::takeRun(const GOp* ops, size_t num, int numQubits, int kMax, int lo
     return i;
 }
 
-void fuse::planPass(const GOp* ops, size_t num, int numQubits, uint64_t localMask, int lowQubits, TileArgs& a,
-                    std::vector<TOp>& prog) {
+namespace fuse {
+
+// the tile slots of localMask, and the pass's fixed fields of `a`
+void passShape(int numQubits, uint64_t localMask, TileArgs& a, Layout& L) {
     std::memset(&a, 0, sizeof(a));
-    Layout L;
     for (int q = 0; q < 64; q++) L.slotOf[q] = -1;
+    for (int s = 0; s < MAXQ; s++) L.regOf[s] = -1;
     int k = 0, nl = 0;
     for (int q = 0; q < numQubits; q++) {
         if ((localMask >> q) & 1ull) {
@@ -240,21 +302,24 @@ void fuse::planPass(const GOp* ops, size_t num, int numQubits, uint64_t localMas
     a.k = k;
     a.numNonLocal = nl;
     a.numTiles = 1ull << nl;
-    // slots of the contiguous low qubits: a layout whose register slots
-    // avoid them reads and writes HBM in runs as long as the row layout's
-    const int low = std::min(lowQubits, k);
-    const uint32_t lowSlots = (1u << low) - 1u;
+}
 
-    // groups of consecutive ops whose register needs fit RB slots; padding
-    // takes the highest free slots outside the low run first
-    const size_t start = prog.size();
+// Emits ops[from, to) as register groups (a GROUP op, then the ops whose
+// register needs fit RB slots with it, and so on).  Padding takes the
+// highest free slots outside the low run first, so that the first and last
+// groups can load and store HBM directly.
+struct GroupState {
     int numGroups = 0;
     uint32_t firstG = 0, lastG = 0;
-    size_t i = 0;
-    while (i < num) {
+};
+
+void emitGroups(const GOp* ops, size_t from, size_t to, Layout& L, int k, int low, std::vector<TOp>& prog,
+                GroupState& st) {
+    size_t i = from;
+    while (i < to) {
         uint32_t G = 0;
         size_t j = i;
-        for (; j < num; j++) {
+        for (; j < to; j++) {
             uint32_t u = G | regNeed(ops[j], L);
             if (__builtin_popcount(u) > RB) break;
             G = u;
@@ -266,31 +331,340 @@ void fuse::planPass(const GOp* ops, size_t num, int numQubits, uint64_t localMas
         grp.h.kind = F_GROUP;
         for (int sl = 0; sl < k; sl++) L.regOf[sl] = -1;
         int* slots = reinterpret_cast<int*>(grp.b);
-        for (int sl = 0, b = 0; sl < k; sl++)
-            if ((G >> sl) & 1u) {
-                slots[b] = sl;
-                L.regOf[sl] = b++;
+        // register bits in the order uncontrolled one-qubit gates first use
+        // their slots, so runs of them land on bits 0, 1, 2, ... in order
+        int nb = 0;
+        for (size_t p = i; p < j && nb < RB; p++) {
+            const GOp& g = ops[p];
+            if (g.kind != F_DENSE1 || g.cMask) continue;
+            const int sl = L.slotOf[g.t1];
+            if (L.regOf[sl] >= 0) continue;
+            slots[nb] = sl;
+            L.regOf[sl] = nb++;
+        }
+        for (int sl = 0; sl < k; sl++)
+            if (((G >> sl) & 1u) && L.regOf[sl] < 0) {
+                slots[nb] = sl;
+                L.regOf[sl] = nb++;
             }
         prog.push_back(grp);
-        if (numGroups == 0) firstG = G;
-        lastG = G;
-        numGroups++;
-        for (; i < j; i++) prog.push_back(toTile(ops[i], L));
+        if (st.numGroups == 0) st.firstG = G;
+        st.lastG = G;
+        st.numGroups++;
+        while (i < j) {
+            // a run of 2-4 uncontrolled one-qubit gates on register bits
+            // 0, 1, ... in order becomes one DENSE1N op (same arithmetic)
+            size_t n = 0;
+            while (n < (size_t)RB && i + n < j && ops[i + n].kind == F_DENSE1 && !ops[i + n].cMask &&
+                   L.regOf[L.slotOf[ops[i + n].t1]] == (int)n)
+                n++;
+            if (n >= 2) {
+                TOp t = toTile(ops[i], L);
+                t.h.kind = F_DENSE1N;
+                t.h.rb2 = (int)n;
+                for (size_t m = 1; m < n; m++) std::memcpy(t.fx + 8 * (m - 1), ops[i + m].f, 8 * sizeof(double));
+                prog.push_back(t);
+                i += n;
+            } else {
+                prog.push_back(toTile(ops[i], L));
+                i++;
+            }
+        }
     }
-    a.firstDirect = (firstG & lowSlots) == 0;
-    a.lastDirect = (lastG & lowSlots) == 0;
-    a.usesLds = !a.firstDirect || !a.lastDirect || numGroups > 1;
+}
+
+// every qubit an op reads or writes: targets, controls, diagonal masks
+uint64_t supportMask(const GOp& g) {
+    uint64_t m = g.cMask | denseMask(g);
+    switch (g.kind) {
+        case F_DIAG1: m |= 1ull << g.t1; break;
+        case F_PARITY: m |= g.m1; break;
+        case F_PAULI: m |= g.m1 | g.m2; break;
+        default: break;
+    }
+    return m;
+}
+
+// g applied to a vector of 2^BQ amplitudes whose bit b is global qubit q[b]
+// (every qubit of g is one of them), in FP64, with the same semantics as
+// the single-gate kernels
+void applySmall(const GOp& g, const int* q, std::complex<double>* v) {
+    typedef std::complex<double> cx;
+    auto loc = [&](uint64_t m) {
+        uint32_t r = 0;
+        for (int b = 0; b < BQ; b++)
+            if ((m >> q[b]) & 1ull) r |= 1u << b;
+        return r;
+    };
+    auto bitOf = [&](int qubit) {
+        for (int b = 0; b < BQ; b++)
+            if (q[b] == qubit) return b;
+        return -1;
+    };
+    const uint32_t cm = loc(g.cMask), cv = loc(g.cVal);
+    const int D = 1 << BQ;
+    auto fc = [&](int i) { return cx(g.f[2 * i], g.f[2 * i + 1]); };
+    switch (g.kind) {
+        case F_DENSE1: {
+            const uint32_t t = 1u << bitOf(g.t1);
+            for (int j = 0; j < D; j++) {
+                if ((j & t) || (j & cm) != cv) continue;
+                const cx a0 = v[j], a1 = v[j | t];
+                v[j] = fc(0) * a0 + fc(1) * a1;
+                v[j | t] = fc(2) * a0 + fc(3) * a1;
+            }
+            break;
+        }
+        case F_DENSE2: {
+            const uint32_t t1 = 1u << bitOf(g.t1), t2 = 1u << bitOf(g.t2);
+            const uint32_t off[4] = {0, t1, t2, t1 | t2};
+            for (int j = 0; j < D; j++) {
+                if ((j & (t1 | t2)) || (j & cm) != cv) continue;
+                cx x[4], y[4];
+                for (int c = 0; c < 4; c++) x[c] = v[j | off[c]];
+                for (int r = 0; r < 4; r++) {
+                    y[r] = 0;
+                    for (int c = 0; c < 4; c++) y[r] += fc(4 * r + c) * x[c];
+                }
+                for (int r = 0; r < 4; r++) v[j | off[r]] = y[r];
+            }
+            break;
+        }
+        case F_DIAG1: {
+            const uint32_t t = 1u << bitOf(g.t1);
+            for (int j = 0; j < D; j++)
+                if ((j & cm) == cv) v[j] *= (j & t) ? fc(1) : fc(0);
+            break;
+        }
+        case F_PARITY: {
+            const uint32_t m = loc(g.m1);
+            for (int j = 0; j < D; j++)
+                if ((j & cm) == cv) v[j] *= (__builtin_popcount(j & m) & 1) ? fc(1) : fc(0);
+            break;
+        }
+        case F_PAULI: {
+            const uint32_t mxy = loc(g.m1), myz = loc(g.m2);
+            const uint32_t top = 1u << (31 - __builtin_clz(mxy));
+            for (int j = 0; j < D; j++) {
+                if ((j & top) || (j & cm) != cv) continue;
+                const int kk = j ^ mxy;
+                const double sA = (__builtin_popcount(j & myz) & 1) ? -1 : 1;
+                const double sB = (__builtin_popcount(kk & myz) & 1) ? -1 : 1;
+                const cx va = v[j], vb = v[kk];
+                v[j] = fc(0) * va + fc(1) * sB * vb;
+                v[kk] = fc(0) * vb + fc(1) * sA * va;
+            }
+            break;
+        }
+        case F_SWAP: {
+            const uint32_t t1 = 1u << bitOf(g.t1), t2 = 1u << bitOf(g.t2);
+            for (int j = 0; j < D; j++) {
+                if (!(j & t1) || (j & t2) || (j & cm) != cv) continue;
+                std::swap(v[j], v[j ^ (t1 | t2)]);
+            }
+            break;
+        }
+        case F_DEPOL1: {
+            const uint32_t k1 = 1u << bitOf(g.t1), k2 = 1u << bitOf(g.t2);
+            const double c1 = g.f[0], c2 = g.f[1], c3 = g.f[2];
+            for (int j = 0; j < D; j++) {
+                if (j & (k1 | k2)) continue;
+                const cx a00 = v[j], a11 = v[j | k1 | k2];
+                v[j] = c1 * a00 + c2 * a11;
+                v[j | k1] *= c3;
+                v[j | k2] *= c3;
+                v[j | k1 | k2] = c1 * a11 + c2 * a00;
+            }
+            break;
+        }
+        case F_DEPH1: {
+            const uint32_t k1 = 1u << bitOf(g.t1), k2 = 1u << bitOf(g.t2);
+            for (int j = 0; j < D; j++)
+                if (((j & k1) != 0) != ((j & k2) != 0)) v[j] *= g.f[0];
+            break;
+        }
+        default: break;
+    }
+}
+
+// the BLOCK op of ops[from, to), whose qubits all lie in the BQ local
+// qubits of S; its matrix is appended to `blocks` as the MFMA operands
+TOp makeBlock(const GOp* ops, size_t from, size_t to, uint64_t S, const Layout& L, std::vector<float>& blocks) {
+    typedef std::complex<double> cx;
+    const int D = 1 << BQ;
+    int q[BQ];
+    for (int b = 0, qq = 0; qq < 64 && b < BQ; qq++)
+        if ((S >> qq) & 1ull) q[b++] = qq;
+    // U column by column: the run applied to each basis vector
+    std::vector<cx> U(D * D);
+    for (int c = 0; c < D; c++) {
+        cx v[D];
+        for (int j = 0; j < D; j++) v[j] = (j == c) ? 1.0 : 0.0;
+        for (size_t i = from; i < to; i++) applySmall(ops[i], q, v);
+        for (int r = 0; r < D; r++) U[r * D + c] = v[r];
+    }
+    // A[r][kk] of the real form [[Re U, -Im U], [Im U, Re U]], packed as
+    // [K-step s][half][lane] = A[half*32 + lane%32][2s + lane/32]
+    const size_t off = blocks.size();
+    blocks.resize(off + 32 * 2 * 64);
+    float* P = blocks.data() + off;
+    for (int s = 0; s < 32; s++)
+        for (int half = 0; half < 2; half++)
+            for (int lane = 0; lane < 64; lane++) {
+                const int r = lane % 32, kk = 2 * s + lane / 32;
+                const cx u = U[r * D + (kk % 32)];
+                double val;
+                if (half == 0) val = kk < 32 ? u.real() : -u.imag();
+                else val = kk < 32 ? u.imag() : u.real();
+                P[(2 * s + half) * 64 + lane] = (float)val;
+            }
+    TOp t;
+    std::memset(&t, 0, sizeof(t));
+    t.h.kind = F_BLOCK;
+    int* slots = reinterpret_cast<int*>(t.b);
+    for (int b = 0; b < BQ; b++) slots[b] = L.slotOf[q[b]];
+    t.b[3] = (uint64_t)off;
+    return t;
+}
+
+}  // namespace fuse
+
+void fuse::planPass(const GOp* ops, size_t num, int numQubits, uint64_t localMask, int lowQubits, TileArgs& a,
+                    std::vector<TOp>& prog) {
+    Layout L;
+    passShape(numQubits, localMask, a, L);
+    const int k = a.k;
+    const int low = std::min(lowQubits, k);
+    const uint32_t lowSlots = (1u << low) - 1u;
+    const size_t start = prog.size();
+    GroupState st;
+    emitGroups(ops, 0, num, L, k, low, prog, st);
+    a.firstDirect = (st.firstG & lowSlots) == 0;
+    a.lastDirect = (st.lastG & lowSlots) == 0;
+    a.usesLds = !a.firstDirect || !a.lastDirect || st.numGroups > 1;
+    a.numOps = (int)(prog.size() - start);
+}
+
+void fuse::planPassBlocks(const GOp* ops, size_t num, int numQubits, uint64_t localMask, int lowQubits,
+                          int blockMinGates, TileArgs& a, std::vector<TOp>& prog, std::vector<float>& blocks) {
+    Layout L;
+    passShape(numQubits, localMask, a, L);
+    const int k = a.k;
+    const int low = std::min(lowQubits, k);
+    const uint32_t lowSlots = (1u << low) - 1u;
+    const size_t start = prog.size();
+    GroupState st;
+    bool firstItemGroup = true, lastItemGroup = true, any = false;
+    size_t pending = 0;  // first op not yet emitted
+    size_t i = 0;
+    while (i < num) {
+        // the longest run from i whose qubits fit BQ local qubits
+        uint64_t S = 0;
+        size_t j = i;
+        for (; j < num; j++) {
+            const uint64_t sup = supportMask(ops[j]);
+            if (sup & ~localMask) break;
+            const uint64_t u = S | sup;
+            if (__builtin_popcountll(u) > BQ) break;
+            S = u;
+        }
+        if (k >= 10 && (int)(j - i) >= blockMinGates && blockMinGates > 0) {
+            if (pending < i) {
+                if (!any) firstItemGroup = true;
+                emitGroups(ops, pending, i, L, k, low, prog, st);
+                any = true;
+                lastItemGroup = true;
+            }
+            // pad S with other local qubits
+            for (int qq = 0; qq < numQubits && __builtin_popcountll(S) < BQ; qq++)
+                if ((localMask >> qq) & 1ull) S |= 1ull << qq;
+            if (!any) firstItemGroup = false;
+            prog.push_back(makeBlock(ops, i, j, S, L, blocks));
+            any = true;
+            lastItemGroup = false;
+            i = pending = j;
+        } else {
+            i++;
+        }
+    }
+    if (pending < num) {
+        if (!any) firstItemGroup = true;
+        emitGroups(ops, pending, num, L, k, low, prog, st);
+        lastItemGroup = true;
+    }
+    // a pass that starts or ends with a BLOCK moves the tile through LDS in
+    // the row layout; one that starts with a BLOCK still begins with a GROUP
+    // op (its layout is never used), so ops[0] is always a layout
+    if (!firstItemGroup) {
+        TOp grp;
+        std::memset(&grp, 0, sizeof(grp));
+        grp.h.kind = F_GROUP;
+        int* slots = reinterpret_cast<int*>(grp.b);
+        for (int b = 0; b < RB; b++) slots[b] = k - RB + b;
+        prog.insert(prog.begin() + start, grp);
+    }
+    a.firstDirect = firstItemGroup && (st.firstG & lowSlots) == 0;
+    a.lastDirect = lastItemGroup && (st.lastG & lowSlots) == 0;
+    a.usesLds = true;
     a.numOps = (int)(prog.size() - start);
 }
 
 namespace fuse {
+
+// algorithmic real flops of one op on a state of n index bits: a complex
+// multiply is 6, a complex multiply-add 8; amplitudes failing the controls
+// are not counted
+double opFlops(const GOp& g, int n) {
+    double perAmp = 0;
+    switch (g.kind) {
+        case F_DENSE1: perAmp = 16; break;   // 4 complex MACs per pair
+        case F_DENSE2: perAmp = 32; break;   // 16 complex MACs per 4 amplitudes
+        case F_DIAG1: perAmp = 6; break;     // one complex multiply
+        case F_PARITY: perAmp = 6; break;
+        case F_PAULI: perAmp = 14; break;    // multiply + multiply-add per amplitude
+        case F_DEPOL1: perAmp = 4; break;    // 16 per 4 amplitudes
+        case F_DEPH1: perAmp = 1; break;     // 4 per 4 amplitudes
+        default: perAmp = 0; break;
+    }
+    return perAmp * std::ldexp(1.0, n - __builtin_popcountll(g.cMask));
+}
 
 // launches one pass applying ops [begin, end) of the queue
 int launchPass(void* amps, const Queue& qu, const std::vector<GOp>& ops, size_t begin, size_t end, uint64_t localMask) {
     TileArgs a;
     std::vector<TOp>& prog = g_prog;
     prog.clear();
-    planPass(ops.data() + begin, end - begin, qu.numQubits, localMask, g_cfg.lowQubits, a, prog);
+    if (qu.prec == QA_FP32 && g_cfg.blockMinGates > 0) {
+        g_blockData.clear();
+        planPassBlocks(ops.data() + begin, end - begin, qu.numQubits, localMask, g_cfg.lowQubits, g_cfg.blockMinGates,
+                       a, prog, g_blockData);
+        if (!g_blockData.empty()) {
+            const size_t nf = g_blockData.size();
+            if (nf > BLOCK_RING_FLOATS) {
+                setError("fusion: a pass holds more MFMA blocks than the block ring");
+                return QA_ERR_ARG;
+            }
+            if (!g_blockDev) {
+                if (hipHostMalloc(&g_blockHost, BLOCK_RING_FLOATS * sizeof(float), hipHostMallocDefault) != hipSuccess ||
+                    hipMalloc(&g_blockDev, BLOCK_RING_FLOATS * sizeof(float)) != hipSuccess) {
+                    setError("fusion: block ring allocation failed");
+                    return QA_ERR_OOM;
+                }
+                g_blockPos = 0;
+            }
+            if (g_blockPos + nf > BLOCK_RING_FLOATS) {
+                QA_HIP(hipStreamSynchronize(stream()));
+                g_blockPos = 0;
+            }
+            std::memcpy(g_blockHost + g_blockPos, g_blockData.data(), nf * sizeof(float));
+            QA_HIP(hipMemcpyAsync(g_blockDev + g_blockPos, g_blockHost + g_blockPos, nf * sizeof(float),
+                                  hipMemcpyHostToDevice, stream()));
+            a.blocks = g_blockDev + g_blockPos;
+            g_blockPos += nf;
+        }
+    } else {
+        planPass(ops.data() + begin, end - begin, qu.numQubits, localMask, g_cfg.lowQubits, a, prog);
+    }
     const int k = a.k;
 
     size_t num = prog.size();
@@ -306,8 +680,8 @@ int launchPass(void* amps, const Queue& qu, const std::vector<GOp>& ops, size_t 
     const size_t lds = a.usesLds ? ldsBytes(qu.prec, k) : 0;
     const unsigned block = 1u << (k - RB);
     const bool nt = std::min(g_cfg.lowQubits, k) >= STREAM_LOW_BIT;
-    const void* fn = qu.prec == QA_FP64 ? (nt ? (const void*)k_tilePass<double, 1> : (const void*)k_tilePass<double, 0>)
-                                        : (nt ? (const void*)k_tilePass<float, 1> : (const void*)k_tilePass<float, 0>);
+    const bool pf = g_cfg.prefetch;
+    const void* fn = tileKernel(qu.prec, nt, pf);
     const unsigned grid = (unsigned)std::min<uint64_t>(a.numTiles, residentBlocks(fn, block, lds));
     std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
     if (g_stats.timing) {
@@ -320,17 +694,12 @@ int launchPass(void* amps, const Queue& qu, const std::vector<GOp>& ops, size_t 
         ev = &g_stats.pool[g_stats.used++];
         QA_HIP(hipEventRecord(ev->first, stream()));
     }
-    if (qu.prec == QA_FP64) {
-        if (nt) k_tilePass<double, 1><<<grid, block, lds, stream()>>>((double*)amps, a, d);
-        else k_tilePass<double, 0><<<grid, block, lds, stream()>>>((double*)amps, a, d);
-    } else {
-        if (nt) k_tilePass<float, 1><<<grid, block, lds, stream()>>>((float*)amps, a, d);
-        else k_tilePass<float, 0><<<grid, block, lds, stream()>>>((float*)amps, a, d);
-    }
+    launchTileAny(qu.prec, nt, pf, grid, block, lds, amps, a, d);
     QA_LAUNCH_CHECK();
     if (ev) QA_HIP(hipEventRecord(ev->second, stream()));
     g_stats.passes++;
     g_stats.gates += (int64_t)(end - begin);
+    for (size_t i = begin; i < end; i++) g_stats.flops += opFlops(ops[i], qu.numQubits);
     return QA_OK;
 }
 
@@ -436,6 +805,11 @@ void shutdown() {
     g_ringHost = nullptr;
     g_ringDev = nullptr;
     g_ringPos = 0;
+    if (g_blockHost) (void)hipHostFree(g_blockHost);
+    if (g_blockDev) (void)hipFree(g_blockDev);
+    g_blockHost = nullptr;
+    g_blockDev = nullptr;
+    g_blockPos = 0;
 }
 
 const Config& config() { return g_cfg; }
@@ -461,12 +835,10 @@ int qa_setFusion(int enabled, int tileQubits, int lowQubits, int maxQueue) {
         QA_ARG_CHECK(need <= (size_t)maxLds, "qa_setFusion: a %d-qubit tile needs %zu bytes of LDS, the device allows %d",
                      tileQubits, need, maxLds);
         if (need > 65536) {
-            const void* fns[] = {(const void*)qa::fuse::k_tilePass<double, 0>,
-                                 (const void*)qa::fuse::k_tilePass<double, 1>,
-                                 (const void*)qa::fuse::k_tilePass<float, 0>,
-                                 (const void*)qa::fuse::k_tilePass<float, 1>};
-            for (const void* fn : fns)
+            for (int v = 0; v < 8; v++) {
+                const void* fn = qa::fuse::tileKernel((v & 4) ? QA_FP64 : QA_FP32, v & 2, v & 1);
                 QA_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+            }
         }
     }
     std::lock_guard<std::mutex> lk(qa::fuse::g_mu);
@@ -493,10 +865,11 @@ int qa_setFusionTiming(int enabled) {
     qa::fuse::g_stats.timing = enabled != 0;
     qa::fuse::g_stats.used = 0;
     qa::fuse::g_stats.passes = qa::fuse::g_stats.gates = 0;
+    qa::fuse::g_stats.flops = 0;
     return QA_OK;
 }
 
-int qa_getFusionStats(int64_t* passes, int64_t* gates, double* passMs) {
+int qa_getFusionStats(int64_t* passes, int64_t* gates, double* passMs, double* flops) {
     QA_REQUIRE_INIT();
     QA_FLUSH_ALL();
     std::lock_guard<std::mutex> lk(qa::fuse::g_mu);
@@ -511,8 +884,27 @@ int qa_getFusionStats(int64_t* passes, int64_t* gates, double* passMs) {
     if (passes) *passes = st.passes;
     if (gates) *gates = st.gates;
     if (passMs) *passMs = total;
+    if (flops) *flops = st.flops;
     st.used = 0;
     st.passes = st.gates = 0;
+    st.flops = 0;
+    return QA_OK;
+}
+
+int qa_setFusionPrefetch(int enabled) {
+    QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
+    std::lock_guard<std::mutex> lk(qa::fuse::g_mu);
+    qa::fuse::g_cfg.prefetch = enabled != 0;
+    return QA_OK;
+}
+
+int qa_setFusionBlocks(int minGates) {
+    QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
+    QA_ARG_CHECK(minGates >= 0 && minGates <= 4096, "qa_setFusionBlocks: minGates must be in [0, 4096]");
+    std::lock_guard<std::mutex> lk(qa::fuse::g_mu);
+    qa::fuse::g_cfg.blockMinGates = minGates;
     return QA_OK;
 }
 

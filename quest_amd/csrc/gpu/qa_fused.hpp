@@ -19,10 +19,13 @@ struct GOp {
 
 struct Config {
     bool enabled = false;
-    int tileQubits = 12;   // FP64 tile = 2^12 amplitudes (64 KiB of LDS); FP32 tiles get one more qubit
+    int tileQubits = 12;   // tile = 2^12 amplitudes (64 KiB of LDS in FP64)
     int lowQubits = 4;     // always-local low qubits: tile rows are 2^4 contiguous amplitudes
     int maxQueue = 512;    // flush when this many gates are queued
     int minQubits = 1;
+    bool prefetch = true;  // tile passes prefetch each workgroup's next tile into registers
+    int blockMinGates = 0; // complex64 states: runs of at least this many gates on 5 qubits are
+                           // multiplied into one 32x32 block applied with FP32 MFMA (0 = off)
 };
 
 bool enabledFor(void* amps, int prec, int numQubits);

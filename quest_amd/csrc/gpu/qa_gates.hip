@@ -92,7 +92,7 @@ struct M2 {
 template <typename T, int IT, int NT>
 __global__ void __launch_bounds__(BLOCK) k_oneTargDense(T* __restrict__ amps, uint64_t numWork,
                                                        InsertSpec ins, int targ, M1<T> m) {
-    const uint64_t base = (uint64_t)blockIdx.x * (BLOCK * IT) + threadIdx.x;
+    const uint64_t base = blockId() * (BLOCK * IT) + threadIdx.x;
     const uint64_t tbit = 1ull << targ;
     uint64_t i0[IT];
     ct<T> a0[IT], a1[IT];
@@ -123,7 +123,7 @@ __global__ void __launch_bounds__(BLOCK) k_oneTargDense(T* __restrict__ amps, ui
 template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_twoTargDense(T* __restrict__ amps, uint64_t numWork,
                                                        InsertSpec ins, int targ1, int targ2, M2<T> m) {
-    const uint64_t base = (uint64_t)blockIdx.x * (BLOCK * 2) + threadIdx.x;
+    const uint64_t base = blockId() * (BLOCK * 2) + threadIdx.x;
     const uint64_t b1 = 1ull << targ1, b2 = 1ull << targ2;
     const uint64_t off[4] = {0, b1, b2, b1 | b2};
 #pragma unroll
@@ -151,7 +151,7 @@ __global__ void __launch_bounds__(BLOCK) k_fewTargDense(T* __restrict__ amps, ui
                                                        InsertSpec ins, TargSpec ts,
                                                        const double* __restrict__ mat, int conj) {
     constexpr int D = 1 << K;
-    const uint64_t n = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const uint64_t n = blockId() * BLOCK + threadIdx.x;
     if (n >= numWork) return;
     const uint64_t i0 = insertBits(n, ins);
     uint64_t off[D];
@@ -210,7 +210,7 @@ __global__ void __launch_bounds__(BLOCK) k_manyTargDense(T* __restrict__ amps, u
 template <typename T, int IT, int NT>
 __global__ void __launch_bounds__(BLOCK) k_oneTargDiag(T* __restrict__ amps, uint64_t numWork,
                                                       InsertSpec ins, int targ, ct<T> d0, ct<T> d1) {
-    const uint64_t base = (uint64_t)blockIdx.x * (BLOCK * IT) + threadIdx.x;
+    const uint64_t base = blockId() * (BLOCK * IT) + threadIdx.x;
     uint64_t idx[IT];
     ct<T> a[IT];
 #pragma unroll
@@ -235,7 +235,7 @@ template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_twoTargDiag(T* __restrict__ amps, uint64_t numWork,
                                                       InsertSpec ins, int targ1, int targ2, ct<T> d0,
                                                       ct<T> d1, ct<T> d2, ct<T> d3) {
-    const uint64_t base = (uint64_t)blockIdx.x * (BLOCK * ITEMS) + threadIdx.x;
+    const uint64_t base = blockId() * (BLOCK * ITEMS) + threadIdx.x;
 #pragma unroll
     for (int j = 0; j < ITEMS; j++) {
         uint64_t n = base + (uint64_t)j * BLOCK;
@@ -254,7 +254,7 @@ __global__ void __launch_bounds__(BLOCK) k_anyTargDiag(T* __restrict__ amps, uin
                                                       InsertSpec ins, TargSpec ts,
                                                       const double* __restrict__ elems, int conj,
                                                       int hasPower, double expRe, double expIm) {
-    const uint64_t n = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const uint64_t n = blockId() * BLOCK + threadIdx.x;
     if (n >= numWork) return;
     uint64_t i = insertBits(n, ins);
     uint64_t t = valueOfBits(i, ts);
@@ -277,7 +277,7 @@ __global__ void __launch_bounds__(BLOCK) k_allTargDiag(T* __restrict__ amps, uin
                                                       const double* __restrict__ elems, int hasPower,
                                                       double expRe, double expIm, int densN,
                                                       int multiplyOnly) {
-    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const uint64_t i = blockId() * BLOCK + threadIdx.x;
     if (i >= numAmps) return;
     auto elem = [&](uint64_t k) {
         double er = elems[2 * k], ei = elems[2 * k + 1];
@@ -307,7 +307,7 @@ __global__ void __launch_bounds__(BLOCK) k_allTargDiag(T* __restrict__ amps, uin
 template <typename T, int IT, int NT>
 __global__ void __launch_bounds__(BLOCK) k_swap(T* __restrict__ amps, uint64_t numWork,
                                                InsertSpec ins, uint64_t flip) {
-    const uint64_t base = (uint64_t)blockIdx.x * (BLOCK * IT) + threadIdx.x;
+    const uint64_t base = blockId() * (BLOCK * IT) + threadIdx.x;
     uint64_t ia[IT];
     ct<T> a[IT], b[IT];
 #pragma unroll
@@ -336,7 +336,7 @@ template <typename T, int IT, int NT>
 __global__ void __launch_bounds__(BLOCK) k_pauli(T* __restrict__ amps, uint64_t numWork,
                                                 InsertSpec ins, uint64_t maskXY, uint64_t maskYZ,
                                                 ct<T> ampFac, ct<T> pairFac) {
-    const uint64_t base = (uint64_t)blockIdx.x * (BLOCK * IT) + threadIdx.x;
+    const uint64_t base = blockId() * (BLOCK * IT) + threadIdx.x;
     uint64_t ia[IT];
     ct<T> a[IT], b[IT];
 #pragma unroll
@@ -370,7 +370,7 @@ template <typename T, int IT, int NT>
 __global__ void __launch_bounds__(BLOCK) k_phaseGadget(T* __restrict__ amps, uint64_t numWork,
                                                       InsertSpec ins, uint64_t targMask, ct<T> f0,
                                                       ct<T> f1) {
-    const uint64_t base = (uint64_t)blockIdx.x * (BLOCK * IT) + threadIdx.x;
+    const uint64_t base = blockId() * (BLOCK * IT) + threadIdx.x;
     uint64_t idx[IT];
     ct<T> a[IT];
 #pragma unroll
@@ -412,7 +412,7 @@ template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_superpose(T* __restrict__ out, const T* __restrict__ in1,
                                                     const T* __restrict__ in2, uint64_t numAmps,
                                                     ct<T> fo, ct<T> f1, ct<T> f2) {
-    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const uint64_t i = blockId() * BLOCK + threadIdx.x;
     if (i >= numAmps) return;
     ct<T> r = cmulT(fo, ld(out, i));
     r = cfmaT(f1, ld(in1, i), r);
@@ -439,7 +439,7 @@ void launchTwoTargDense(void* amps, uint64_t work, const InsertSpec& ins, int t1
 template <typename T>
 int launchAnyTargDense(void* amps, uint64_t work, const InsertSpec& ins, const TargSpec& ts,
                        const double* mat, int conj) {
-    unsigned g = gridFor(work, 1);
+    dim3 g = gridFor(work, 1);
     switch (ts.num) {
         case 3: k_fewTargDense<T, 3><<<g, BLOCK, 0, stream()>>>((T*)amps, work, ins, ts, mat, conj); break;
         case 4: k_fewTargDense<T, 4><<<g, BLOCK, 0, stream()>>>((T*)amps, work, ins, ts, mat, conj); break;

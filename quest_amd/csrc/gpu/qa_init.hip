@@ -11,7 +11,7 @@ namespace {
 
 template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_fill(T* __restrict__ amps, uint64_t num, T re, T im) {
-    const uint64_t base = (uint64_t)blockIdx.x * (BLOCK * ITEMS) + threadIdx.x;
+    const uint64_t base = blockId() * (BLOCK * ITEMS) + threadIdx.x;
 #pragma unroll
     for (int j = 0; j < ITEMS; j++) {
         uint64_t i = base + (uint64_t)j * BLOCK;
@@ -22,7 +22,7 @@ __global__ void __launch_bounds__(BLOCK) k_fill(T* __restrict__ amps, uint64_t n
 // a_i = (2i/10, (2i+1)/10)  (cpu_subroutines.cpp: cpu_statevec_initDebugState_sub)
 template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_debug(T* __restrict__ amps, uint64_t num) {
-    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const uint64_t i = blockId() * BLOCK + threadIdx.x;
     if (i >= num) return;
     double re = (double)(2 * i) / 10.;
     double im = (double)(2 * i + 1) / 10.;
@@ -43,7 +43,7 @@ __device__ __forceinline__ double unit(uint64_t x) {  // (0, 1]
 // the distribution of randomiser.cpp rand_getThreadPrivateRandomAmp
 template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_random(T* __restrict__ amps, uint64_t num, uint64_t seed) {
-    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const uint64_t i = blockId() * BLOCK + threadIdx.x;
     if (i >= num) return;
     uint64_t k = seed * 0x9e3779b97f4a7c15ull + i * 0xd1b54a32d192ed03ull;
     double u1 = unit(mix64(k ^ 0x1ull)), u3 = unit(mix64(k ^ 0x3ull));
@@ -57,7 +57,7 @@ __global__ void __launch_bounds__(BLOCK) k_random(T* __restrict__ amps, uint64_t
 template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_projSV(T* __restrict__ amps, uint64_t num, uint64_t qubitMask,
                                                  uint64_t outcomeMask, T renorm) {
-    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const uint64_t i = blockId() * BLOCK + threadIdx.x;
     if (i >= num) return;
     T f = ((i & qubitMask) == outcomeMask) ? renorm : (T)0;
     ct<T> a = ld(amps, i);
@@ -68,7 +68,7 @@ __global__ void __launch_bounds__(BLOCK) k_projSV(T* __restrict__ amps, uint64_t
 template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_projDM(T* __restrict__ amps, uint64_t num, int N, uint64_t qubitMask,
                                                  uint64_t outcomeMask, T renorm) {
-    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const uint64_t i = blockId() * BLOCK + threadIdx.x;
     if (i >= num) return;
     uint64_t r = i & ((1ull << N) - 1), c = i >> N;
     bool keep = ((r & qubitMask) == outcomeMask) && ((c & qubitMask) == outcomeMask);

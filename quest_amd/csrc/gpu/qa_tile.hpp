@@ -34,7 +34,8 @@ namespace fuse {
 
 constexpr int RB = 4;        // register slots per group
 constexpr int RA = 1 << RB;  // amplitudes per thread
-enum { F_GROUP = 100 };
+enum { F_GROUP = 100, F_BLOCK = 101, F_DENSE1N = 102 };
+constexpr int BQ = 5;  // qubits of an MFMA block (32 x 32 complex)
 
 // the first 64 bytes of an op, fetched with one scalar load
 struct TOpHead {
@@ -121,6 +122,7 @@ struct TileArgs {
     int lastDirect;   // the last group stores its layout to HBM
     int usesLds;      // some layout change goes through LDS
     uint64_t numTiles;
+    const float* blocks;    // packed block matrices of the pass's BLOCK ops
     int localPos[MAXQ];     // global positions of the tile slots (ascending)
     int nonLocalPos[MAXQ];  // global positions of the other qubits (ascending)
 };
@@ -402,6 +404,29 @@ __device__ __forceinline__ void applyOpC(ct<T>* r, const TOp* __restrict__ op, c
                 case 3: regDense1<T, 3, C>(r, h, f); break;
             }
             break;
+        case F_DENSE1N: {
+            // N = rb2 uncontrolled one-qubit matrices on register bits 0..N-1
+            // in that order: straight-line code, one dispatch for N gates
+            const double* fx = op->fx;
+            switch (h.rb2) {
+                case 2:
+                    regDense1<T, 0, false>(r, h, f);
+                    regDense1<T, 1, false>(r, h, fx);
+                    break;
+                case 3:
+                    regDense1<T, 0, false>(r, h, f);
+                    regDense1<T, 1, false>(r, h, fx);
+                    regDense1<T, 2, false>(r, h, fx + 8);
+                    break;
+                case 4:
+                    regDense1<T, 0, false>(r, h, f);
+                    regDense1<T, 1, false>(r, h, fx);
+                    regDense1<T, 2, false>(r, h, fx + 8);
+                    regDense1<T, 3, false>(r, h, fx + 16);
+                    break;
+            }
+            break;
+        }
         case F_DENSE2: {
             const double* fx = op->fx;
             switch (h.rb1 * RB + h.rb2) {
@@ -466,6 +491,70 @@ __device__ __forceinline__ void applyOp(ct<T>* r, const TOp* __restrict__ op, co
 
 __device__ __forceinline__ const int* groupSlots(const TOp& op) { return reinterpret_cast<const int*>(op.b); }
 
+// ---- FP32 MFMA blocks ---------------------------------------------------------
+//
+// A BLOCK op applies a 32x32 complex matrix U to the 5 tile slots in its
+// body (b[0..2] as ints; b[3] = offset of its packed matrix in
+// TileArgs::blocks) for every column of the tile (the 32 amplitudes that
+// share the other tile bits).  In real form Y = [[Re U, -Im U], [Im U, Re U]]
+// [Re X; Im X], 64 x 64 times 64 x 32 columns per wave, as 32 K-steps of
+// v_mfma_f32_32x32x2f32 for each of the two 32-row halves.  Lane l holds
+// column l%32 of the wave's 32 columns; for K-step s it supplies
+// A[l%32 (+32)][2s + l/32] (packed host-side at [s][half][lane]) and
+// X[2s + l/32][l%32]; result register i of lane l is row 8(i/4) + 4(l/32) +
+// i%4.  The tile is in LDS; waves = columns / 32 exactly (tiles of 2^k,
+// k >= 10, with 2^(k-4) threads).
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ void runBlock32(ct<float>* tile, int k, const int* slots, const float* __restrict__ A,
+                                           uint32_t tid) {
+    const uint32_t lane = tid & 63u, h = lane >> 5;
+    const uint32_t col = (tid >> 6) * 32u + (lane & 31u);
+    uint32_t sb[BQ], sm = 0;
+#pragma unroll
+    for (int b = 0; b < BQ; b++) {
+        sb[b] = 1u << slots[b];
+        sm |= sb[b];
+    }
+    uint32_t base = 0;
+    for (int s = 0, i = 0; s < k; s++) {
+        if ((sm >> s) & 1u) continue;
+        base |= ((col >> i) & 1u) << s;
+        i++;
+    }
+    const uint32_t hb = h ? sb[0] : 0u;
+    float xr[16], xi[16];
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+        // amplitude 2t + h of the column
+        uint32_t j = base | hb;
+#pragma unroll
+        for (int b = 1; b < BQ; b++)
+            if ((t >> (b - 1)) & 1) j |= sb[b];
+        const ct<float> v = tile[swz(j)];
+        xr[t] = v.re;
+        xi[t] = v.im;
+    }
+    f32x16 dTop = {0}, dBot = {0};
+#pragma unroll
+    for (int s = 0; s < 32; s++) {
+        const float aTop = A[(2 * s) * 64 + lane], aBot = A[(2 * s + 1) * 64 + lane];
+        const float x = s < 16 ? xr[s] : xi[s - 16];
+        dTop = __builtin_amdgcn_mfma_f32_32x32x2f32(aTop, x, dTop, 0, 0, 0);
+        dBot = __builtin_amdgcn_mfma_f32_32x32x2f32(aBot, x, dBot, 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const int rowConst = 8 * (i / 4) + (i % 4);  // plus 4h
+        uint32_t j = base;
+#pragma unroll
+        for (int b = 0; b < BQ; b++)
+            if ((rowConst >> b) & 1) j |= sb[b];
+        if (h) j |= sb[2];  // the 4h term is bit 2 of the row
+        tile[swz(j)] = ct<float>{dTop[i], dBot[i]};
+    }
+}
+
 // the tile's load layout: the first group's when it loads HBM directly,
 // else the row layout whose register slots are the top RB slots
 __device__ __forceinline__ void passLayouts(RegLayout& row, RegLayout& first, const TileArgs& a,
@@ -491,7 +580,17 @@ __device__ __forceinline__ void tileBody(T* __restrict__ amps, const TileArgs& a
         fetchOp(ops + o, ch, cb);
         OpRegs op;
         unpackOp(ch, cb, op);
-        if (op.h.kind == F_GROUP) {
+        if (op.h.kind == F_BLOCK) {
+            if constexpr (sizeof(T) == 4) {
+                if (inRegs) RegIO<T, NT, 0>::ldsStore(r, tile, cur);
+                __syncthreads();
+                const int bslots[BQ] = {(int)cb[0], (int)cb[1], (int)cb[2], (int)cb[3], (int)cb[4]};
+                runBlock32(reinterpret_cast<ct<float>*>(tile), a.k, bslots,
+                           a.blocks + (cb[6] | ((uint64_t)cb[7] << 32)), tid);
+                __syncthreads();
+                inRegs = false;
+            }
+        } else if (op.h.kind == F_GROUP) {
             int slots[RB];
 #pragma unroll
             for (int b = 0; b < RB; b++) slots[b] = (int)cb[b];
@@ -508,7 +607,7 @@ __device__ __forceinline__ void tileBody(T* __restrict__ amps, const TileArgs& a
     if (a.lastDirect) {
         RegIO<T, NT, 0>::hbmStore(r, amps, base | cur.tg, cur);
     } else {
-        RegIO<T, NT, 0>::ldsStore(r, tile, cur);
+        if (inRegs) RegIO<T, NT, 0>::ldsStore(r, tile, cur);
         __syncthreads();
         RegIO<T, NT, 0>::ldsLoad(r, tile, row);
         RegIO<T, NT, 0>::hbmStore(r, amps, base | row.tg, row);
@@ -535,6 +634,13 @@ size_t takeRun(const GOp* ops, size_t num, int numQubits, int kMax, int low, uin
 // and appends the ops, with their GROUP ops, to `prog`
 void planPass(const GOp* ops, size_t num, int numQubits, uint64_t localMask, int lowQubits, TileArgs& a,
               std::vector<TOp>& prog);
+
+// the same for a complex64 state, where runs of at least blockMinGates ops
+// whose qubits (targets, controls, masks) fit 5 local qubits become one
+// BLOCK op; their packed 32x32 matrices are appended to `blocks` and each
+// BLOCK op holds its offset there (a.blocks is left for the caller)
+void planPassBlocks(const GOp* ops, size_t num, int numQubits, uint64_t localMask, int lowQubits, int blockMinGates,
+                    TileArgs& a, std::vector<TOp>& prog, std::vector<float>& blocks);
 
 }  // namespace fuse
 }  // namespace qa

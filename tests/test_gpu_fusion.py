@@ -250,3 +250,55 @@ def test_fusion_config_validation(api):
             api.set_fusion(True, *bad)
     api.set_fusion(True, 10, 2, 64)
     assert api.get_fusion() == (1, 10, 2, 64)
+
+
+@pytest.mark.parametrize("min_gates", [2, 4])
+@pytest.mark.parametrize("dens", [False, True])
+def test_fp32_mfma_blocks_track_oracle(api, min_gates, dens):
+    """complex64 states with runs of gates on 5 qubits multiplied into
+    32x32 blocks applied with FP32 MFMA: within 1e-5 (relative L2 of the
+    amplitudes, absolute on results) of the FP64 oracle, for tiles of 2^10
+    to 2^12 amplitudes."""
+    from oracle import qoracle as O
+    n = 7 if dens else 15
+    say(f"[gpu] fusion fp32 mfma blocks n={n} dens={dens} min_gates={min_gates}")
+    c = fusion_mix(n, 160, 81 + dens, dens=dens)
+    want, wres = O.run(c)
+    lib = api.gpu()
+    for tile in (10, 12):
+        api.set_fusion(True, tile, 4, 512)
+        assert lib.qa_setFusionBlocks(min_gates) == 0
+        try:
+            q = api.createCustomPrecisionQureg(n, int(dens), 1)
+            q, res = qc.run(c, q)
+            got = api.get_amps(q)
+            api.destroyQureg(q)
+        finally:
+            lib.qa_setFusionBlocks(0)
+        assert rel_l2(got, want) <= 1e-5, tile
+        compare_results(res, wres, tol=1e-5)
+
+
+def test_fp32_mfma_blocks_layered_family(api):
+    """The plain-method family (one-qubit Haar gates and two-qubit phases)
+    at 16 qubits: the blocks path and the gate-by-gate complex64 path agree
+    to 1e-5 and both track the complex128 oracle."""
+    from oracle import qoracle as O
+    say("[gpu] fusion fp32 mfma layered")
+    c = qc.layered(16, 6, 91, probs=True, num_expec=2)
+    want, wres = O.run(c)
+    lib = api.gpu()
+    api.set_fusion(True, 12, 4, 512)
+    outs = []
+    for mg in (0, 3):
+        assert lib.qa_setFusionBlocks(mg) == 0
+        try:
+            q = api.createCustomPrecisionQureg(16, 0, 1)
+            q, res = qc.run(c, q)
+            outs.append(api.get_amps(q))
+            api.destroyQureg(q)
+        finally:
+            lib.qa_setFusionBlocks(0)
+        assert rel_l2(outs[-1], want) <= 1e-5
+        compare_results(res, wres, tol=1e-5)
+    assert rel_l2(outs[0], outs[1]) <= 1e-5

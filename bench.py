@@ -428,6 +428,9 @@ def main() -> int:
     ap.add_argument("--block-min-gates", type=int, default=4,
                     help="precision mode: runs of at least this many gates on 5 qubits become FP32 MFMA blocks")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibration-copy", action="store_true",
+                    help="copy the state once (one device copy dispatch of known size) before the timed steps, "
+                         "as the calibration point of rocprofv3 FETCH_SIZE / WRITE_SIZE runs")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -484,6 +487,10 @@ def main() -> int:
     calls = build_step(Q, qc, circ, q)
     gpu = Q.gpu()
     import ctypes as C
+
+    if args.calibration_copy:
+        Q.destroyQureg(Q.createCloneQureg(q))
+        Q.syncQuESTEnv()
 
     for w in range(args.warmup):
         t = time.perf_counter()

@@ -43,7 +43,7 @@ namespace fuse {
 // more registers of amplitudes per thread); PF = 0: load, work, store.
 template <typename T, int NT, int PF>
 __device__ __forceinline__ void tilePass(T* __restrict__ amps, const TileArgs& a, const TOp* __restrict__ ops) {
-    extern __shared__ unsigned char smem[];
+    extern __shared__ __align__(16) unsigned char smem[];
     ct<T>* tile = reinterpret_cast<ct<T>*>(smem);
     const uint32_t tid = threadIdx.x;
     RegLayout row, first;
@@ -311,15 +311,53 @@ void passShape(int numQubits, uint64_t localMask, TileArgs& a, Layout& L) {
 struct GroupState {
     int numGroups = 0;
     uint32_t firstG = 0, lastG = 0;
+    bool anyItem = false;    // a group or an LDS op was emitted
+    bool firstIsLds = false, lastIsLds = false;  // the pass starts / ends with an LDS op
 };
+
+// a GROUP op whose layout is never used: keeps ops[0] a layout when a pass
+// starts with an op that works on LDS
+TOp placeholderGroup(int k) {
+    TOp grp;
+    std::memset(&grp, 0, sizeof(grp));
+    grp.h.kind = F_GROUP;
+    int* slots = reinterpret_cast<int*>(grp.b);
+    for (int b = 0; b < RB; b++) slots[b] = k - RB + b;
+    return grp;
+}
+
+// a Pauli tensor/gadget with more X/Y qubits than register bits: applied
+// to the tile in LDS
+bool isLdsOp(const GOp& g) { return g.kind == F_PAULI && __builtin_popcountll(g.m1) > RB; }
+
+TOp ldsPauli(const GOp& g, const Layout& L) {
+    Layout T = L;  // every local bit counted in the tile part
+    for (int s = 0; s < MAXQ; s++) T.regOf[s] = -1;
+    TOp t = toTile(g, T);
+    t.h.kind = F_PAULI_LDS;
+    uint32_t r;
+    uint64_t tile, nonlocal;
+    splitMask(g.m1, T, r, tile, nonlocal);
+    t.b[6] = tile;  // X/Y bits (all local)
+    return t;
+}
 
 void emitGroups(const GOp* ops, size_t from, size_t to, Layout& L, int k, int low, std::vector<TOp>& prog,
                 GroupState& st) {
     size_t i = from;
     while (i < to) {
+        if (isLdsOp(ops[i])) {
+            prog.push_back(ldsPauli(ops[i], L));
+            if (st.numGroups == 0 && !st.anyItem) st.firstIsLds = true;
+            st.lastIsLds = true;
+            st.anyItem = true;
+            i++;
+            continue;
+        }
         uint32_t G = 0;
         size_t j = i;
         for (; j < to; j++) {
+            if (isLdsOp(ops[j])) break;
             uint32_t u = G | regNeed(ops[j], L);
             if (__builtin_popcount(u) > RB) break;
             G = u;
@@ -351,6 +389,8 @@ void emitGroups(const GOp* ops, size_t from, size_t to, Layout& L, int k, int lo
         if (st.numGroups == 0) st.firstG = G;
         st.lastG = G;
         st.numGroups++;
+        st.lastIsLds = false;
+        st.anyItem = true;
         while (i < j) {
             // a run of 2-4 uncontrolled one-qubit gates on register bits
             // 0, 1, ... in order becomes one DENSE1N op (same arithmetic)
@@ -539,9 +579,10 @@ void fuse::planPass(const GOp* ops, size_t num, int numQubits, uint64_t localMas
     const size_t start = prog.size();
     GroupState st;
     emitGroups(ops, 0, num, L, k, low, prog, st);
-    a.firstDirect = (st.firstG & lowSlots) == 0;
-    a.lastDirect = (st.lastG & lowSlots) == 0;
-    a.usesLds = !a.firstDirect || !a.lastDirect || st.numGroups > 1;
+    if (st.firstIsLds) prog.insert(prog.begin() + start, placeholderGroup(k));
+    a.firstDirect = !st.firstIsLds && (st.firstG & lowSlots) == 0;
+    a.lastDirect = !st.lastIsLds && (st.lastG & lowSlots) == 0;
+    a.usesLds = !a.firstDirect || !a.lastDirect || st.numGroups > 1 || st.firstIsLds || st.lastIsLds;
     a.numOps = (int)(prog.size() - start);
 }
 
@@ -592,17 +633,12 @@ void fuse::planPassBlocks(const GOp* ops, size_t num, int numQubits, uint64_t lo
         emitGroups(ops, pending, num, L, k, low, prog, st);
         lastItemGroup = true;
     }
+    if (st.firstIsLds && firstItemGroup) firstItemGroup = false;
+    if (st.lastIsLds) lastItemGroup = false;
     // a pass that starts or ends with a BLOCK moves the tile through LDS in
     // the row layout; one that starts with a BLOCK still begins with a GROUP
     // op (its layout is never used), so ops[0] is always a layout
-    if (!firstItemGroup) {
-        TOp grp;
-        std::memset(&grp, 0, sizeof(grp));
-        grp.h.kind = F_GROUP;
-        int* slots = reinterpret_cast<int*>(grp.b);
-        for (int b = 0; b < RB; b++) slots[b] = k - RB + b;
-        prog.insert(prog.begin() + start, grp);
-    }
+    if (!firstItemGroup) prog.insert(prog.begin() + start, placeholderGroup(k));
     a.firstDirect = firstItemGroup && (st.firstG & lowSlots) == 0;
     a.lastDirect = lastItemGroup && (st.lastG & lowSlots) == 0;
     a.usesLds = true;
@@ -759,7 +795,10 @@ int enqueue(void* amps, int prec, int numQubits, const GOp& op, bool& accepted) 
     int kMax = std::min(numQubits, g_cfg.tileQubits);
     int lowCount = std::min(g_cfg.lowQubits, kMax);
     uint64_t lowMask = (1ull << lowCount) - 1ull;
-    if (__builtin_popcountll(denseMask(op) | lowMask) > kMax || __builtin_popcountll(denseMask(op)) > RB)
+    // (a Pauli tensor or gadget with more than RB X/Y qubits is applied to
+    // the LDS copy of the tile instead of in registers)
+    if (__builtin_popcountll(denseMask(op) | lowMask) > kMax ||
+        (__builtin_popcountll(denseMask(op)) > RB && op.kind != F_PAULI))
         return QA_OK;
     qu.ops.push_back(op);
     accepted = true;

@@ -81,7 +81,7 @@ struct RegZero<T, RA> {
 template <typename T>
 __global__ void __launch_bounds__(512) k_batchPass(const PassDesc* __restrict__ passes, int numPasses,
                                                    const TOp* __restrict__ ops, uint64_t totalTiles) {
-    extern __shared__ unsigned char smem[];
+    extern __shared__ __align__(16) unsigned char smem[];
     ct<T>* tile = reinterpret_cast<ct<T>*>(smem);
     const uint32_t tid = threadIdx.x;
     for (uint64_t g = blockIdx.x; g < totalTiles; g += gridDim.x) {

@@ -34,7 +34,7 @@ namespace fuse {
 
 constexpr int RB = 4;        // register slots per group
 constexpr int RA = 1 << RB;  // amplitudes per thread
-enum { F_GROUP = 100, F_BLOCK = 101, F_DENSE1N = 102 };
+enum { F_GROUP = 100, F_BLOCK = 101, F_DENSE1N = 102, F_PAULI_LDS = 103 };
 constexpr int BQ = 5;  // qubits of an MFMA block (32 x 32 complex)
 
 // the first 64 bytes of an op, fetched with one scalar load
@@ -185,14 +185,30 @@ __device__ __forceinline__ uint64_t offQ(const RegLayout& L) {
     return o;
 }
 
+// LDS tile accesses move a whole amplitude per instruction (16 bytes in
+// FP64, ds_read_b128 / ds_write_b128): split 8-byte halves would make lanes
+// 16 apart meet in the same banks under the row swizzle
+template <typename T>
+__device__ __forceinline__ ct<T> ldsGet(const ct<T>* tile, uint32_t i) {
+    const typename Amp<T>::V x = reinterpret_cast<const typename Amp<T>::V*>(tile)[i];
+    return ct<T>{x.x, x.y};
+}
+template <typename T>
+__device__ __forceinline__ void ldsPut(ct<T>* tile, uint32_t i, ct<T> a) {
+    typename Amp<T>::V x;
+    x.x = a.re;
+    x.y = a.im;
+    reinterpret_cast<typename Amp<T>::V*>(tile)[i] = x;
+}
+
 template <typename T, int NT, int V>
 struct RegIO {
     __device__ __forceinline__ static void ldsLoad(ct<T>* r, const ct<T>* tile, const RegLayout& L) {
-        r[V] = tile[swz(L.jb | offJ<V>(L))];
+        r[V] = ldsGet(tile, swz(L.jb | offJ<V>(L)));
         RegIO<T, NT, V + 1>::ldsLoad(r, tile, L);
     }
     __device__ __forceinline__ static void ldsStore(const ct<T>* r, ct<T>* tile, const RegLayout& L) {
-        tile[swz(L.jb | offJ<V>(L))] = r[V];
+        ldsPut(tile, swz(L.jb | offJ<V>(L)), r[V]);
         RegIO<T, NT, V + 1>::ldsStore(r, tile, L);
     }
     __device__ __forceinline__ static void hbmLoad(ct<T>* r, const T* amps, uint64_t at, const RegLayout& L) {
@@ -491,6 +507,30 @@ __device__ __forceinline__ void applyOp(ct<T>* r, const TOp* __restrict__ op, co
 
 __device__ __forceinline__ const int* groupSlots(const TOp& op) { return reinterpret_cast<const int*>(op.b); }
 
+// A Pauli tensor / gadget with more X/Y qubits than register bits, applied
+// to the tile in LDS: every thread takes pairs (jA, jA ^ xy) with the top
+// X/Y bit of jA clear; same arithmetic as the single-gate kernel
+template <typename T>
+__device__ __forceinline__ void ldsPauli(ct<T>* tile, int k, const OpRegs& o, uint64_t xy, uint64_t base,
+                                         uint32_t tid, uint32_t nt) {
+    const int top = 63 - __clzll((long long)xy);
+    const ct<T> ampFac = cf<T>(o.f, 0), pairFac = cf<T>(o.f, 1);
+    const int basePar = parity64(base & o.gm);
+    const uint32_t half = 1u << (k - 1);
+    for (uint32_t q = tid; q < half; q += nt) {
+        const uint32_t jA = ((q >> top) << (top + 1)) | (q & ((1u << top) - 1u));
+        if ((jA & o.h.jcMask) != o.h.jcVal) continue;
+        const uint32_t jB = jA ^ (uint32_t)xy;
+        const ct<T> va = ldsGet(tile, swz(jA)), vb = ldsGet(tile, swz(jB));
+        const T sA = (parity64(jA & o.jm) ^ basePar) ? (T)-1 : (T)1;
+        const T sB = (parity64(jB & o.jm) ^ basePar) ? (T)-1 : (T)1;
+        const ct<T> pB = {pairFac.re * sB, pairFac.im * sB};
+        const ct<T> pA = {pairFac.re * sA, pairFac.im * sA};
+        ldsPut(tile, swz(jA), cfmaT(pB, vb, cmulT(ampFac, va)));
+        ldsPut(tile, swz(jB), cfmaT(pA, va, cmulT(ampFac, vb)));
+    }
+}
+
 // ---- FP32 MFMA blocks ---------------------------------------------------------
 //
 // A BLOCK op applies a 32x32 complex matrix U to the 5 tile slots in its
@@ -531,7 +571,7 @@ __device__ __forceinline__ void runBlock32(ct<float>* tile, int k, const int* sl
 #pragma unroll
         for (int b = 1; b < BQ; b++)
             if ((t >> (b - 1)) & 1) j |= sb[b];
-        const ct<float> v = tile[swz(j)];
+        const ct<float> v = ldsGet(tile, swz(j));
         xr[t] = v.re;
         xi[t] = v.im;
     }
@@ -551,7 +591,7 @@ __device__ __forceinline__ void runBlock32(ct<float>* tile, int k, const int* sl
         for (int b = 0; b < BQ; b++)
             if ((rowConst >> b) & 1) j |= sb[b];
         if (h) j |= sb[2];  // the 4h term is bit 2 of the row
-        tile[swz(j)] = ct<float>{dTop[i], dBot[i]};
+        ldsPut(tile, swz(j), ct<float>{dTop[i], dBot[i]});
     }
 }
 
@@ -580,7 +620,14 @@ __device__ __forceinline__ void tileBody(T* __restrict__ amps, const TileArgs& a
         fetchOp(ops + o, ch, cb);
         OpRegs op;
         unpackOp(ch, cb, op);
-        if (op.h.kind == F_BLOCK) {
+        if (op.h.kind == F_PAULI_LDS) {
+            if (inRegs) RegIO<T, NT, 0>::ldsStore(r, tile, cur);
+            __syncthreads();
+            if ((base & op.h.gcMask) == op.h.gcVal)
+                ldsPauli<T>(tile, a.k, op, ((uint64_t)cb[13] << 32) | cb[12], base, tid, blockDim.x);
+            __syncthreads();
+            inRegs = false;
+        } else if (op.h.kind == F_BLOCK) {
             if constexpr (sizeof(T) == 4) {
                 if (inRegs) RegIO<T, NT, 0>::ldsStore(r, tile, cur);
                 __syncthreads();

@@ -302,3 +302,40 @@ def test_fp32_mfma_blocks_layered_family(api):
         assert rel_l2(outs[-1], want) <= 1e-5
         compare_results(res, wres, tol=1e-5)
     assert rel_l2(outs[0], outs[1]) <= 1e-5
+
+
+@pytest.mark.parametrize("shape", [(12, 4), (8, 2), (13, 5)])
+@pytest.mark.parametrize("dens", [False, True])
+def test_wide_pauli_gadgets_fused_in_lds(api, shape, dens):
+    """Pauli tensors and gadgets on more X/Y qubits than a thread's register
+    bits are applied to the tile in LDS: bit-identical to the single-gate
+    kernel, also with controls and when they open or close a pass."""
+    from oracle import qoracle as O
+    n = 6 if dens else 14
+    say(f"[gpu] fusion wide pauli n={n} dens={dens} shape={shape}")
+    rng = qc.SplitMix64(97 + dens)
+    c = qc.Circuit(n, dens, [], ("zero",) if dens else ("random", 5), "wide_pauli")
+    c.ops.append(("PG", "XYXYXYX"[: min(7, n)], list(range(min(7, n))), rng.angle()))
+    for i in range(40):
+        k = 5 + rng.randint(min(6, n - 5))
+        targs = rng.sample(n, k)
+        chars = "".join("XY"[rng.randint(2)] for _ in range(k))
+        if i % 3 == 0:
+            c.ops.append(("PG", chars, targs, rng.angle()))
+        elif i % 3 == 1:
+            rest = [q for q in range(n) if q not in targs]
+            ctrls = [rest[j] for j in rng.sample(len(rest), 1)] if rest else []
+            c.ops.append(("CPG", ctrls, chars, targs, rng.angle()) if ctrls else ("PSTR", chars, targs))
+        else:
+            c.ops.append(("PSTR", chars, targs))
+        t = rng.randint(n)
+        c.ops.append(("U1", t, qc.haar_unitary(rng, 2)))
+    c.ops.append(("PG", "YZXY"[: min(4, n)] + "X", list(range(min(4, n))) + [n - 1], rng.angle()))
+    c.ops.append(("OUT_TOTALPROB",))
+    c.ops.append(("OUT_EXPEC", qc.random_pauli_chars(rng, n), list(range(n))))
+    ref, rres = run_amps(api, c, None)
+    got, gres = run_amps(api, c, shape)
+    assert np.array_equal(got, ref), f"max |diff| {np.max(np.abs(got - ref))}"
+    compare_results(gres, rres, tol=0.0)
+    want, wres = O.run(c)
+    assert rel_l2(got, want) <= AMP_TOL

@@ -119,3 +119,28 @@ def test_no_cpu_fallback_without_gpu():
         pytest.skip("a GPU is visible here")
     with pytest.raises(Q.QuESTError, match="GPU"):
         Q.initQuESTEnv()
+
+
+def test_serve_request_packing():
+    """Host-side packing of serving requests (no GPU): Pauli strings to
+    (X, Y, Z) masks, circuit ops to gate records, and the request family's
+    shape (qubits and depth within their ranges, 8 strings each)."""
+    from quest_amd import serve as S
+    assert S.pauli_masks("XYZI", [0, 1, 2, 3]) == (0b0001, 0b0010, 0b0100)
+    assert S.pauli_masks("zx", [5, 2]) == (1 << 2, 0, 1 << 5)
+    with pytest.raises(ValueError):
+        S.pauli_masks("Q", [0])
+    arr, n = S.pack_gates([("RY", 3, 0.5), ("CZ", 1, 2), ("U1", 0, [[0, 1], [1, 0]]), ("OUT_PROB", 0, 0)])
+    assert n == 3
+    assert (arr[0].kind, arr[0].t1, arr[0].angle) == (S.SG["RY"], 3, 0.5)
+    assert (arr[1].kind, arr[1].t1, arr[1].t2) == (S.SG["CZ"], 1, 2)
+    assert list(arr[2].m) == [0, 0, 1, 0, 1, 0, 0, 0]
+    with pytest.raises(ValueError):
+        S.pack_gates([("KRAUS", [0, 1], [])])
+    reqs = S.serving_requests(50, 3)
+    for c in reqs:
+        assert 10 <= c.num_qubits <= 24
+        layers = sum(1 for op in c.ops if op[0] == "RY") // c.num_qubits
+        assert 4 <= layers <= 32
+        assert len(S.paulis_of(c)) == 8
+    assert len({c.num_qubits for c in reqs}) > 5

@@ -83,6 +83,9 @@ int qa_serveWait(void* engine, int64_t id, double* expectations, double* latency
    requests completed, and the largest number running at once. */
 int qa_serveStats(void* engine, int64_t* steps, int64_t* passes, int64_t* admitted, int64_t* completed,
                   int64_t* maxRunning);
+/* Milliseconds the engine thread has spent planning and launching steps,
+   and waiting for steps to finish on the GPU. */
+int qa_serveTimes(void* engine, double* planMs, double* waitMs);
 
 #ifdef __cplusplus
 }

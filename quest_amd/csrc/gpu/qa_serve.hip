@@ -313,6 +313,7 @@ struct Engine {
     std::thread th;
 
     std::atomic<int64_t> nSteps{0}, nPasses{0}, nAdmitted{0}, nCompleted{0}, maxRunning{0};
+    std::atomic<int64_t> planNs{0}, waitNs{0};  // engine thread: planning + launching, waiting for steps
 };
 
 bool hipOk(hipError_t e, std::string& err, const char* what) {
@@ -484,8 +485,10 @@ bool retireStep(Engine& E, std::string& err) {
     const int idx = E.inflight.front();
     E.inflight.pop_front();
     StepBuf& sb = E.steps[idx];
+    const Clock::time_point t0 = Clock::now();
     if (!hipOk(hipEventSynchronize(sb.done), err, "serve: step wait")) return false;
     const Clock::time_point now = Clock::now();
+    E.waitNs += std::chrono::duration_cast<std::chrono::nanoseconds>(now - t0).count();
     {
         std::lock_guard<std::mutex> lk(E.mu);
         for (Request* r : sb.finishing) {
@@ -555,7 +558,9 @@ void engineLoop(Engine* e) {
         if (work) {
             if ((int)E.inflight.size() == NSTEP && !retireStep(E, err)) break;
             StepBuf& sb = E.steps[E.stepHead];
+            const Clock::time_point t0 = Clock::now();
             if (!launchStep(E, sb, err)) break;
+            E.planNs += std::chrono::duration_cast<std::chrono::nanoseconds>(Clock::now() - t0).count();
             E.inflight.push_back(E.stepHead);
             E.stepHead = (E.stepHead + 1) % NSTEP;
         }
@@ -826,6 +831,14 @@ int qa_serveWait(void* engine, int64_t id, double* expectations, double* latency
     if (latencyMs) *latencyMs = std::chrono::duration<double, std::milli>(r->finished - r->submitted).count();
     E->byId.erase(it);
     delete r;
+    return QA_OK;
+}
+
+int qa_serveTimes(void* engine, double* planMs, double* waitMs) {
+    QA_ARG_CHECK(engine, "qa_serveTimes: null engine");
+    Engine* E = (Engine*)engine;
+    if (planMs) *planMs = E->planNs.load() / 1e6;
+    if (waitMs) *waitMs = E->waitNs.load() / 1e6;
     return QA_OK;
 }
 

@@ -111,7 +111,11 @@ class Engine:
     def stats(self) -> dict:
         v = [C.c_int64() for _ in range(5)]
         _check(self._lib.qa_serveStats(self._h, *[C.byref(x) for x in v]), "qa_serveStats")
-        return dict(zip(["steps", "passes", "admitted", "completed", "max_running"], [x.value for x in v]))
+        out = dict(zip(["steps", "passes", "admitted", "completed", "max_running"], [x.value for x in v]))
+        pm, wm = C.c_double(), C.c_double()
+        _check(self._lib.qa_serveTimes(self._h, C.byref(pm), C.byref(wm)), "qa_serveTimes")
+        out.update(plan_ms=pm.value, wait_ms=wm.value)
+        return out
 
     def close(self) -> None:
         if self._h:
@@ -147,6 +151,7 @@ def _lib():
         lib.qa_servePoll.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_int)]
         lib.qa_serveWait.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         lib.qa_serveStats.argtypes = [C.c_void_p] + [C.POINTER(C.c_int64)] * 5
+        lib.qa_serveTimes.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         lib.qa_lastError.restype = C.c_char_p
         _LIB = lib
     return _LIB

@@ -133,10 +133,16 @@ __device__ __forceinline__ uint64_t deposit(uint64_t v, const int* pos, int num)
     return r;
 }
 
-// LDS position of tile amplitude j: the low 4 index bits are XORed with the
-// next 4, so threads whose amplitudes are 16 apart (register slots 0..3)
+// LDS position of tile amplitude j: the low index bits are XORed with the
+// next ones, so threads whose amplitudes are 16 apart (register slots 0..3)
 // still spread over all banks
-__device__ __forceinline__ uint32_t swz(uint32_t j) { return j ^ ((j >> 4) & 15u); }
+// (complex128: 16 bytes, 16 amplitudes span the 64 four-byte banks; complex64:
+// 8 bytes, so 32 amplitudes do, and 5 bits are XORed with the next 5)
+template <typename T>
+__device__ __forceinline__ uint32_t swz(uint32_t j) {
+    if constexpr (sizeof(T) == 8) return j ^ ((j >> 4) & 15u);
+    else return j ^ ((j >> 5) & 31u);
+}
 
 // a register layout: tile slots (sb) and global bits (qb) of the register
 // bits, and this thread's tile index (jb) and global offset (tg) with the
@@ -204,11 +210,11 @@ __device__ __forceinline__ void ldsPut(ct<T>* tile, uint32_t i, ct<T> a) {
 template <typename T, int NT, int V>
 struct RegIO {
     __device__ __forceinline__ static void ldsLoad(ct<T>* r, const ct<T>* tile, const RegLayout& L) {
-        r[V] = ldsGet(tile, swz(L.jb | offJ<V>(L)));
+        r[V] = ldsGet(tile, swz<T>(L.jb | offJ<V>(L)));
         RegIO<T, NT, V + 1>::ldsLoad(r, tile, L);
     }
     __device__ __forceinline__ static void ldsStore(const ct<T>* r, ct<T>* tile, const RegLayout& L) {
-        ldsPut(tile, swz(L.jb | offJ<V>(L)), r[V]);
+        ldsPut(tile, swz<T>(L.jb | offJ<V>(L)), r[V]);
         RegIO<T, NT, V + 1>::ldsStore(r, tile, L);
     }
     __device__ __forceinline__ static void hbmLoad(ct<T>* r, const T* amps, uint64_t at, const RegLayout& L) {
@@ -521,13 +527,13 @@ __device__ __forceinline__ void ldsPauli(ct<T>* tile, int k, const OpRegs& o, ui
         const uint32_t jA = ((q >> top) << (top + 1)) | (q & ((1u << top) - 1u));
         if ((jA & o.h.jcMask) != o.h.jcVal) continue;
         const uint32_t jB = jA ^ (uint32_t)xy;
-        const ct<T> va = ldsGet(tile, swz(jA)), vb = ldsGet(tile, swz(jB));
+        const ct<T> va = ldsGet(tile, swz<T>(jA)), vb = ldsGet(tile, swz<T>(jB));
         const T sA = (parity64(jA & o.jm) ^ basePar) ? (T)-1 : (T)1;
         const T sB = (parity64(jB & o.jm) ^ basePar) ? (T)-1 : (T)1;
         const ct<T> pB = {pairFac.re * sB, pairFac.im * sB};
         const ct<T> pA = {pairFac.re * sA, pairFac.im * sA};
-        ldsPut(tile, swz(jA), cfmaT(pB, vb, cmulT(ampFac, va)));
-        ldsPut(tile, swz(jB), cfmaT(pA, va, cmulT(ampFac, vb)));
+        ldsPut(tile, swz<T>(jA), cfmaT(pB, vb, cmulT(ampFac, va)));
+        ldsPut(tile, swz<T>(jB), cfmaT(pA, va, cmulT(ampFac, vb)));
     }
 }
 
@@ -571,7 +577,7 @@ __device__ __forceinline__ void runBlock32(ct<float>* tile, int k, const int* sl
 #pragma unroll
         for (int b = 1; b < BQ; b++)
             if ((t >> (b - 1)) & 1) j |= sb[b];
-        const ct<float> v = ldsGet(tile, swz(j));
+        const ct<float> v = ldsGet(tile, swz<float>(j));
         xr[t] = v.re;
         xi[t] = v.im;
     }
@@ -591,7 +597,7 @@ __device__ __forceinline__ void runBlock32(ct<float>* tile, int k, const int* sl
         for (int b = 0; b < BQ; b++)
             if ((rowConst >> b) & 1) j |= sb[b];
         if (h) j |= sb[2];  // the 4h term is bit 2 of the row
-        ldsPut(tile, swz(j), ct<float>{dTop[i], dBot[i]});
+        ldsPut(tile, swz<float>(j), ct<float>{dTop[i], dBot[i]});
     }
 }
 

@@ -425,6 +425,9 @@ def main() -> int:
                     help="tileQubits,lowQubits,maxQueue of the gate-fusion queue, or 'off'")
     ap.add_argument("--no-unfused", action="store_true", help="skip the one-step unfused comparison")
     ap.add_argument("--prefetch", type=int, default=1, help="tile passes prefetch the next tile (1) or not (0)")
+    ap.add_argument("--exact", type=int, default=0,
+                    help="exact fusion (1): gates in queue order with single-gate arithmetic; 0: passes reorder "
+                         "commuting gates and merge runs on the same one or two qubits into one matrix")
     ap.add_argument("--block-min-gates", type=int, default=4,
                     help="precision mode: runs of at least this many gates on 5 qubits become FP32 MFMA blocks")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -478,6 +481,7 @@ def main() -> int:
     else:
         Q.set_fusion(False)
     Q.gpu().qa_setFusionPrefetch(int(args.prefetch))
+    Q.set_fusion_exact(bool(args.exact))
     fusion_cfg = Q.get_fusion()
     env = Q.getQuESTEnv()
     log(f"bench: {n} qubits, {gates} gates + {reductions} reductions per step, GPU={env.isGpuAccelerated}, "
@@ -586,7 +590,7 @@ def main() -> int:
             "state_bytes": state,
             "parallelism": "single GPU",
             "fusion": ({"tile_qubits": fusion_cfg[1], "low_qubits": fusion_cfg[2], "max_queue": fusion_cfg[3],
-                        "passes_per_step": passes.value / args.steps,
+                        "exact": bool(args.exact), "passes_per_step": passes.value / args.steps,
                         "gates_per_pass": (fgates.value / passes.value) if passes.value else None}
                        if fused else "off"),
         },

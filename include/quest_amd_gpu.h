@@ -106,12 +106,18 @@ int qa_eventDestroy(void* event);
 /* When enabled, the one/two-qubit dense, one-qubit diagonal, swap, Pauli
    tensor/gadget, phase gadget, one-qubit dephasing and one-qubit
    depolarising entry points queue their gate instead of launching it.
-   Queued gates are applied, in order and with arithmetic identical to the
-   unfused kernels, by tiled passes over the state: each workgroup holds a
+   Queued gates are applied by tiled passes over the state: each workgroup holds a
    tile of 2^tileQubits amplitudes (whose low `lowQubits` qubits are always
    in the tile) and applies runs of gates whose dense targets lie in 4 of
-   the tile's qubits in registers.  A gate is queued when its dense targets
-   (Pauli X/Y bits included) number at most 4; every other entry point
+   the tile's qubits in registers.  A pass takes the queued gates in order
+   while their dense targets fit one tile.  Unless fusion is exact
+   (qa_setFusionExact), it also takes later gates that commute with the
+   ones passed over, and multiplies (FP64, on the host) each run of its
+   gates confined to the same one or two qubits into one matrix, so results
+   differ from the unfused kernels by rounding only; exact fusion gives
+   bit-identical results.  A gate is queued when its
+   dense targets number at most 4, or it is a Pauli tensor/gadget whose X/Y
+   qubits, with the low qubits, fit one tile; every other entry point
    (reductions, copies, getters, setters, sync, event records) first applies
    what is queued.  Queues are per state, states of fewer than 4 qubits are
    never queued, and a queue is flushed after `maxQueue` gates.  Changing
@@ -131,6 +137,13 @@ int qa_setFusionBlocks(int minGates);
 /* tile passes with (1, the default) or without (0) each workgroup's next
    tile prefetched into registers while it works on the current one */
 int qa_setFusionPrefetch(int enabled);
+/* exact = 1: passes take queued gates in queue order and apply each with
+   the arithmetic of its single-gate kernel.  exact = 0 (the default):
+   passes may take gates out of order -- two gates are reordered only when
+   neither involves a dense target of the other (on shared qubits both are
+   then diagonal or controls), which makes them commute exactly -- and runs
+   of gates on the same one or two qubits become one matrix. */
+int qa_setFusionExact(int exact);
 /* counters of tile passes launched, the gates they applied and those
    gates' algorithmic real flops (a complex multiply counts 6, a complex
    multiply-add 8) since the last qa_getFusionStats (which resets them); with

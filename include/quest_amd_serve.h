@@ -14,7 +14,10 @@
  *     admitted as soon as a slot and memory free up (continuous batching);
  *   - every engine step launches one batched tile pass that advances every
  *     running request by one run of gates (all gates of a state of at most
- *     tileQubits qubits; otherwise the gates whose targets fit one tile),
+ *     tileQubits qubits; otherwise the gates whose targets fit one tile,
+ *     later gates joining when they commute with those passed over, and
+ *     runs of gates on the same one or two qubits multiplied into one
+ *     matrix),
  *     then one batched reduction for the requests whose gates are done;
  *   - expectations accumulate in FP64 with compensated summation, in an
  *     order fixed by the request alone, so a request's results do not depend

@@ -435,6 +435,27 @@ def set_fusion(enabled: bool, tile_qubits: int = 12, low_qubits: int = 4, max_qu
     setQuESTFusion(int(bool(enabled)), int(tile_qubits), int(low_qubits), int(max_queue))
 
 
+def fusion_stats() -> dict:
+    """Tile passes launched, the gates they applied, their milliseconds (with
+    pass timing on) and algorithmic flops since the last call (which resets
+    the counters)."""
+    passes, gates, ms, flops = C.c_int64(), C.c_int64(), C.c_double(), C.c_double()
+    rc = gpu().qa_getFusionStats(C.byref(passes), C.byref(gates), C.byref(ms), C.byref(flops))
+    if rc != 0:
+        raise QuESTError(f"qa_getFusionStats failed ({rc}): {_gpu_last_error()}")
+    return {"passes": passes.value, "gates": gates.value, "pass_ms": ms.value, "flops": flops.value}
+
+
+def set_fusion_exact(exact: bool) -> None:
+    """Exact fusion applies queued gates in order with their single-gate
+    arithmetic (bit-identical to unfused runs); otherwise (the default)
+    passes reorder commuting gates and multiply runs of gates on the same
+    one or two qubits into one matrix."""
+    rc = gpu().qa_setFusionExact(C.c_int(int(bool(exact))))
+    if rc != 0:
+        raise QuESTError(f"qa_setFusionExact failed ({rc}): {_gpu_last_error()}")
+
+
 def ensure_env() -> None:
     if not isQuESTEnvInit():
         initQuESTEnv()

@@ -64,6 +64,52 @@ __device__ __forceinline__ ct<T> cfmaT(ct<T> m, ct<T> a, ct<T> acc) {
     return ct<T>{fmaT(-m.im, a.im, re), fmaT(m.im, a.re, im)};
 }
 
+// Matrix-vector products over a pair or quad of amplitudes, y = M x, with
+// y[k] finished by the two multiply-adds m_kk.re * x[k] that read x[k] last:
+// every partial sum is formed before any output, so each output can take
+// its input's registers (a kernel that keeps amplitudes in registers across
+// many gates then needs no copies).  Row k sums, in order: the cross term of
+// Im m_kk, then m_kj x[j] for j != k ascending, then Re m_kk x[k].
+template <typename T>
+__device__ __forceinline__ ct<T> crowStartT(ct<T> mkk, ct<T> xk) {
+    return ct<T>{-(mkk.im * xk.im), mkk.im * xk.re};
+}
+template <typename T>
+__device__ __forceinline__ void crowAddT(ct<T>& acc, ct<T> m, ct<T> x) {
+    acc.re = fmaT(m.re, x.re, acc.re);
+    acc.re = fmaT(-m.im, x.im, acc.re);
+    acc.im = fmaT(m.re, x.im, acc.im);
+    acc.im = fmaT(m.im, x.re, acc.im);
+}
+template <typename T>
+__device__ __forceinline__ ct<T> crowEndT(ct<T> mkk, ct<T> xk, ct<T> acc) {
+    return ct<T>{fmaT(mkk.re, xk.re, acc.re), fmaT(mkk.re, xk.im, acc.im)};
+}
+// [x0; x1] <- [[m00, m01], [m10, m11]] [x0; x1]
+template <typename T>
+__device__ __forceinline__ void cmat2T(ct<T> m00, ct<T> m01, ct<T> m10, ct<T> m11, ct<T>& x0, ct<T>& x1) {
+    ct<T> p = crowStartT(m00, x0), q = crowStartT(m11, x1);
+    crowAddT(p, m01, x1);
+    crowAddT(q, m10, x0);
+    x0 = crowEndT(m00, x0, p);
+    x1 = crowEndT(m11, x1, q);
+}
+// x <- M x for 4 amplitudes, M(i) = element i of the row-major 4x4
+template <typename T, typename E>
+__device__ __forceinline__ void cmat4T(const E& M, ct<T>& x0, ct<T>& x1, ct<T>& x2, ct<T>& x3) {
+    ct<T>* x[4] = {&x0, &x1, &x2, &x3};
+    ct<T> acc[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        acc[k] = crowStartT(M(5 * k), *x[k]);
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if (j != k) crowAddT(acc[k], M(4 * k + j), *x[j]);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) *x[k] = crowEndT(M(5 * k), *x[k], acc[k]);
+}
+
 template <typename T>
 __device__ __forceinline__ ct<T> ld(const T* __restrict__ amps, uint64_t i) {
     typedef typename Amp<T>::V V;

@@ -17,8 +17,11 @@
 // tile index.
 //
 // Each gate's arithmetic on an amplitude is the same expression as in the
-// single-gate kernels (qa_gates.hip, qa_channels.hip), applied in the same
-// order, so fused and unfused runs give identical amplitudes.
+// single-gate kernels (qa_gates.hip, qa_channels.hip).  In exact mode the
+// gates are applied in queue order, so fused and unfused runs give identical
+// amplitudes; by default a pass also takes later gates that commute with
+// those it passes over, and multiplies runs of gates on the same one or two
+// qubits into one matrix on the host, which changes rounding only.
 #include <algorithm>
 #include <cmath>
 #include <complex>
@@ -283,6 +286,29 @@ size_t fuse::takeRun(const GOp* ops, size_t num, int numQubits, int kMax, int lo
     return i;
 }
 
+void fuse::pickPass(const GOp* ops, size_t num, int numQubits, int kMax, int low, uint64_t& localMask,
+                    std::vector<uint32_t>& picked) {
+    uint64_t L = (low >= 64) ? ~0ull : ((1ull << low) - 1ull);
+    const uint64_t all = (numQubits >= 64) ? ~0ull : ((1ull << numQubits) - 1ull);
+    uint64_t skipDense = 0, skipTouch = 0;  // qubits of the ops passed over
+    for (size_t i = 0; i < num; i++) {
+        const uint64_t d = denseMask(ops[i]);
+        const uint64_t u = L | d;
+        if ((d & skipTouch) == 0 && (supportMask(ops[i]) & skipDense) == 0 && __builtin_popcountll(u) <= kMax) {
+            L = u;
+            picked.push_back((uint32_t)i);
+            continue;
+        }
+        skipDense |= d;
+        skipTouch |= supportMask(ops[i]);
+        // nothing involving a qubit can join once every qubit is a dense
+        // target of a skipped op
+        if (skipDense == all) break;
+    }
+    for (int q = 0; q < numQubits && __builtin_popcountll(L) < kMax; q++) L |= 1ull << q;
+    localMask = L;
+}
+
 namespace fuse {
 
 // the tile slots of localMask, and the pass's fixed fields of `a`
@@ -413,7 +439,6 @@ void emitGroups(const GOp* ops, size_t from, size_t to, Layout& L, int k, int lo
     }
 }
 
-// every qubit an op reads or writes: targets, controls, diagonal masks
 uint64_t supportMask(const GOp& g) {
     uint64_t m = g.cMask | denseMask(g);
     switch (g.kind) {
@@ -425,24 +450,24 @@ uint64_t supportMask(const GOp& g) {
     return m;
 }
 
-// g applied to a vector of 2^BQ amplitudes whose bit b is global qubit q[b]
+// g applied to a vector of 2^nq amplitudes whose bit b is global qubit q[b]
 // (every qubit of g is one of them), in FP64, with the same semantics as
 // the single-gate kernels
-void applySmall(const GOp& g, const int* q, std::complex<double>* v) {
+void applySmall(const GOp& g, const int* q, int nq, std::complex<double>* v) {
     typedef std::complex<double> cx;
     auto loc = [&](uint64_t m) {
         uint32_t r = 0;
-        for (int b = 0; b < BQ; b++)
+        for (int b = 0; b < nq; b++)
             if ((m >> q[b]) & 1ull) r |= 1u << b;
         return r;
     };
     auto bitOf = [&](int qubit) {
-        for (int b = 0; b < BQ; b++)
+        for (int b = 0; b < nq; b++)
             if (q[b] == qubit) return b;
         return -1;
     };
     const uint32_t cm = loc(g.cMask), cv = loc(g.cVal);
-    const int D = 1 << BQ;
+    const int D = 1 << nq;
     auto fc = [&](int i) { return cx(g.f[2 * i], g.f[2 * i + 1]); };
     switch (g.kind) {
         case F_DENSE1: {
@@ -540,7 +565,7 @@ TOp makeBlock(const GOp* ops, size_t from, size_t to, uint64_t S, const Layout& 
     for (int c = 0; c < D; c++) {
         cx v[D];
         for (int j = 0; j < D; j++) v[j] = (j == c) ? 1.0 : 0.0;
-        for (size_t i = from; i < to; i++) applySmall(ops[i], q, v);
+        for (size_t i = from; i < to; i++) applySmall(ops[i], q, BQ, v);
         for (int r = 0; r < D; r++) U[r * D + c] = v[r];
     }
     // A[r][kk] of the real form [[Re U, -Im U], [Im U, Re U]], packed as
@@ -567,7 +592,91 @@ TOp makeBlock(const GOp* ops, size_t from, size_t to, uint64_t S, const Layout& 
     return t;
 }
 
+// ops whose action is confined to one or two qubits (controls included),
+// and those among them that are diagonal
+bool mergeable(const GOp& g) {
+    switch (g.kind) {
+        case F_DENSE1: case F_DENSE2: case F_DIAG1: case F_PARITY: case F_PAULI: case F_SWAP: case F_DEPOL1:
+        case F_DEPH1: break;
+        default: return false;
+    }
+    const int n = __builtin_popcountll(supportMask(g));
+    return n >= 1 && n <= 2;
+}
+bool diagonalOp(const GOp& g) { return g.kind == F_DIAG1 || g.kind == F_PARITY || g.kind == F_DEPH1; }
+
 }  // namespace fuse
+
+void fuse::mergePass(const GOp* ops, size_t num, uint64_t localMask, std::vector<GOp>& out) {
+    struct Block {
+        uint64_t qubits;
+        std::vector<uint32_t> members;  // positions in ops, ascending
+        bool dense;                     // some member is not diagonal
+    };
+    std::vector<Block> open;  // pairwise disjoint qubit sets
+    auto emit = [&](const Block& b) {
+        if (b.members.size() == 1 || !b.dense) {
+            for (uint32_t m : b.members) out.push_back(ops[m]);
+            return;
+        }
+        int q[2], nq = 0;
+        for (int x = 0; x < 64 && nq < 2; x++)
+            if ((b.qubits >> x) & 1ull) q[nq++] = x;
+        const int D = 1 << nq;
+        std::complex<double> M[16];
+        for (int c = 0; c < D; c++) {
+            std::complex<double> v[4] = {};
+            v[c] = 1.0;
+            for (uint32_t m : b.members) applySmall(ops[m], q, nq, v);
+            for (int r = 0; r < D; r++) M[r * D + c] = v[r];
+        }
+        GOp g;
+        g.kind = nq == 1 ? F_DENSE1 : F_DENSE2;
+        g.t1 = q[0];
+        g.t2 = nq == 2 ? q[1] : -1;
+        for (int i = 0; i < D * D; i++) {
+            g.f[2 * i] = M[i].real();
+            g.f[2 * i + 1] = M[i].imag();
+        }
+        out.push_back(g);
+    };
+    for (size_t i = 0; i < num; i++) {
+        const GOp& g = ops[i];
+        const uint64_t T = supportMask(g);
+        uint64_t U = T;
+        bool dense = !diagonalOp(g);
+        for (const Block& b : open)
+            if (b.qubits & T) {
+                U |= b.qubits;
+                dense |= b.dense;
+            }
+        // a block that ends up dense becomes one matrix on its qubits, which
+        // must then all be local
+        const bool fits = mergeable(g) && __builtin_popcountll(U) <= 2 && (!dense || (U & ~localMask) == 0);
+        Block nb{fits ? U : T, {}, !diagonalOp(g)};
+        for (size_t b = 0; b < open.size();) {
+            if (!(open[b].qubits & T)) {
+                b++;
+                continue;
+            }
+            if (fits) {
+                nb.members.insert(nb.members.end(), open[b].members.begin(), open[b].members.end());
+                nb.dense |= open[b].dense;
+            } else {
+                emit(open[b]);
+            }
+            open.erase(open.begin() + b);
+        }
+        if (!mergeable(g) || (!fits && !diagonalOp(g) && (T & ~localMask) != 0)) {
+            out.push_back(g);
+            continue;
+        }
+        std::sort(nb.members.begin(), nb.members.end());
+        nb.members.push_back((uint32_t)i);
+        open.push_back(std::move(nb));
+    }
+    for (const Block& b : open) emit(b);
+}
 
 void fuse::planPass(const GOp* ops, size_t num, int numQubits, uint64_t localMask, int lowQubits, TileArgs& a,
                     std::vector<TOp>& prog) {
@@ -665,8 +774,10 @@ double opFlops(const GOp& g, int n) {
     return perAmp * std::ldexp(1.0, n - __builtin_popcountll(g.cMask));
 }
 
-// launches one pass applying ops [begin, end) of the queue
-int launchPass(void* amps, const Queue& qu, const std::vector<GOp>& ops, size_t begin, size_t end, uint64_t localMask) {
+// launches one pass applying ops [begin, end) of `ops`, which stand for
+// numGates queued gates
+int launchPass(void* amps, const Queue& qu, const std::vector<GOp>& ops, size_t begin, size_t end, uint64_t localMask,
+               size_t numGates) {
     TileArgs a;
     std::vector<TOp>& prog = g_prog;
     prog.clear();
@@ -734,13 +845,16 @@ int launchPass(void* amps, const Queue& qu, const std::vector<GOp>& ops, size_t 
     QA_LAUNCH_CHECK();
     if (ev) QA_HIP(hipEventRecord(ev->second, stream()));
     g_stats.passes++;
-    g_stats.gates += (int64_t)(end - begin);
+    g_stats.gates += (int64_t)numGates;
     for (size_t i = begin; i < end; i++) g_stats.flops += opFlops(ops[i], qu.numQubits);
     return QA_OK;
 }
 
-// greedy planner: extend the local set while the next gate's dense targets
-// still fit the tile, then launch the run as one pass
+// greedy planner: in queue order, extend the local set while the next gate's
+// dense targets still fit the tile, then launch the run as one pass; unless
+// exact, gates that do not fit are passed over, later gates that commute
+// with them join the pass, and the pass's runs of gates on the same one or
+// two qubits are multiplied into one matrix
 int flushQueue(void* amps, Queue& qu) {
     if (qu.ops.empty()) return QA_OK;
     if (!ensureRing()) {
@@ -751,18 +865,49 @@ int flushQueue(void* amps, Queue& qu) {
     const int kMax = std::min(n, g_cfg.tileQubits);
     const int low = std::min(g_cfg.lowQubits, kMax);
     std::vector<GOp>& ops = qu.ops;
-    size_t i = 0;
     int rc = QA_OK;
-    while (i < ops.size() && rc == QA_OK) {
+    if (g_cfg.exact) {
+        size_t i = 0;
+        while (i < ops.size() && rc == QA_OK) {
+            uint64_t L = 0;
+            size_t run = takeRun(ops.data() + i, ops.size() - i, n, kMax, low, L);
+            if (run == 0) {
+                setError("fusion: a gate needs more local qubits than the tile holds");
+                rc = QA_ERR_ARG;
+                break;
+            }
+            rc = launchPass(amps, qu, ops, i, i + run, L, run);
+            i += run;
+        }
+        ops.clear();
+        return rc;
+    }
+    std::vector<uint32_t> picked;
+    std::vector<GOp> pass, rest, merged;
+    while (!ops.empty() && rc == QA_OK) {
         uint64_t L = 0;
-        size_t run = takeRun(ops.data() + i, ops.size() - i, n, kMax, low, L);
-        if (run == 0) {
+        picked.clear();
+        pickPass(ops.data(), ops.size(), n, kMax, low, L, picked);
+        if (picked.empty()) {
             setError("fusion: a gate needs more local qubits than the tile holds");
             rc = QA_ERR_ARG;
             break;
         }
-        rc = launchPass(amps, qu, ops, i, i + run, L);
-        i += run;
+        pass.clear();
+        rest.clear();
+        size_t p = 0;
+        for (size_t i = 0; i < ops.size(); i++) {
+            if (p < picked.size() && picked[p] == i) {
+                pass.push_back(ops[i]);
+                p++;
+            } else {
+                rest.push_back(ops[i]);
+            }
+        }
+        merged.clear();
+        mergePass(pass.data(), pass.size(), L, merged);
+        rc = launchPass(amps, qu, merged, 0, merged.size(), L, pass.size());
+        ops.swap(rest);
     }
     ops.clear();
     return rc;
@@ -935,6 +1080,14 @@ int qa_setFusionPrefetch(int enabled) {
     QA_FLUSH_ALL();
     std::lock_guard<std::mutex> lk(qa::fuse::g_mu);
     qa::fuse::g_cfg.prefetch = enabled != 0;
+    return QA_OK;
+}
+
+int qa_setFusionExact(int exact) {
+    QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
+    std::lock_guard<std::mutex> lk(qa::fuse::g_mu);
+    qa::fuse::g_cfg.exact = exact != 0;
     return QA_OK;
 }
 

@@ -109,12 +109,9 @@ __global__ void __launch_bounds__(BLOCK) k_oneTargDense(T* __restrict__ amps, ui
     for (int j = 0; j < IT; j++) {
         uint64_t n = base + (uint64_t)j * BLOCK;
         if (n < numWork) {
-            ct<T> b0 = cmulT(m.m00, a0[j]);
-            b0 = cfmaT(m.m01, a1[j], b0);
-            ct<T> b1 = cmulT(m.m10, a0[j]);
-            b1 = cfmaT(m.m11, a1[j], b1);
-            stx<NT>(amps, i0[j], b0);
-            stx<NT>(amps, i0[j] | tbit, b1);
+            cmat2T(m.m00, m.m01, m.m10, m.m11, a0[j], a1[j]);
+            stx<NT>(amps, i0[j], a0[j]);
+            stx<NT>(amps, i0[j] | tbit, a1[j]);
         }
     }
 }
@@ -134,13 +131,9 @@ __global__ void __launch_bounds__(BLOCK) k_twoTargDense(T* __restrict__ amps, ui
         ct<T> a[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) a[k] = ldnt(amps, i0 | off[k]);
+        cmat4T<T>([&](int e) { return m.e[e]; }, a[0], a[1], a[2], a[3]);
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            ct<T> acc = cmulT(m.e[4 * r], a[0]);
-#pragma unroll
-            for (int c = 1; c < 4; c++) acc = cfmaT(m.e[4 * r + c], a[c], acc);
-            stnt(amps, i0 | off[r], acc);
-        }
+        for (int r = 0; r < 4; r++) stnt(amps, i0 | off[r], a[r]);
     }
 }
 
@@ -226,7 +219,7 @@ __global__ void __launch_bounds__(BLOCK) k_oneTargDiag(T* __restrict__ amps, uin
         uint64_t n = base + (uint64_t)j * BLOCK;
         if (n < numWork) {
             ct<T> d = (targ < 0 || ((idx[j] >> targ) & 1ull)) ? d1 : d0;
-            stx<NT>(amps, idx[j], cmulT(a[j], d));
+            stx<NT>(amps, idx[j], cmulT(d, a[j]));
         }
     }
 }
@@ -243,7 +236,7 @@ __global__ void __launch_bounds__(BLOCK) k_twoTargDiag(T* __restrict__ amps, uin
         uint64_t i = insertBits(n, ins);
         int k = (int)(((i >> targ2) & 1ull) << 1 | ((i >> targ1) & 1ull));
         ct<T> d = (k == 0) ? d0 : (k == 1) ? d1 : (k == 2) ? d2 : d3;
-        stnt(amps, i, cmulT(ldnt(amps, i), d));
+        stnt(amps, i, cmulT(d, ldnt(amps, i)));
     }
 }
 
@@ -269,7 +262,7 @@ __global__ void __launch_bounds__(BLOCK) k_anyTargDiag(T* __restrict__ amps, uin
     }
     if (conj) ei = -ei;
     ct<T> d = {(T)er, (T)ei};
-    st(amps, i, cmulT(ld(amps, i), d));
+    st(amps, i, cmulT(d, ld(amps, i)));
 }
 
 template <typename T>
@@ -300,7 +293,7 @@ __global__ void __launch_bounds__(BLOCK) k_allTargDiag(T* __restrict__ amps, uin
         if (!multiplyOnly) f = cmul(f, cconj(elem(i >> densN)));
     }
     ct<T> a = ld(amps, i);
-    st(amps, i, cmulT(a, toT<T>(f)));
+    st(amps, i, cmulT(toT<T>(f), a));
 }
 
 // swap amplitudes where (targ1, targ2) = (1, 0) and (0, 1)
@@ -357,10 +350,9 @@ __global__ void __launch_bounds__(BLOCK) k_pauli(T* __restrict__ amps, uint64_t 
             T sB = parity64(ib & maskYZ) ? (T)-1 : (T)1;
             ct<T> pB = {pairFac.re * sB, pairFac.im * sB};
             ct<T> pA = {pairFac.re * sA, pairFac.im * sA};
-            ct<T> na = cfmaT(pB, b[j], cmulT(ampFac, a[j]));
-            ct<T> nb = cfmaT(pA, a[j], cmulT(ampFac, b[j]));
-            stx<NT>(amps, ia[j], na);
-            stx<NT>(amps, ib, nb);
+            cmat2T(ampFac, pB, pA, ampFac, a[j], b[j]);
+            stx<NT>(amps, ia[j], a[j]);
+            stx<NT>(amps, ib, b[j]);
         }
     }
 }
@@ -384,7 +376,7 @@ __global__ void __launch_bounds__(BLOCK) k_phaseGadget(T* __restrict__ amps, uin
 #pragma unroll
     for (int j = 0; j < IT; j++) {
         uint64_t n = base + (uint64_t)j * BLOCK;
-        if (n < numWork) stx<NT>(amps, idx[j], cmulT(a[j], parity64(idx[j] & targMask) ? f1 : f0));
+        if (n < numWork) stx<NT>(amps, idx[j], cmulT(parity64(idx[j] & targMask) ? f1 : f0, a[j]));
     }
 }
 

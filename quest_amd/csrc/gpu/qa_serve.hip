@@ -261,8 +261,7 @@ enum Phase { QUEUED, RUNNING, REDUCING, DONE, FAILED };
 struct Request {
     int64_t id = 0;
     int n = 0;
-    std::vector<GOp> ops;
-    size_t next = 0;
+    std::vector<GOp> ops;  // not yet applied
     bool firstPass = true;
     std::vector<std::array<uint64_t, 3>> paulis;
     uint64_t off = 0;
@@ -366,30 +365,46 @@ bool launchStep(Engine& E, StepBuf& sb, std::string& err) {
     std::map<int, std::vector<PassDesc>> byK;
     std::vector<TOp> prog;
     std::vector<ExpJob> jobs;
+    std::vector<uint32_t> picked;
+    std::vector<GOp> pass, rest, merged;
     uint64_t chunks = 0;
     int outs = 0;
     for (Request* r : E.running) {
         if (r->phase != RUNNING) continue;
-        if (r->next < r->ops.size()) {
+        if (!r->ops.empty()) {
             const int kMax = std::min(r->n, E.tileQ);
             const int low = std::min(E.lowQ, kMax);
             uint64_t L = 0;
-            const size_t run = takeRun(r->ops.data() + r->next, r->ops.size() - r->next, r->n, kMax, low, L);
-            if (run == 0) {
+            picked.clear();
+            pickPass(r->ops.data(), r->ops.size(), r->n, kMax, low, L, picked);
+            if (picked.empty()) {
                 err = "serve: a gate needs more local qubits than the tile holds";
                 return false;
+            }
+            pass.clear();
+            rest.clear();
+            size_t p = 0;
+            for (size_t i = 0; i < r->ops.size(); i++) {
+                if (p < picked.size() && picked[p] == i) {
+                    pass.push_back(r->ops[i]);
+                    p++;
+                } else {
+                    rest.push_back(r->ops[i]);
+                }
             }
             PassDesc pd;
             std::memset(&pd, 0, sizeof(pd));
             pd.amps = r->amps;
             pd.initZero = r->firstPass ? 1 : 0;
             pd.opsOffset = (uint32_t)prog.size();
-            planPass(r->ops.data() + r->next, run, r->n, L, E.lowQ, pd.a, prog);
+            merged.clear();
+            mergePass(pass.data(), pass.size(), L, merged);
+            planPass(merged.data(), merged.size(), r->n, L, E.lowQ, pd.a, prog);
             r->firstPass = false;
-            r->next += run;
+            r->ops.swap(rest);
             byK[pd.a.k].push_back(pd);
         }
-        if (r->next == r->ops.size()) {
+        if (r->ops.empty()) {
             r->phase = REDUCING;
             r->outBase = outs;
             for (auto& p : r->paulis) {

@@ -257,32 +257,20 @@ __device__ __forceinline__ void regDense1(ct<T>* r, const TOpHead& h, const doub
     for (int v = 0; v < RA; v++) {
         if (v & (1 << B)) continue;
         if (!regCond<C>(v, h)) continue;
-        const ct<T> a0 = r[v], a1 = r[v | (1 << B)];
-        ct<T> b0 = cmulT(m00, a0);
-        b0 = cfmaT(m01, a1, b0);
-        ct<T> b1 = cmulT(m10, a0);
-        b1 = cfmaT(m11, a1, b1);
-        r[v] = b0;
-        r[v | (1 << B)] = b1;
+        cmat2T(m00, m01, m10, m11, r[v], r[v | (1 << B)]);
     }
 }
 
 // element i (0..15) of a 4x4 matrix whose elements 0..3 are in f and 4..15
-// in fx (memory)
-template <typename T, int I>
-__device__ __forceinline__ ct<T> m4(const double* f, const double* __restrict__ fx) {
-    if (I < 4) return ct<T>{(T)f[2 * I], (T)f[2 * I + 1]};
-    return ct<T>{(T)fx[2 * (I - 4)], (T)fx[2 * (I - 4) + 1]};
-}
-
-template <typename T, int ROW>
-__device__ __forceinline__ ct<T> dense2Row(const ct<T>* x, const double* f, const double* __restrict__ fx) {
-    ct<T> acc = cmulT(m4<T, 4 * ROW>(f, fx), x[0]);
-    acc = cfmaT(m4<T, 4 * ROW + 1>(f, fx), x[1], acc);
-    acc = cfmaT(m4<T, 4 * ROW + 2>(f, fx), x[2], acc);
-    acc = cfmaT(m4<T, 4 * ROW + 3>(f, fx), x[3], acc);
-    return acc;
-}
+// in fx (memory); i is a constant once the callers' loops are unrolled
+template <typename T>
+struct M4 {
+    const double* f;
+    const double* __restrict__ fx;
+    __device__ __forceinline__ ct<T> operator()(int i) const {
+        return i < 4 ? ct<T>{(T)f[2 * i], (T)f[2 * i + 1]} : ct<T>{(T)fx[2 * (i - 4)], (T)fx[2 * (i - 4) + 1]};
+    }
+};
 
 template <typename T, int B1, int B2, bool C>
 __device__ __forceinline__ void regDense2(ct<T>* r, const TOpHead& h, const double* f, const double* __restrict__ fx) {
@@ -291,13 +279,7 @@ __device__ __forceinline__ void regDense2(ct<T>* r, const TOpHead& h, const doub
     for (int v = 0; v < RA; v++) {
         if (v & ((1 << B1) | (1 << B2))) continue;
         if (!regCond<C>(v, h)) continue;
-        ct<T> x[4];
-#pragma unroll
-        for (int c = 0; c < 4; c++) x[c] = r[v | off[c]];
-        r[v | off[0]] = dense2Row<T, 0>(x, f, fx);
-        r[v | off[1]] = dense2Row<T, 1>(x, f, fx);
-        r[v | off[2]] = dense2Row<T, 2>(x, f, fx);
-        r[v | off[3]] = dense2Row<T, 3>(x, f, fx);
+        cmat4T<T>(M4<T>{f, fx}, r[v | off[0]], r[v | off[1]], r[v | off[2]], r[v | off[3]]);
     }
 }
 
@@ -356,13 +338,11 @@ __device__ __forceinline__ void regPauli(ct<T>* r, const TOpHead& h, const doubl
         if (v & (1 << top)) continue;
         if (!regCond<C>(v, h)) continue;
         const int w = v ^ MXY;
-        const ct<T> va = r[v], vb = r[w];
         const T sA = ((__popc((uint32_t)v & h.rm2) & 1) ^ thrPar) ? (T)-1 : (T)1;
         const T sB = ((__popc((uint32_t)w & h.rm2) & 1) ^ thrPar) ? (T)-1 : (T)1;
         const ct<T> pB = {pairFac.re * sB, pairFac.im * sB};
         const ct<T> pA = {pairFac.re * sA, pairFac.im * sA};
-        r[v] = cfmaT(pB, vb, cmulT(ampFac, va));
-        r[w] = cfmaT(pA, va, cmulT(ampFac, vb));
+        cmat2T(ampFac, pB, pA, ampFac, r[v], r[w]);
     }
 }
 
@@ -376,7 +356,7 @@ __device__ __forceinline__ void regDiag1R(ct<T>* r, const TOpHead& h, const doub
     for (int v = 0; v < RA; v++) {
         if (PH && !(v & (1 << B))) continue;
         if (!regCond<C>(v, h)) continue;
-        r[v] = cmulT(r[v], (v & (1 << B)) ? d1 : d0);
+        r[v] = cmulT((v & (1 << B)) ? d1 : d0, r[v]);
     }
 }
 
@@ -388,7 +368,7 @@ __device__ __forceinline__ void regDiag1T(ct<T>* r, const TOpHead& h, const doub
 #pragma unroll
     for (int v = 0; v < RA; v++) {
         if (!regCond<C>(v, h)) continue;
-        r[v] = cmulT(r[v], d);
+        r[v] = cmulT(d, r[v]);
     }
 }
 
@@ -398,7 +378,7 @@ __device__ __forceinline__ void regParity(ct<T>* r, const TOpHead& h, const doub
 #pragma unroll
     for (int v = 0; v < RA; v++) {
         if (!regCond<C>(v, h)) continue;
-        r[v] = cmulT(r[v], ((__popc((uint32_t)v & h.rm1) & 1) ^ thrPar) ? f1 : f0);
+        r[v] = cmulT(((__popc((uint32_t)v & h.rm1) & 1) ^ thrPar) ? f1 : f0, r[v]);
     }
 }
 
@@ -527,13 +507,14 @@ __device__ __forceinline__ void ldsPauli(ct<T>* tile, int k, const OpRegs& o, ui
         const uint32_t jA = ((q >> top) << (top + 1)) | (q & ((1u << top) - 1u));
         if ((jA & o.h.jcMask) != o.h.jcVal) continue;
         const uint32_t jB = jA ^ (uint32_t)xy;
-        const ct<T> va = ldsGet(tile, swz<T>(jA)), vb = ldsGet(tile, swz<T>(jB));
+        ct<T> va = ldsGet(tile, swz<T>(jA)), vb = ldsGet(tile, swz<T>(jB));
         const T sA = (parity64(jA & o.jm) ^ basePar) ? (T)-1 : (T)1;
         const T sB = (parity64(jB & o.jm) ^ basePar) ? (T)-1 : (T)1;
         const ct<T> pB = {pairFac.re * sB, pairFac.im * sB};
         const ct<T> pA = {pairFac.re * sA, pairFac.im * sA};
-        ldsPut(tile, swz<T>(jA), cfmaT(pB, vb, cmulT(ampFac, va)));
-        ldsPut(tile, swz<T>(jB), cfmaT(pA, va, cmulT(ampFac, vb)));
+        cmat2T(ampFac, pB, pA, ampFac, va, vb);
+        ldsPut(tile, swz<T>(jA), va);
+        ldsPut(tile, swz<T>(jB), vb);
     }
 }
 
@@ -677,10 +658,34 @@ unsigned residentBlocks(const void* fn, unsigned block, size_t lds);
 // diagonal ops may involve any qubit)
 uint64_t denseMask(const GOp& g);
 
+// every qubit an op involves: dense and diagonal targets, controls, parity
+// and Pauli masks
+uint64_t supportMask(const GOp& g);
+
 // the longest prefix of ops[0..num) whose dense targets, with the `low`
 // lowest qubits, fit kMax local qubits; localMask is set to that local set
 // padded to exactly kMax qubits with the lowest free ones
 size_t takeRun(const GOp* ops, size_t num, int numQubits, int kMax, int low, uint64_t& localMask);
+
+// the ops of the next pass when ops may be reordered: scanning ops[0..num)
+// in order, an op joins the pass when its dense targets fit the local set
+// (grown up to kMax qubits, the `low` lowest always in it) and it commutes
+// with every op passed over before it.  Two ops are taken to commute when
+// neither involves a dense target of the other: on their shared qubits both
+// are then block-diagonal, so they commute exactly (their roundings differ
+// from the queue order).  Appends the positions taken, in queue order, to
+// `picked` and sets localMask as takeRun does.
+void pickPass(const GOp* ops, size_t num, int numQubits, int kMax, int low, uint64_t& localMask,
+              std::vector<uint32_t>& picked);
+
+// ops[0..num) of one pass (dense targets inside localMask) with every run
+// of gates confined to the same one or two qubits (controls included)
+// multiplied, in FP64 on the host, into one dense matrix applied as one op:
+// each gate joins the open run that shares a qubit with it, runs that a gate
+// bridges are joined, and a run is emitted when a gate involving its qubits
+// cannot join it (so everything emitted before it commutes with it).  Runs
+// of diagonal gates only, and runs of one gate, are emitted unchanged.
+void mergePass(const GOp* ops, size_t num, uint64_t localMask, std::vector<GOp>& out);
 
 // the program of one pass applying ops[0..num) (dense targets all inside
 // localMask) to a state of numQubits index bits: fills every field of `a`

@@ -511,6 +511,8 @@ def main() -> int:
     wall = time.perf_counter() - t0
     passes, fgates, pass_ms, pass_flops = C.c_int64(), C.c_int64(), C.c_double(), C.c_double()
     gpu.qa_getFusionStats(C.byref(passes), C.byref(fgates), C.byref(pass_ms), C.byref(pass_flops))
+    prog_ops, layouts = C.c_int64(), C.c_int64()
+    gpu.qa_getFusionProgramStats(C.byref(prog_ops), C.byref(layouts))
     gpu.qa_setFusionTiming(0)
     value = args.steps * gates / (total_ms / 1e3)
     state = 16 * (1 << (2 * n if dens else n))
@@ -523,7 +525,8 @@ def main() -> int:
         achieved = pass_bytes / avg_s
         log(f"tile passes: {passes.value} ({fgates.value} gates, {fgates.value / passes.value:.2f} per pass), "
             f"avg {avg_s * 1e3:.3f} ms = {achieved / 1e9:.0f} GB/s; passes are "
-            f"{100 * pass_ms.value / total_ms:.1f}% of the timed region")
+            f"{100 * pass_ms.value / total_ms:.1f}% of the timed region; per pass {prog_ops.value / passes.value:.1f} "
+            f"program ops, {layouts.value / passes.value:.2f} register-layout changes")
         traffic = None
         tfile = ROOT / "profiles" / f"k_tilePass_{n}q_fp64_traffic.json"
         if tfile.exists():
@@ -539,6 +542,8 @@ def main() -> int:
             "bytes_per_launch": pass_bytes,
             "avg_launch_ms": avg_s * 1e3,
             "launches": passes.value,
+            "program_ops_per_launch": prog_ops.value / passes.value,
+            "layout_changes_per_launch": layouts.value / passes.value,
             "share_of_step_time": pass_ms.value / total_ms,
             # the same launches against the FP64 vector peak: the gates a
             # pass applies are FP64 arithmetic on the registers

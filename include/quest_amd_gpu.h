@@ -144,14 +144,19 @@ int qa_setFusionPrefetch(int enabled);
    then diagonal or controls), which makes them commute exactly -- and runs
    of gates on the same one or two qubits become one matrix. */
 int qa_setFusionExact(int exact);
-/* counters of tile passes launched, the gates they applied and those
-   gates' algorithmic real flops (a complex multiply counts 6, a complex
-   multiply-add 8) since the last qa_getFusionStats (which resets them); with
+/* counters of tile passes launched, the gates they applied and the real
+   flops of the ops they executed (gates after merging; a complex multiply
+   counts 6, a complex multiply-add 8) since the last qa_getFusionStats
+   or qa_setFusionTiming (which reset them); with
    timing on, each pass launch is bracketed by HIP events on the library
    stream and passMs sums their elapsed times (qa_getFusionStats waits for
    them) */
 int qa_setFusionTiming(int enabled);
 int qa_getFusionStats(int64_t* passes, int64_t* gates, double* passMs, double* flops);
+/* program size of the tile passes since the last call (which resets them):
+   ops the passes executed (gates after merging, plus layout changes) and,
+   among them, register-layout changes through LDS */
+int qa_getFusionProgramStats(int64_t* ops, int64_t* layouts);
 
 /* ------------------------------------------------------------------ */
 /* statevector gates             (gpu_subroutines.hpp:33-70)          */

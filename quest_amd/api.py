@@ -437,13 +437,19 @@ def set_fusion(enabled: bool, tile_qubits: int = 12, low_qubits: int = 4, max_qu
 
 def fusion_stats() -> dict:
     """Tile passes launched, the gates they applied, their milliseconds (with
-    pass timing on) and algorithmic flops since the last call (which resets
-    the counters)."""
+    pass timing on), the flops and program ops they executed and their
+    register-layout changes, since the last call (which resets the
+    counters)."""
     passes, gates, ms, flops = C.c_int64(), C.c_int64(), C.c_double(), C.c_double()
     rc = gpu().qa_getFusionStats(C.byref(passes), C.byref(gates), C.byref(ms), C.byref(flops))
     if rc != 0:
         raise QuESTError(f"qa_getFusionStats failed ({rc}): {_gpu_last_error()}")
-    return {"passes": passes.value, "gates": gates.value, "pass_ms": ms.value, "flops": flops.value}
+    ops, layouts = C.c_int64(), C.c_int64()
+    rc = gpu().qa_getFusionProgramStats(C.byref(ops), C.byref(layouts))
+    if rc != 0:
+        raise QuESTError(f"qa_getFusionProgramStats failed ({rc}): {_gpu_last_error()}")
+    return {"passes": passes.value, "gates": gates.value, "pass_ms": ms.value, "flops": flops.value,
+            "ops": ops.value, "layouts": layouts.value}
 
 
 def set_fusion_exact(exact: bool) -> None:

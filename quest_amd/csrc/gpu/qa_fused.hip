@@ -135,6 +135,7 @@ Config g_cfg;  // disabled by default
 // around each tile-pass launch (for measuring the pass kernel alone)
 struct Stats {
     int64_t passes = 0, gates = 0;
+    int64_t progOps = 0, layouts = 0;  // program entries of the passes, and their GROUP ops
     double flops = 0;  // algorithmic real floating-point operations of the passes
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
@@ -678,6 +679,38 @@ void fuse::mergePass(const GOp* ops, size_t num, uint64_t localMask, std::vector
     for (const Block& b : open) emit(b);
 }
 
+void fuse::clusterPass(const GOp* ops, size_t num, std::vector<GOp>& out) {
+    std::vector<uint32_t> rem(num), next;
+    for (size_t i = 0; i < num; i++) rem[i] = (uint32_t)i;
+    while (!rem.empty()) {
+        uint64_t R = 0, skipDense = 0, skipTouch = 0;
+        bool closed = false;  // an LDS op ended this round's register group
+        next.clear();
+        for (uint32_t i : rem) {
+            const GOp& g = ops[i];
+            const uint64_t d = denseMask(g), t = supportMask(g);
+            bool take = (d & skipTouch) == 0 && (t & skipDense) == 0;
+            if (take && d) {
+                if (isLdsOp(g)) {
+                    take = !closed && R == 0;
+                    closed = take;
+                } else {
+                    take = !closed && __builtin_popcountll(R | d) <= RB;
+                }
+            }
+            if (take) {
+                R |= d;
+                out.push_back(g);
+            } else {
+                skipDense |= d;
+                skipTouch |= t;
+                next.push_back(i);
+            }
+        }
+        rem.swap(next);
+    }
+}
+
 void fuse::planPass(const GOp* ops, size_t num, int numQubits, uint64_t localMask, int lowQubits, TileArgs& a,
                     std::vector<TOp>& prog) {
     Layout L;
@@ -846,6 +879,8 @@ int launchPass(void* amps, const Queue& qu, const std::vector<GOp>& ops, size_t 
     if (ev) QA_HIP(hipEventRecord(ev->second, stream()));
     g_stats.passes++;
     g_stats.gates += (int64_t)numGates;
+    g_stats.progOps += (int64_t)num;
+    for (size_t i = 0; i < num; i++) g_stats.layouts += prog[i].h.kind == F_GROUP;
     for (size_t i = begin; i < end; i++) g_stats.flops += opFlops(ops[i], qu.numQubits);
     return QA_OK;
 }
@@ -904,9 +939,12 @@ int flushQueue(void* amps, Queue& qu) {
                 rest.push_back(ops[i]);
             }
         }
+        const size_t numGates = pass.size();
         merged.clear();
         mergePass(pass.data(), pass.size(), L, merged);
-        rc = launchPass(amps, qu, merged, 0, merged.size(), L, pass.size());
+        pass.clear();
+        clusterPass(merged.data(), merged.size(), pass);
+        rc = launchPass(amps, qu, pass, 0, pass.size(), L, numGates);
         ops.swap(rest);
     }
     ops.clear();
@@ -1049,6 +1087,7 @@ int qa_setFusionTiming(int enabled) {
     qa::fuse::g_stats.timing = enabled != 0;
     qa::fuse::g_stats.used = 0;
     qa::fuse::g_stats.passes = qa::fuse::g_stats.gates = 0;
+    qa::fuse::g_stats.progOps = qa::fuse::g_stats.layouts = 0;
     qa::fuse::g_stats.flops = 0;
     return QA_OK;
 }
@@ -1072,6 +1111,17 @@ int qa_getFusionStats(int64_t* passes, int64_t* gates, double* passMs, double* f
     st.used = 0;
     st.passes = st.gates = 0;
     st.flops = 0;
+    return QA_OK;
+}
+
+int qa_getFusionProgramStats(int64_t* ops, int64_t* layouts) {
+    QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
+    std::lock_guard<std::mutex> lk(qa::fuse::g_mu);
+    qa::fuse::Stats& st = qa::fuse::g_stats;
+    if (ops) *ops = st.progOps;
+    if (layouts) *layouts = st.layouts;
+    st.progOps = st.layouts = 0;
     return QA_OK;
 }
 

@@ -399,7 +399,9 @@ bool launchStep(Engine& E, StepBuf& sb, std::string& err) {
             pd.opsOffset = (uint32_t)prog.size();
             merged.clear();
             mergePass(pass.data(), pass.size(), L, merged);
-            planPass(merged.data(), merged.size(), r->n, L, E.lowQ, pd.a, prog);
+            pass.clear();
+            clusterPass(merged.data(), merged.size(), pass);
+            planPass(pass.data(), pass.size(), r->n, L, E.lowQ, pd.a, prog);
             r->firstPass = false;
             r->ops.swap(rest);
             byK[pd.a.k].push_back(pd);

@@ -687,6 +687,13 @@ void pickPass(const GOp* ops, size_t num, int numQubits, int kMax, int low, uint
 // of diagonal gates only, and runs of one gate, are emitted unchanged.
 void mergePass(const GOp* ops, size_t num, uint64_t localMask, std::vector<GOp>& out);
 
+// the ops of one pass reordered so that ops on the same register group are
+// applied together: rounds that each take, in order, every op that commutes
+// with the ops passed over in the round and whose dense targets fit, with
+// those taken so far, in RB qubits (a Pauli op applied in LDS ends a round's
+// dense ops).  Each round then needs one register layout.
+void clusterPass(const GOp* ops, size_t num, std::vector<GOp>& out);
+
 // the program of one pass applying ops[0..num) (dense targets all inside
 // localMask) to a state of numQubits index bits: fills every field of `a`
 // and appends the ops, with their GROUP ops, to `prog`

@@ -160,7 +160,7 @@ bool ensureRing() {
 
 uint64_t denseMask(const GOp& g) {
     switch (g.kind) {
-        case F_DENSE1: return 1ull << g.t1;
+        case F_DENSE1: case F_DENSE1V: return 1ull << g.t1;
         case F_DENSE2: case F_SWAP: case F_DEPOL1: case F_DEPH1: return (1ull << g.t1) | (1ull << g.t2);
         case F_PAULI: return g.m1;
         default: return 0;  // diagonal: any qubit
@@ -250,6 +250,18 @@ TOp toTile(const GOp& g, const Layout& L) {
         t.b[5] = gm2;
     }
     if (g.kind == F_DENSE2) std::memcpy(t.fx, g.f + 8, sizeof(t.fx));
+    if (g.kind == F_DENSE1V) {
+        // variant qubits (non-local): the lowest in rb2, the other (+1) in pad
+        int nv = 0;
+        h.pad = 0;
+        for (int x = 0; x < 64; x++)
+            if ((g.m1 >> x) & 1ull) {
+                if (nv == 0) h.rb2 = x;
+                else h.pad = (uint32_t)(x + 1);
+                nv++;
+            }
+        std::memcpy(t.fx, g.f + 8, sizeof(t.fx));
+    }
     if (g.kind == F_DIAG1 && h.rb1 >= 0 && ((h.rcMask >> h.rb1) & 1u)) {
         // phase-only diagonal (its target is one of its controls, set to 1):
         // the register kernel skips the amplitudes with the bit clear itself
@@ -443,6 +455,7 @@ void emitGroups(const GOp* ops, size_t from, size_t to, Layout& L, int k, int lo
 uint64_t supportMask(const GOp& g) {
     uint64_t m = g.cMask | denseMask(g);
     switch (g.kind) {
+        case F_DENSE1V: m |= g.m1; break;
         case F_DIAG1: m |= 1ull << g.t1; break;
         case F_PARITY: m |= g.m1; break;
         case F_PAULI: m |= g.m1 | g.m2; break;
@@ -478,6 +491,22 @@ void applySmall(const GOp& g, const int* q, int nq, std::complex<double>* v) {
                 const cx a0 = v[j], a1 = v[j | t];
                 v[j] = fc(0) * a0 + fc(1) * a1;
                 v[j | t] = fc(2) * a0 + fc(3) * a1;
+            }
+            break;
+        }
+        case F_DENSE1V: {
+            const uint32_t t = 1u << bitOf(g.t1);
+            int vb[2], nv = 0;
+            for (int x = 0; x < 64 && nv < 2; x++)
+                if ((g.m1 >> x) & 1ull) vb[nv++] = bitOf(x);
+            for (int j = 0; j < D; j++) {
+                if (j & t) continue;
+                int w = 0;
+                for (int b = 0; b < nv; b++) w |= ((j >> vb[b]) & 1) << b;
+                const double* m = g.f + 8 * w;
+                const cx a0 = v[j], a1 = v[j | t];
+                v[j] = cx(m[0], m[1]) * a0 + cx(m[2], m[3]) * a1;
+                v[j | t] = cx(m[4], m[5]) * a0 + cx(m[6], m[7]) * a1;
             }
             break;
         }
@@ -610,7 +639,8 @@ bool diagonalOp(const GOp& g) { return g.kind == F_DIAG1 || g.kind == F_PARITY |
 
 void fuse::mergePass(const GOp* ops, size_t num, uint64_t localMask, std::vector<GOp>& out) {
     struct Block {
-        uint64_t qubits;
+        uint64_t qubits;                // local qubits
+        uint64_t variants;              // non-local qubits (members are diagonal in them)
         std::vector<uint32_t> members;  // positions in ops, ascending
         bool dense;                     // some member is not diagonal
     };
@@ -620,43 +650,69 @@ void fuse::mergePass(const GOp* ops, size_t num, uint64_t localMask, std::vector
             for (uint32_t m : b.members) out.push_back(ops[m]);
             return;
         }
-        int q[2], nq = 0;
-        for (int x = 0; x < 64 && nq < 2; x++)
+        // the block's local qubits, then its variant qubits, ascending
+        int q[4], nq = 0;
+        for (int x = 0; x < 64; x++)
             if ((b.qubits >> x) & 1ull) q[nq++] = x;
-        const int D = 1 << nq;
-        std::complex<double> M[16];
+        const int nl = nq;
+        for (int x = 0; x < 64; x++)
+            if ((b.variants >> x) & 1ull) q[nq++] = x;
+        const int D = 1 << nq, DL = 1 << nl;
+        std::complex<double> M[64];
         for (int c = 0; c < D; c++) {
-            std::complex<double> v[4] = {};
+            std::complex<double> v[8] = {};
             v[c] = 1.0;
             for (uint32_t m : b.members) applySmall(ops[m], q, nq, v);
             for (int r = 0; r < D; r++) M[r * D + c] = v[r];
         }
         GOp g;
-        g.kind = nq == 1 ? F_DENSE1 : F_DENSE2;
         g.t1 = q[0];
-        g.t2 = nq == 2 ? q[1] : -1;
-        for (int i = 0; i < D * D; i++) {
-            g.f[2 * i] = M[i].real();
-            g.f[2 * i + 1] = M[i].imag();
+        if (nq == nl) {
+            g.kind = nl == 1 ? F_DENSE1 : F_DENSE2;
+            g.t2 = nl == 2 ? q[1] : -1;
+            for (int i = 0; i < D * D; i++) {
+                g.f[2 * i] = M[i].real();
+                g.f[2 * i + 1] = M[i].imag();
+            }
+        } else {
+            // one 2x2 matrix per value w of the variant qubits: the diagonal
+            // block of M at local index r | w << 1
+            g.kind = F_DENSE1V;
+            g.m1 = b.variants;
+            for (int w = 0; w < (D >> 1); w++)
+                for (int r = 0; r < DL; r++)
+                    for (int c = 0; c < DL; c++) {
+                        const std::complex<double> e = M[(r | w << 1) * D + (c | w << 1)];
+                        g.f[8 * w + 2 * (2 * r + c)] = e.real();
+                        g.f[8 * w + 2 * (2 * r + c) + 1] = e.imag();
+                    }
         }
         out.push_back(g);
     };
     for (size_t i = 0; i < num; i++) {
         const GOp& g = ops[i];
         const uint64_t T = supportMask(g);
-        uint64_t U = T;
+        uint64_t UL = T & localMask, UN = T & ~localMask;
         bool dense = !diagonalOp(g);
         for (const Block& b : open)
-            if (b.qubits & T) {
-                U |= b.qubits;
+            if ((b.qubits | b.variants) & T) {
+                UL |= b.qubits;
+                UN |= b.variants;
                 dense |= b.dense;
             }
-        // a block that ends up dense becomes one matrix on its qubits, which
-        // must then all be local
-        const bool fits = mergeable(g) && __builtin_popcountll(U) <= 2 && (!dense || (U & ~localMask) == 0);
-        Block nb{fits ? U : T, {}, !diagonalOp(g)};
+        // a block that ends up dense becomes one matrix on its (at most two)
+        // local qubits, or, with non-local qubits, one 2x2 matrix per value of
+        // at most two of them, chosen per tile
+        const int nl = __builtin_popcountll(UL), nn = __builtin_popcountll(UN);
+        bool fits = mergeable(g);
+        if (fits) {
+            if (!dense) fits = nl + nn <= 2;
+            else if (nn == 0) fits = nl <= 2;
+            else fits = nl <= 1 && nn <= 2;
+        }
+        Block nb{fits ? UL : (T & localMask), fits ? UN : (T & ~localMask), {}, !diagonalOp(g)};
         for (size_t b = 0; b < open.size();) {
-            if (!(open[b].qubits & T)) {
+            if (!((open[b].qubits | open[b].variants) & T)) {
                 b++;
                 continue;
             }
@@ -668,7 +724,7 @@ void fuse::mergePass(const GOp* ops, size_t num, uint64_t localMask, std::vector
             }
             open.erase(open.begin() + b);
         }
-        if (!mergeable(g) || (!fits && !diagonalOp(g) && (T & ~localMask) != 0)) {
+        if (!mergeable(g)) {
             out.push_back(g);
             continue;
         }
@@ -795,7 +851,7 @@ namespace fuse {
 double opFlops(const GOp& g, int n) {
     double perAmp = 0;
     switch (g.kind) {
-        case F_DENSE1: perAmp = 16; break;   // 4 complex MACs per pair
+        case F_DENSE1: case F_DENSE1V: perAmp = 16; break;   // 4 complex MACs per pair
         case F_DENSE2: perAmp = 32; break;   // 16 complex MACs per 4 amplitudes
         case F_DIAG1: perAmp = 6; break;     // one complex multiply
         case F_PARITY: perAmp = 6; break;

@@ -6,15 +6,17 @@
 namespace qa {
 namespace fuse {
 
-enum Kind : int { F_DENSE1 = 1, F_DENSE2, F_DIAG1, F_PARITY, F_PAULI, F_SWAP, F_DEPOL1, F_DEPH1 };
+enum Kind : int { F_DENSE1 = 1, F_DENSE2, F_DIAG1, F_PARITY, F_PAULI, F_SWAP, F_DEPOL1, F_DEPH1, F_DENSE1V };
 
 // a queued gate in global index space
 struct GOp {
     int kind = 0;
     int t1 = -1, t2 = -1;            // targets (DEPOL1/DEPH1: ket, bra)
     uint64_t cMask = 0, cVal = 0;    // control condition: (i & cMask) == cVal
-    uint64_t m1 = 0, m2 = 0;         // PARITY: parity mask; PAULI: maskXY, maskYZ
-    double f[32] = {0};              // matrix / factors (re, im interleaved)
+    uint64_t m1 = 0, m2 = 0;         // PARITY: parity mask; PAULI: maskXY, maskYZ; DENSE1V: variant qubits
+    double f[32] = {0};              // matrix / factors (re, im interleaved); DENSE1V: one 2x2
+                                     // matrix per value w of the (at most 2) variant qubits, at
+                                     // f[8w], bit b of w = the b-th lowest variant qubit
 };
 
 struct Config {

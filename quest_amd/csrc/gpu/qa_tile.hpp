@@ -53,6 +53,9 @@ static_assert(sizeof(TOpHead) == 64, "op header is one 64-byte scalar load");
 // scalar load each, and (for two-qubit dense matrices only) the rest of
 // the matrix.  Body by kind:
 //   DENSE1, DENSE2: b = matrix elements 0..3 (re, im interleaved)
+//   DENSE1V: b = the 2x2 matrix for variant 0, fx = variants 1..3; the
+//     variant is the value of the non-local qubits rb2 (bit 0) and pad - 1
+//     (bit 1, when pad > 0) at the tile
 //   DIAG1, PARITY, PAULI: b[0..3] = two complex factors (d0, d1 / f0, f1 /
 //     ampFac, pairFac), b[4] = the tile part of the target bit (DIAG1), the
 //     parity mask (PARITY) or the maskYZ bits (PAULI) outside the
@@ -406,6 +409,20 @@ __device__ __forceinline__ void applyOpC(ct<T>* r, const TOp* __restrict__ op, c
                 case 3: regDense1<T, 3, C>(r, h, f); break;
             }
             break;
+        case F_DENSE1V: {
+            // a 2x2 matrix per value of one or two non-local qubits: the
+            // tile's value picks it (uniform over the workgroup)
+            uint32_t w = (uint32_t)((base >> h.rb2) & 1ull);
+            if (h.pad) w |= (uint32_t)((base >> (h.pad - 1)) & 1ull) << 1;
+            const double* fv = w ? op->fx + 8 * (w - 1) : f;
+            switch (h.rb1) {
+                case 0: regDense1<T, 0, C>(r, h, fv); break;
+                case 1: regDense1<T, 1, C>(r, h, fv); break;
+                case 2: regDense1<T, 2, C>(r, h, fv); break;
+                case 3: regDense1<T, 3, C>(r, h, fv); break;
+            }
+            break;
+        }
         case F_DENSE1N: {
             // N = rb2 uncontrolled one-qubit matrices on register bits 0..N-1
             // in that order: straight-line code, one dispatch for N gates
@@ -683,8 +700,12 @@ void pickPass(const GOp* ops, size_t num, int numQubits, int kMax, int low, uint
 // multiplied, in FP64 on the host, into one dense matrix applied as one op:
 // each gate joins the open run that shares a qubit with it, runs that a gate
 // bridges are joined, and a run is emitted when a gate involving its qubits
-// cannot join it (so everything emitted before it commutes with it).  Runs
-// of diagonal gates only, and runs of one gate, are emitted unchanged.
+// cannot join it (so everything emitted before it commutes with it).  A run
+// on one local qubit whose gates also involve one or two non-local qubits
+// (as controls or diagonal factors, which makes the run block-diagonal in
+// them) becomes one DENSE1V op: a 2x2 matrix per value of those qubits,
+// picked per tile.  Runs of diagonal gates only, and runs of one gate, are
+// emitted unchanged.
 void mergePass(const GOp* ops, size_t num, uint64_t localMask, std::vector<GOp>& out);
 
 // the ops of one pass reordered so that ops on the same register group are

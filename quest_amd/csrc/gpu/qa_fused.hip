@@ -381,8 +381,52 @@ TOp ldsPauli(const GOp& g, const Layout& L) {
     return t;
 }
 
+// appends the PHASES terms of a diagonal gate (DIAG1, or a PARITY on one
+// qubit) under layout L; false, appending nothing, for any other gate or
+// when the terms would not fit one op
+bool phaseTerms(const GOp& g, const Layout& L, std::vector<PhaseTerm>& terms) {
+    uint64_t t;
+    std::complex<double> f0, f1;
+    if (g.kind == F_DIAG1) {
+        t = 1ull << g.t1;
+        f0 = {g.f[0], g.f[1]};
+        f1 = {g.f[2], g.f[3]};
+    } else if (g.kind == F_PARITY && __builtin_popcountll(g.m1) == 1) {
+        t = g.m1;
+        f0 = {g.f[0], g.f[1]};
+        f1 = {g.f[2], g.f[3]};
+    } else {
+        return false;
+    }
+    // the target bit set (factor f1) and clear (f0) under the controls;
+    // a phase-only gate has its target among its controls
+    PhaseTerm add[2];
+    int na = 0;
+    const uint64_t m = g.cMask | t;
+    for (int bit = 1; bit >= 0; bit--) {
+        if ((g.cMask & t) && ((g.cVal & t) != 0) != (bit == 1)) continue;
+        const std::complex<double> f = bit ? f1 : f0;
+        if (f == std::complex<double>(1.0, 0.0)) continue;
+        const uint64_t v = (g.cVal & ~t) | (bit ? t : 0);
+        PhaseTerm& pt = add[na++];
+        uint64_t jm, jv;
+        splitMask(m, L, pt.rm, jm, pt.gm);
+        splitMask(v, L, pt.rv, jv, pt.gv);
+        pt.jm = (uint32_t)jm;
+        pt.jv = (uint32_t)jv;
+        pt.fre = f.real();
+        pt.fim = f.imag();
+    }
+    // register part: empty, or one bit that must be set
+    for (int i = 0; i < na; i++)
+        if (__builtin_popcount(add[i].rm) > 1 || add[i].rv != add[i].rm) return false;
+    if ((int)terms.size() + na > MAX_PHASE_TERMS) return false;
+    for (int i = 0; i < na; i++) terms.push_back(add[i]);
+    return true;
+}
+
 void emitGroups(const GOp* ops, size_t from, size_t to, Layout& L, int k, int low, std::vector<TOp>& prog,
-                GroupState& st) {
+                GroupState& st, bool phases) {
     size_t i = from;
     while (i < to) {
         if (isLdsOp(ops[i])) {
@@ -431,6 +475,28 @@ void emitGroups(const GOp* ops, size_t from, size_t to, Layout& L, int k, int lo
         st.lastIsLds = false;
         st.anyItem = true;
         while (i < j) {
+            if (phases) {
+                // a run of diagonal gates becomes one PHASES op
+                std::vector<PhaseTerm> terms;
+                size_t n = 0;
+                while (i + n < j && phaseTerms(ops[i + n], L, terms)) n++;
+                if (n >= 2) {
+                    std::stable_sort(terms.begin(), terms.end(),
+                                     [](const PhaseTerm& x, const PhaseTerm& y) { return x.rm < y.rm; });
+                    TOp t;
+                    std::memset(&t, 0, sizeof(t));
+                    t.h.kind = F_PHASES;
+                    t.h.rb1 = (int)terms.size();
+                    for (const PhaseTerm& pt : terms) {
+                        const int sl = pt.rm ? 1 + __builtin_ctz(pt.rm) : 0;
+                        t.h.rm1 += 1u << (4 * sl);
+                    }
+                    std::memcpy(t.b, terms.data(), terms.size() * sizeof(PhaseTerm));
+                    prog.push_back(t);
+                    i += n;
+                    continue;
+                }
+            }
             // a run of 2-4 uncontrolled one-qubit gates on register bits
             // 0, 1, ... in order becomes one DENSE1N op (same arithmetic)
             size_t n = 0;
@@ -768,7 +834,7 @@ void fuse::clusterPass(const GOp* ops, size_t num, std::vector<GOp>& out) {
 }
 
 void fuse::planPass(const GOp* ops, size_t num, int numQubits, uint64_t localMask, int lowQubits, TileArgs& a,
-                    std::vector<TOp>& prog) {
+                    std::vector<TOp>& prog, bool phases) {
     Layout L;
     passShape(numQubits, localMask, a, L);
     const int k = a.k;
@@ -776,7 +842,7 @@ void fuse::planPass(const GOp* ops, size_t num, int numQubits, uint64_t localMas
     const uint32_t lowSlots = (1u << low) - 1u;
     const size_t start = prog.size();
     GroupState st;
-    emitGroups(ops, 0, num, L, k, low, prog, st);
+    emitGroups(ops, 0, num, L, k, low, prog, st, phases);
     if (st.firstIsLds) prog.insert(prog.begin() + start, placeholderGroup(k));
     a.firstDirect = !st.firstIsLds && (st.firstG & lowSlots) == 0;
     a.lastDirect = !st.lastIsLds && (st.lastG & lowSlots) == 0;
@@ -810,7 +876,7 @@ void fuse::planPassBlocks(const GOp* ops, size_t num, int numQubits, uint64_t lo
         if (k >= 10 && (int)(j - i) >= blockMinGates && blockMinGates > 0) {
             if (pending < i) {
                 if (!any) firstItemGroup = true;
-                emitGroups(ops, pending, i, L, k, low, prog, st);
+                emitGroups(ops, pending, i, L, k, low, prog, st, false);
                 any = true;
                 lastItemGroup = true;
             }
@@ -828,7 +894,7 @@ void fuse::planPassBlocks(const GOp* ops, size_t num, int numQubits, uint64_t lo
     }
     if (pending < num) {
         if (!any) firstItemGroup = true;
-        emitGroups(ops, pending, num, L, k, low, prog, st);
+        emitGroups(ops, pending, num, L, k, low, prog, st, false);
         lastItemGroup = true;
     }
     if (st.firstIsLds && firstItemGroup) firstItemGroup = false;
@@ -899,7 +965,7 @@ int launchPass(void* amps, const Queue& qu, const std::vector<GOp>& ops, size_t 
             g_blockPos += nf;
         }
     } else {
-        planPass(ops.data() + begin, end - begin, qu.numQubits, localMask, g_cfg.lowQubits, a, prog);
+        planPass(ops.data() + begin, end - begin, qu.numQubits, localMask, g_cfg.lowQubits, a, prog, !g_cfg.exact);
     }
     const int k = a.k;
 

@@ -401,7 +401,7 @@ bool launchStep(Engine& E, StepBuf& sb, std::string& err) {
             mergePass(pass.data(), pass.size(), L, merged);
             pass.clear();
             clusterPass(merged.data(), merged.size(), pass);
-            planPass(pass.data(), pass.size(), r->n, L, E.lowQ, pd.a, prog);
+            planPass(pass.data(), pass.size(), r->n, L, E.lowQ, pd.a, prog, true);
             r->firstPass = false;
             r->ops.swap(rest);
             byK[pd.a.k].push_back(pd);

@@ -34,7 +34,7 @@ namespace fuse {
 
 constexpr int RB = 4;        // register slots per group
 constexpr int RA = 1 << RB;  // amplitudes per thread
-enum { F_GROUP = 100, F_BLOCK = 101, F_DENSE1N = 102, F_PAULI_LDS = 103 };
+enum { F_GROUP = 100, F_BLOCK = 101, F_DENSE1N = 102, F_PAULI_LDS = 103, F_PHASES = 104 };
 constexpr int BQ = 5;  // qubits of an MFMA block (32 x 32 complex)
 
 // the first 64 bytes of an op, fetched with one scalar load
@@ -65,9 +65,24 @@ static_assert(sizeof(TOpHead) == 64, "op header is one 64-byte scalar load");
 struct alignas(64) TOp {
     TOpHead h;
     uint64_t b[8];
-    double fx[24];  // DENSE2: matrix elements 4..15
+    double fx[40];  // DENSE2: matrix elements 4..15
 };
 static_assert(sizeof(TOp) % 64 == 0, "ops stay 64-byte aligned in arrays");
+
+// PHASES: a run of diagonal gates as terms stored from b onwards; term k
+// multiplies every amplitude whose index i has (i & mask) == value by f,
+// with the condition split into its register (rm, rv), tile (jm, jv) and
+// non-local (gm, gv) parts.  The register part is empty or one register bit
+// that must be set; terms come sorted by it (empty first, then bits 0..RB-1)
+// with the number of each in a nibble of the head's rm1, so each thread
+// multiplies the factors of one slot together first.
+struct PhaseTerm {
+    uint32_t rm, rv, jm, jv;
+    uint64_t gm, gv;
+    double fre, fim;
+};
+static_assert(sizeof(PhaseTerm) == 48, "phase terms pack 8 to an op");
+constexpr int MAX_PHASE_TERMS = (int)((sizeof(TOp) - sizeof(TOpHead)) / sizeof(PhaseTerm));
 
 typedef uint32_t u32x16 __attribute__((ext_vector_type(16)));
 
@@ -409,6 +424,33 @@ __device__ __forceinline__ void applyOpC(ct<T>* r, const TOp* __restrict__ op, c
                 case 3: regDense1<T, 3, C>(r, h, f); break;
             }
             break;
+        case F_PHASES: {
+            // per-thread factors: A for every amplitude, B[b] for those with
+            // register bit b set; the terms come sorted by that slot, with
+            // the count of each slot in a nibble of rm1
+            const PhaseTerm* pt = reinterpret_cast<const PhaseTerm*>(op->b);
+            ct<T> acc[1 + RB];
+            int k = 0;
+#pragma unroll
+            for (int sl = 0; sl <= RB; sl++) {
+                acc[sl] = ct<T>{(T)1, (T)0};
+                const int ns = (int)((h.rm1 >> (4 * sl)) & 15u);
+                for (int e = 0; e < ns; e++, k++) {
+                    const PhaseTerm t = pt[k];
+                    const bool on = (jb & t.jm) == t.jv && (base & t.gm) == t.gv;
+                    acc[sl] = cmulT(ct<T>{on ? (T)t.fre : (T)1, on ? (T)t.fim : (T)0}, acc[sl]);
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < RA; v++) {
+                ct<T> x = cmulT(acc[0], r[v]);
+#pragma unroll
+                for (int b = 0; b < RB; b++)
+                    if (v & (1 << b)) x = cmulT(acc[1 + b], x);
+                r[v] = x;
+            }
+            break;
+        }
         case F_DENSE1V: {
             // a 2x2 matrix per value of one or two non-local qubits: the
             // tile's value picks it (uniform over the workgroup)
@@ -717,9 +759,11 @@ void clusterPass(const GOp* ops, size_t num, std::vector<GOp>& out);
 
 // the program of one pass applying ops[0..num) (dense targets all inside
 // localMask) to a state of numQubits index bits: fills every field of `a`
-// and appends the ops, with their GROUP ops, to `prog`
+// and appends the ops, with their GROUP ops, to `prog`; with `phases`, runs
+// of diagonal gates become PHASES ops (their factors are then multiplied
+// before they reach the amplitudes, which changes rounding)
 void planPass(const GOp* ops, size_t num, int numQubits, uint64_t localMask, int lowQubits, TileArgs& a,
-              std::vector<TOp>& prog);
+              std::vector<TOp>& prog, bool phases = false);
 
 // the same for a complex64 state, where runs of at least blockMinGates ops
 // whose qubits (targets, controls, masks) fit 5 local qubits become one

@@ -225,15 +225,58 @@ __device__ __forceinline__ void ldsPut(ct<T>* tile, uint32_t i, ct<T> a) {
     reinterpret_cast<typename Amp<T>::V*>(tile)[i] = x;
 }
 
+// byte offset of tile amplitude j in LDS: the swizzle is linear over XOR,
+// and a register layout's amplitude indices are jb | offJ<V> with disjoint
+// bits, so each thread computes swz(jb) once and every register's offset is
+// one XOR with a uniform constant
+template <typename T>
+__device__ __forceinline__ uint32_t ldsByte(uint32_t j) {
+    return swz<T>(j) * (uint32_t)sizeof(typename Amp<T>::V);
+}
+template <typename T>
+__device__ __forceinline__ ct<T> ldsGetB(const ct<T>* tile, uint32_t byteOff) {
+    const typename Amp<T>::V x =
+        *reinterpret_cast<const typename Amp<T>::V*>(reinterpret_cast<const unsigned char*>(tile) + byteOff);
+    return ct<T>{x.x, x.y};
+}
+template <typename T>
+__device__ __forceinline__ void ldsPutB(ct<T>* tile, uint32_t byteOff, ct<T> a) {
+    typename Amp<T>::V x;
+    x.x = a.re;
+    x.y = a.im;
+    *reinterpret_cast<typename Amp<T>::V*>(reinterpret_cast<unsigned char*>(tile) + byteOff) = x;
+}
+
 template <typename T, int NT, int V>
 struct RegIO {
+    __device__ __forceinline__ static void ldsLoadAt(ct<T>* r, const ct<T>* tile, uint32_t sj, const RegLayout& L) {
+        r[V] = ldsGetB(tile, sj ^ ldsByte<T>(offJ<V>(L)));
+        RegIO<T, NT, V + 1>::ldsLoadAt(r, tile, sj, L);
+    }
+    __device__ __forceinline__ static void ldsStoreAt(const ct<T>* r, ct<T>* tile, uint32_t sj, const RegLayout& L) {
+        ldsPutB(tile, sj ^ ldsByte<T>(offJ<V>(L)), r[V]);
+        RegIO<T, NT, V + 1>::ldsStoreAt(r, tile, sj, L);
+    }
+    // (the thread's base offset and the layout's slots are made opaque so
+    // that the per-register offsets are formed at each use, not hoisted out
+    // of the tile loop into sixteen more live registers)
+    __device__ __forceinline__ static RegLayout opaque(const RegLayout& L, uint32_t& sj) {
+        RegLayout o = L;
+        sj = ldsByte<T>(L.jb);
+        asm volatile("" : "+v"(sj));
+#pragma unroll
+        for (int b = 0; b < RB; b++) asm volatile("" : "+s"(o.sb[b]));
+        return o;
+    }
     __device__ __forceinline__ static void ldsLoad(ct<T>* r, const ct<T>* tile, const RegLayout& L) {
-        r[V] = ldsGet(tile, swz<T>(L.jb | offJ<V>(L)));
-        RegIO<T, NT, V + 1>::ldsLoad(r, tile, L);
+        uint32_t sj;
+        const RegLayout o = opaque(L, sj);
+        ldsLoadAt(r, tile, sj, o);
     }
     __device__ __forceinline__ static void ldsStore(const ct<T>* r, ct<T>* tile, const RegLayout& L) {
-        ldsPut(tile, swz<T>(L.jb | offJ<V>(L)), r[V]);
-        RegIO<T, NT, V + 1>::ldsStore(r, tile, L);
+        uint32_t sj;
+        const RegLayout o = opaque(L, sj);
+        ldsStoreAt(r, tile, sj, o);
     }
     __device__ __forceinline__ static void hbmLoad(ct<T>* r, const T* amps, uint64_t at, const RegLayout& L) {
         r[V] = ldx<NT>(amps, at | offQ<V>(L));
@@ -246,8 +289,8 @@ struct RegIO {
 };
 template <typename T, int NT>
 struct RegIO<T, NT, RA> {
-    __device__ __forceinline__ static void ldsLoad(ct<T>*, const ct<T>*, const RegLayout&) {}
-    __device__ __forceinline__ static void ldsStore(const ct<T>*, ct<T>*, const RegLayout&) {}
+    __device__ __forceinline__ static void ldsLoadAt(ct<T>*, const ct<T>*, uint32_t, const RegLayout&) {}
+    __device__ __forceinline__ static void ldsStoreAt(const ct<T>*, ct<T>*, uint32_t, const RegLayout&) {}
     __device__ __forceinline__ static void hbmLoad(ct<T>*, const T*, uint64_t, const RegLayout&) {}
     __device__ __forceinline__ static void hbmStore(const ct<T>*, T*, uint64_t, const RegLayout&) {}
 };

@@ -1,0 +1,346 @@
+// CPU check of the fusion planner's host-side transformations (test
+// infrastructure; built and run by tests/test_planner.py, never shipped).
+//
+// Random queues of the fusable gate kinds (with controls and masks on local
+// and non-local qubits) on a small state are planned the way a flush plans
+// them -- pickPass, then mergePass and clusterPass (default mode) or
+// takeRun (exact mode) -- and the planned op sequence is applied to a copy
+// of a random state.  Both the queue and the planned sequence are applied
+// with this file's own full-state restatement of each op's semantics (the
+// single-gate kernels' formulas), independent of the planner's helpers; the
+// two results must agree to rounding.  Prints one line per case and exits
+// non-zero on the first mismatch.
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+#include <vector>
+
+#include "qa_tile.hpp"
+
+using namespace qa::fuse;
+typedef std::complex<double> cx;
+
+static int parity(uint64_t x) { return __builtin_popcountll(x) & 1; }
+
+// one op on the full state, as the single-gate kernels define it
+static void apply(const GOp& g, std::vector<cx>& a) {
+    const uint64_t N = a.size();
+    auto ok = [&](uint64_t i) { return (i & g.cMask) == g.cVal; };
+    auto f = [&](int k) { return cx(g.f[2 * k], g.f[2 * k + 1]); };
+    switch (g.kind) {
+        case F_DENSE1: {
+            const uint64_t t = 1ull << g.t1;
+            for (uint64_t i = 0; i < N; i++) {
+                if ((i & t) || !ok(i)) continue;
+                const cx x0 = a[i], x1 = a[i | t];
+                a[i] = f(0) * x0 + f(1) * x1;
+                a[i | t] = f(2) * x0 + f(3) * x1;
+            }
+            break;
+        }
+        case F_DENSE1V: {
+            const uint64_t t = 1ull << g.t1;
+            for (uint64_t i = 0; i < N; i++) {
+                if (i & t) continue;
+                int w = 0, b = 0;
+                for (int q = 0; q < 64; q++)
+                    if ((g.m1 >> q) & 1ull) w |= (int)((i >> q) & 1ull) << b++;
+                const double* m = g.f + 8 * w;
+                const cx x0 = a[i], x1 = a[i | t];
+                a[i] = cx(m[0], m[1]) * x0 + cx(m[2], m[3]) * x1;
+                a[i | t] = cx(m[4], m[5]) * x0 + cx(m[6], m[7]) * x1;
+            }
+            break;
+        }
+        case F_DENSE2: {
+            const uint64_t t1 = 1ull << g.t1, t2 = 1ull << g.t2;
+            const uint64_t off[4] = {0, t1, t2, t1 | t2};
+            for (uint64_t i = 0; i < N; i++) {
+                if ((i & (t1 | t2)) || !ok(i)) continue;
+                cx x[4], y[4];
+                for (int c = 0; c < 4; c++) x[c] = a[i | off[c]];
+                for (int r = 0; r < 4; r++) {
+                    y[r] = 0;
+                    for (int c = 0; c < 4; c++) y[r] += f(4 * r + c) * x[c];
+                }
+                for (int r = 0; r < 4; r++) a[i | off[r]] = y[r];
+            }
+            break;
+        }
+        case F_DIAG1:
+            for (uint64_t i = 0; i < N; i++)
+                if (ok(i)) a[i] *= ((i >> g.t1) & 1ull) ? f(1) : f(0);
+            break;
+        case F_PARITY:
+            for (uint64_t i = 0; i < N; i++)
+                if (ok(i)) a[i] *= parity(i & g.m1) ? f(1) : f(0);
+            break;
+        case F_PAULI: {
+            const uint64_t top = 1ull << (63 - __builtin_clzll(g.m1));
+            for (uint64_t i = 0; i < N; i++) {
+                if ((i & top) || !ok(i)) continue;
+                const uint64_t j = i ^ g.m1;
+                const double sA = parity(i & g.m2) ? -1 : 1, sB = parity(j & g.m2) ? -1 : 1;
+                const cx xa = a[i], xb = a[j];
+                a[i] = f(0) * xa + f(1) * sB * xb;
+                a[j] = f(0) * xb + f(1) * sA * xa;
+            }
+            break;
+        }
+        case F_SWAP: {
+            const uint64_t t1 = 1ull << g.t1, t2 = 1ull << g.t2;
+            for (uint64_t i = 0; i < N; i++)
+                if ((i & t1) && !(i & t2) && ok(i)) std::swap(a[i], a[i ^ (t1 | t2)]);
+            break;
+        }
+        case F_DEPOL1: {
+            const uint64_t k1 = 1ull << g.t1, k2 = 1ull << g.t2;
+            const double c1 = g.f[0], c2 = g.f[1], c3 = g.f[2];
+            for (uint64_t i = 0; i < N; i++) {
+                if (i & (k1 | k2)) continue;
+                const cx a00 = a[i], a11 = a[i | k1 | k2];
+                a[i] = c1 * a00 + c2 * a11;
+                a[i | k1] *= c3;
+                a[i | k2] *= c3;
+                a[i | k1 | k2] = c1 * a11 + c2 * a00;
+            }
+            break;
+        }
+        case F_DEPH1: {
+            const uint64_t k1 = 1ull << g.t1, k2 = 1ull << g.t2;
+            for (uint64_t i = 0; i < N; i++)
+                if (((i & k1) != 0) != ((i & k2) != 0)) a[i] *= g.f[0];
+            break;
+        }
+        default:
+            std::fprintf(stderr, "unknown op kind %d\n", g.kind);
+            std::exit(3);
+    }
+}
+
+struct Rng {
+    std::mt19937_64 g;
+    explicit Rng(uint64_t s) : g(s) {}
+    int below(int n) { return (int)(g() % (uint64_t)n); }
+    double unit() { return std::uniform_real_distribution<double>(-1, 1)(g); }
+};
+
+static void randomUnitary2(Rng& r, double* f) {
+    // e^{i a} [[cos t e^{i b}, -sin t e^{-i c}], [sin t e^{i c}, cos t e^{-i b}]]
+    const double t = r.unit() * 3, b = r.unit() * 3, c = r.unit() * 3, ph = r.unit() * 3;
+    const cx e(std::cos(ph), std::sin(ph));
+    const cx m[4] = {e * std::cos(t) * cx(std::cos(b), std::sin(b)), -e * std::sin(t) * cx(std::cos(c), -std::sin(c)),
+                     e * std::sin(t) * cx(std::cos(c), std::sin(c)), e * std::cos(t) * cx(std::cos(b), -std::sin(b))};
+    for (int i = 0; i < 4; i++) {
+        f[2 * i] = m[i].real();
+        f[2 * i + 1] = m[i].imag();
+    }
+}
+
+// a random fusable gate; `dens` adds the one-qubit channels on (q, q + n/2)
+static GOp randomGate(Rng& r, int n, bool dens, int hot) {
+    GOp g;
+    auto qubit = [&]() { return r.below(4) == 0 ? r.below(n) : n - 1 - r.below(hot); };
+    auto other = [&](int a) {
+        int b;
+        do b = qubit(); while (b == a);
+        return b;
+    };
+    auto ctrl = [&](uint64_t busy) {
+        if (r.below(3)) return;
+        int c;
+        do c = r.below(n); while ((busy >> c) & 1ull);
+        g.cMask |= 1ull << c;
+        if (r.below(4)) g.cVal |= 1ull << c;
+    };
+    const int kind = r.below(dens ? 10 : 8);
+    switch (kind) {
+        case 0: case 1:
+            g.kind = F_DENSE1;
+            g.t1 = qubit();
+            randomUnitary2(r, g.f);
+            ctrl(1ull << g.t1);
+            break;
+        case 2: {
+            g.kind = F_DENSE2;
+            g.t1 = qubit();
+            g.t2 = other(g.t1);
+            for (int i = 0; i < 32; i++) g.f[i] = r.unit();
+            ctrl((1ull << g.t1) | (1ull << g.t2));
+            break;
+        }
+        case 3: {
+            g.kind = F_DIAG1;
+            g.t1 = qubit();
+            const double a0 = r.unit() * 3, a1 = r.unit() * 3;
+            const bool phaseOnly = r.below(2);
+            g.f[0] = phaseOnly ? 1 : std::cos(a0);
+            g.f[1] = phaseOnly ? 0 : std::sin(a0);
+            g.f[2] = std::cos(a1);
+            g.f[3] = std::sin(a1);
+            ctrl(1ull << g.t1);
+            if (phaseOnly) {
+                g.cMask |= 1ull << g.t1;
+                g.cVal |= 1ull << g.t1;
+            }
+            break;
+        }
+        case 4: {
+            g.kind = F_PARITY;
+            const int a = qubit();
+            g.m1 = 1ull << a;
+            if (r.below(2)) g.m1 |= 1ull << other(a);
+            const double th = r.unit() * 3;
+            g.f[0] = std::cos(th);
+            g.f[1] = -std::sin(th);
+            g.f[2] = std::cos(th);
+            g.f[3] = std::sin(th);
+            ctrl(g.m1);
+            break;
+        }
+        case 5: {
+            g.kind = F_PAULI;
+            const int a = qubit(), b = other(a);
+            g.m1 = 1ull << a;
+            if (r.below(2)) g.m1 |= 1ull << b;
+            g.m2 = (r.below(2) ? (1ull << a) : 0) | (r.below(2) ? (1ull << r.below(n)) : 0);
+            const double th = r.unit() * 3;
+            g.f[0] = std::cos(th);
+            g.f[1] = 0;
+            g.f[2] = 0;
+            g.f[3] = std::sin(th);
+            ctrl(g.m1 | g.m2);
+            break;
+        }
+        case 6:
+            g.kind = F_SWAP;
+            g.t1 = qubit();
+            g.t2 = other(g.t1);
+            ctrl((1ull << g.t1) | (1ull << g.t2));
+            break;
+        case 7: {
+            // a controlled phase, the workload's most common diagonal
+            g.kind = F_DIAG1;
+            g.t1 = qubit();
+            const int c = other(g.t1);
+            g.cMask = g.cVal = (1ull << g.t1) | (1ull << c);
+            const double th = r.unit() * 3;
+            g.f[0] = 1;
+            g.f[2] = std::cos(th);
+            g.f[3] = std::sin(th);
+            break;
+        }
+        default: {
+            g.kind = r.below(2) ? F_DEPOL1 : F_DEPH1;
+            const int q = r.below(n / 2);
+            g.t1 = q;
+            g.t2 = q + n / 2;
+            const double p = 0.05 * (r.unit() + 1);
+            g.f[0] = g.kind == F_DEPOL1 ? 1 - 2 * p / 3 : 1 - 2 * p;
+            g.f[1] = 2 * p / 3;
+            g.f[2] = 1 - 4 * p / 3;
+            break;
+        }
+    }
+    return g;
+}
+
+int main(int argc, char** argv) {
+    const int cases = argc > 1 ? std::atoi(argv[1]) : 200;
+    const uint64_t seed0 = argc > 2 ? std::strtoull(argv[2], nullptr, 10) : 1;
+    int checked = 0;
+    for (int c = 0; c < cases; c++) {
+        Rng r(seed0 * 1000003ull + (uint64_t)c);
+        const bool dens = c % 4 == 3;
+        const int n = dens ? 2 * (3 + r.below(3)) : 6 + r.below(5);
+        const int kMax = std::min(n, 4 + r.below(5));
+        const int low = r.below(std::max(1, kMax - 1));
+        const bool exact = c % 5 == 4;
+        const int num = 20 + r.below(120);
+        std::vector<GOp> queue;
+        const uint64_t lowMask = (1ull << low) - 1ull;
+        for (int i = 0; i < num; i++) {
+            GOp g = randomGate(r, n, dens, std::max(2, kMax / 2));
+            // the queue takes only gates whose dense targets fit a tile
+            if (__builtin_popcountll(denseMask(g) | lowMask) > kMax) continue;
+            queue.push_back(g);
+        }
+        std::vector<cx> a(1ull << n);
+        for (auto& x : a) x = cx(r.unit(), r.unit());
+        std::vector<cx> want = a, got = a;
+        for (const GOp& g : queue) apply(g, want);
+
+        std::vector<GOp> ops = queue, pass, rest, merged, planned;
+        std::vector<uint32_t> picked;
+        int passes = 0;
+        while (!ops.empty()) {
+            uint64_t L = 0;
+            pass.clear();
+            rest.clear();
+            if (exact) {
+                const size_t run = takeRun(ops.data(), ops.size(), n, kMax, low, L);
+                if (run == 0) {
+                    std::printf("case %d: empty run\n", c);
+                    return 1;
+                }
+                pass.assign(ops.begin(), ops.begin() + run);
+                rest.assign(ops.begin() + run, ops.end());
+                planned = pass;
+            } else {
+                picked.clear();
+                pickPass(ops.data(), ops.size(), n, kMax, low, L, picked);
+                if (picked.empty()) {
+                    std::printf("case %d: empty pass\n", c);
+                    return 1;
+                }
+                size_t p = 0;
+                for (size_t i = 0; i < ops.size(); i++) {
+                    if (p < picked.size() && picked[p] == i) {
+                        pass.push_back(ops[i]);
+                        p++;
+                    } else {
+                        rest.push_back(ops[i]);
+                    }
+                }
+                merged.clear();
+                mergePass(pass.data(), pass.size(), L, merged);
+                planned.clear();
+                clusterPass(merged.data(), merged.size(), planned);
+                if (planned.size() > pass.size()) {
+                    std::printf("case %d: planning added ops (%zu > %zu)\n", c, planned.size(), pass.size());
+                    return 1;
+                }
+            }
+            // every planned op's dense targets are local to the pass
+            for (const GOp& g : planned) {
+                if (denseMask(g) & ~L) {
+                    std::printf("case %d: op kind %d has dense targets outside the local set\n", c, g.kind);
+                    return 1;
+                }
+                apply(g, got);
+            }
+            if (__builtin_popcountll(L) != kMax || (L & lowMask) != lowMask) {
+                std::printf("case %d: local set %llx is not %d qubits with the low ones\n", c, (unsigned long long)L,
+                            kMax);
+                return 1;
+            }
+            ops.swap(rest);
+            passes++;
+        }
+        double err = 0, norm = 0;
+        for (size_t i = 0; i < a.size(); i++) {
+            err = std::max(err, std::abs(got[i] - want[i]));
+            norm = std::max(norm, std::abs(want[i]));
+        }
+        if (!(err <= 1e-12 * std::max(1.0, norm))) {
+            std::printf("case %d (n=%d k=%d low=%d exact=%d gates=%zu): max error %.3e\n", c, n, kMax, low, exact,
+                        queue.size(), err);
+            return 1;
+        }
+        checked++;
+    }
+    std::printf("planner check: %d cases agree\n", checked);
+    return 0;
+}

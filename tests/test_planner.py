@@ -1,0 +1,38 @@
+"""CPU check of the gate-fusion planner (no GPU): random queues of every
+fusable gate kind, with controls and masks on local and non-local qubits,
+planned as a flush plans them (default mode: commuting gates taken out of
+order, runs on one or two qubits merged -- including the per-tile variant
+matrices of DENSE1V -- and ops clustered by register layout; exact mode:
+queue prefixes) must act on a random state exactly as the queue does, to
+rounding (1e-12).  The planner runs from the built library; the state is
+updated by tests/planner_check.cpp's own restatement of each op."""
+
+import shutil
+import subprocess
+from pathlib import Path
+
+import pytest
+
+from quest_amd import GPU_LIB
+
+ROOT = Path(__file__).resolve().parents[1]
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+
+
+@pytest.fixture(scope="module")
+def planner_check(tmp_path_factory):
+    if not Path(HIPCC).exists():
+        pytest.skip("hipcc not available")
+    exe = tmp_path_factory.mktemp("planner") / "planner_check"
+    cmd = [HIPCC, "-std=c++17", "-O2", f"-I{ROOT / 'quest_amd' / 'csrc' / 'gpu'}", f"-I{ROOT / 'include'}",
+           str(ROOT / "tests" / "planner_check.cpp"), "-o", str(exe), f"-L{GPU_LIB.parent}", "-lquest_amd_gpu",
+           f"-Wl,-rpath,{GPU_LIB.parent}"]
+    subprocess.run(cmd, check=True, capture_output=True, text=True, timeout=300)
+    return exe
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_planned_passes_act_like_the_queue(planner_check, seed):
+    out = subprocess.run([str(planner_check), "300", str(seed)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "300 cases agree" in out.stdout

@@ -286,6 +286,16 @@ int qa_statevec_calcExpecPauliStr(const void* amps, int prec, int numQubits,
                                   uint64_t maskX, uint64_t maskY, uint64_t maskZ, double* out);
 int qa_densmatr_calcExpecPauliStr(const void* amps, int prec, int numQubitsN,
                                   uint64_t maskX, uint64_t maskY, uint64_t maskZ, double* out);
+/* localiser_statevec_calcExpecPauliStrSum (localiser.cpp:2052-2126):
+   out[2] = sum_t coeffs[t] <P_t> over numTerms Pauli strings given as X, Y, Z
+   bit masks, coeffs complex (re, im interleaved).  Strings that share their
+   X/Y mask mix the same amplitude pairs; up to 8 of them are reduced in one
+   read of the state, each term's value bitwise equal to
+   qa_statevec_calcExpecPauliStr's.  The weighted sum is taken in term
+   order.  termValues (2 doubles per term, may be NULL) receives every <P_t>. */
+int qa_statevec_calcExpecPauliStrSum(const void* amps, int prec, int numQubits, int64_t numTerms,
+                                     const uint64_t* maskX, const uint64_t* maskY, const uint64_t* maskZ,
+                                     const double* coeffs, double* termValues, double* out);
 /* out[2]; powMode 0 = none, 1 = complex power, 2 = real power of the real part */
 int qa_statevec_calcExpecFullStateDiagMatr(const void* amps, int prec, int numQubits,
                                            const void* devElems, int powMode,

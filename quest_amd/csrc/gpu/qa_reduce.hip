@@ -12,6 +12,9 @@
 //     value, which is copied to pinned host memory.
 // Amplitudes are widened to FP64 before squaring, so FP32 states reduce in
 // FP64 too.
+#include <algorithm>
+#include <vector>
+
 #include "qa_common.hpp"
 #include "qa_fused.hpp"
 #include "qa_sum.hpp"
@@ -81,6 +84,29 @@ int runReduce(F f, uint64_t numWork, double* out) {
     k_finalize<NV><<<1, BLOCK, 0, stream()>>>(scratch, blocks, scratch + RESULT_OFF);
     QA_LAUNCH_CHECK();
     QA_HIP(hipMemcpyAsync(host, scratch + RESULT_OFF, NV * sizeof(double), hipMemcpyDeviceToHost, stream()));
+    QA_HIP(hipStreamSynchronize(stream()));
+    for (int v = 0; v < NV; v++) out[v] = host[v];
+    return QA_OK;
+}
+
+// runReduce for NV values (more than MAXV): partials and results in a
+// larger slice of the same scratch
+template <int NV, typename F>
+int runReduceN(F f, uint64_t numWork, double* out) {
+    const size_t resOff = (size_t)RED_BLOCKS * NV * 2;
+    double* scratch = devScratch(resOff + NV);
+    double* host = hostScratch(NV);
+    if (!scratch || !host) {
+        setError("reduction scratch unavailable");
+        return QA_ERR_OOM;
+    }
+    uint64_t want = (numWork + (uint64_t)BLOCK * 4 - 1) / ((uint64_t)BLOCK * 4);
+    int blocks = (int)(want < 1 ? 1 : (want > RED_BLOCKS ? RED_BLOCKS : want));
+    k_reduce<NV, F><<<blocks, BLOCK, 0, stream()>>>(f, numWork, scratch);
+    QA_LAUNCH_CHECK();
+    k_finalize<NV><<<1, BLOCK, 0, stream()>>>(scratch, blocks, scratch + resOff);
+    QA_LAUNCH_CHECK();
+    QA_HIP(hipMemcpyAsync(host, scratch + resOff, NV * sizeof(double), hipMemcpyDeviceToHost, stream()));
     QA_HIP(hipStreamSynchronize(stream()));
     for (int v = 0; v < NV; v++) out[v] = host[v];
     return QA_OK;
@@ -165,6 +191,46 @@ struct FExpecPauliPair {
         double s2 = parity64(n & maskYZ) ? -1.0 : 1.0;
         o[0] = s1 * t1.re + s2 * t2.re;
         o[1] = s1 * t1.im + s2 * t2.im;
+    }
+};
+// NT Pauli strings that share their X/Y mask (maskXY), reduced together
+// with one read of the state: per term, the same two values (real,
+// imaginary) FExpecPauliPair (maskXY != 0) or FExpecPauli (maskXY == 0)
+// produce for that term alone, so each term's sum is bitwise the
+// single-string result
+template <typename T, int NT>
+struct FExpecPauliGroupPair {
+    const T* a;
+    InsertSpec topBit;
+    uint64_t maskXY;
+    uint64_t maskYZ[NT];
+    __device__ void operator()(uint64_t w, double* o) const {
+        uint64_t n = insertBits(w, topBit);
+        uint64_t j = n ^ maskXY;
+        cd x = ldd(a, n), y = ldd(a, j);
+        cd t1 = cmul(cconj(x), y), t2 = cmul(cconj(y), x);
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            double s1 = parity64(j & maskYZ[t]) ? -1.0 : 1.0;
+            double s2 = parity64(n & maskYZ[t]) ? -1.0 : 1.0;
+            o[2 * t] = s1 * t1.re + s2 * t2.re;
+            o[2 * t + 1] = s1 * t1.im + s2 * t2.im;
+        }
+    }
+};
+template <typename T, int NT>
+struct FExpecPauliGroupZ {
+    const T* a;
+    uint64_t maskYZ[NT];
+    __device__ void operator()(uint64_t n, double* o) const {
+        cd x = ldd(a, n), y = ldd(a, n);
+        cd tt = cmul(cconj(x), y);
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            double s = parity64(n & maskYZ[t]) ? -1.0 : 1.0;
+            o[2 * t] = s * tt.re;
+            o[2 * t + 1] = s * tt.im;
+        }
     }
 };
 template <typename T>
@@ -546,6 +612,105 @@ int qa_statevec_calcExpecPauliStr(const void* amps, int prec, int nq, uint64_t m
     }
     if (rc == QA_OK) timesPowerOfI(out, __builtin_popcountll(maskY));
     return rc;
+}
+
+}  // extern "C"
+
+namespace qa {
+namespace {
+
+// one launch for up to 8 strings sharing maskXY: out[2t], out[2t+1] = the
+// sum of string t before its i^#Y factor
+template <typename T, int NT>
+int expecGroup(const T* a, int nq, uint64_t mxy, const uint64_t* myz, double* out) {
+    if (mxy) {
+        FExpecPauliGroupPair<T, NT> f;
+        f.a = a;
+        f.topBit.num = 1;
+        f.topBit.pos[0] = 63 - __builtin_clzll(mxy);
+        f.topBit.valueMask = 0;
+        f.maskXY = mxy;
+        for (int t = 0; t < NT; t++) f.maskYZ[t] = myz[t];
+        return runReduceN<2 * NT>(f, 1ull << (nq - 1), out);
+    }
+    FExpecPauliGroupZ<T, NT> f;
+    f.a = a;
+    for (int t = 0; t < NT; t++) f.maskYZ[t] = myz[t];
+    return runReduceN<2 * NT>(f, 1ull << nq, out);
+}
+
+template <typename T>
+int expecGroupAny(const T* a, int nq, uint64_t mxy, const uint64_t* myz, int nt, double* out) {
+    switch (nt) {
+        case 1: return expecGroup<T, 1>(a, nq, mxy, myz, out);
+        case 2: return expecGroup<T, 2>(a, nq, mxy, myz, out);
+        case 3: return expecGroup<T, 3>(a, nq, mxy, myz, out);
+        case 4: return expecGroup<T, 4>(a, nq, mxy, myz, out);
+        case 5: return expecGroup<T, 5>(a, nq, mxy, myz, out);
+        case 6: return expecGroup<T, 6>(a, nq, mxy, myz, out);
+        case 7: return expecGroup<T, 7>(a, nq, mxy, myz, out);
+        default: return expecGroup<T, 8>(a, nq, mxy, myz, out);
+    }
+}
+
+}  // namespace
+}  // namespace qa
+
+extern "C" {
+
+int qa_statevec_calcExpecPauliStrSum(const void* amps, int prec, int nq, int64_t numTerms, const uint64_t* maskX,
+                                     const uint64_t* maskY, const uint64_t* maskZ, const double* coeffs,
+                                     double* termValues, double* out) {
+    QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
+    if (!checkArgs(amps, prec, nq, __func__)) return QA_ERR_ARG;
+    QA_ARG_CHECK(numTerms >= 0 && (numTerms == 0 || (maskX && maskY && maskZ && coeffs)),
+                 "%s: bad term arrays", __func__);
+    const uint64_t all = nq >= 64 ? ~0ull : ((1ull << nq) - 1ull);
+    for (int64_t t = 0; t < numTerms; t++)
+        QA_ARG_CHECK(((maskX[t] | maskY[t] | maskZ[t]) & ~all) == 0 && !(maskX[t] & maskY[t]) &&
+                         !(maskX[t] & maskZ[t]) && !(maskY[t] & maskZ[t]),
+                     "%s: term %lld is not a Pauli string on the qureg", __func__, (long long)t);
+    // terms grouped by X/Y mask (groups in ascending mask order, each in term
+    // order); one launch per 8 terms of a group
+    std::vector<int64_t> order(numTerms);
+    for (int64_t t = 0; t < numTerms; t++) order[t] = t;
+    std::stable_sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
+        return (maskX[x] | maskY[x]) < (maskX[y] | maskY[y]);
+    });
+    std::vector<double> vals(2 * (size_t)numTerms);
+    int rc = QA_OK;
+    for (int64_t g = 0; g < numTerms && rc == QA_OK;) {
+        const uint64_t mxy = maskX[order[g]] | maskY[order[g]];
+        int64_t e = g;
+        while (e < numTerms && (maskX[order[e]] | maskY[order[e]]) == mxy && e - g < 8) e++;
+        const int nt = (int)(e - g);
+        uint64_t myz[8];
+        double res[16];
+        for (int t = 0; t < nt; t++) myz[t] = maskY[order[g + t]] | maskZ[order[g + t]];
+        QA_DISPATCH(prec, rc = expecGroupAny<T>((const T*)amps, nq, mxy, myz, nt, res));
+        for (int t = 0; t < nt && rc == QA_OK; t++) {
+            const int64_t term = order[g + t];
+            double v[2] = {res[2 * t], res[2 * t + 1]};
+            timesPowerOfI(v, __builtin_popcountll(maskY[term]));
+            vals[2 * term] = v[0];
+            vals[2 * term + 1] = v[1];
+        }
+        g = e;
+    }
+    if (rc != QA_OK) return rc;
+    // sum of coefficient x value, in term order
+    double re = 0, im = 0;
+    for (int64_t t = 0; t < numTerms; t++) {
+        const double cr = coeffs[2 * t], ci = coeffs[2 * t + 1], vr = vals[2 * t], vi = vals[2 * t + 1];
+        re += cr * vr - ci * vi;
+        im += cr * vi + ci * vr;
+    }
+    if (termValues)
+        for (int64_t t = 0; t < 2 * numTerms; t++) termValues[t] = vals[t];
+    out[0] = re;
+    out[1] = im;
+    return QA_OK;
 }
 
 int qa_densmatr_calcExpecPauliStr(const void* amps, int prec, int N, uint64_t maskX, uint64_t maskY,

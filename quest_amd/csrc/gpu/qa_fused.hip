@@ -57,7 +57,7 @@ __device__ __forceinline__ void tilePass(T* __restrict__ amps, const TileArgs& a
         ct<T> rn[RA];
         uint64_t t = blockIdx.x;
         if (t < a.numTiles)
-            RegIO<T, NT, 0>::hbmLoad(rn, amps, deposit(t, a.nonLocalPos, a.numNonLocal) | loadL.tg, loadL);
+            RegIO<T, NT, 0>::hbmLoad(rn, amps, deposit(t, a.nonLocalPos, a.numNonLocal) | loadL.tg, loadL, a.localPos);
         for (; t < a.numTiles; t += gridDim.x) {
             const uint64_t base = deposit(t, a.nonLocalPos, a.numNonLocal);
             ct<T> r[RA];
@@ -65,7 +65,8 @@ __device__ __forceinline__ void tilePass(T* __restrict__ amps, const TileArgs& a
             for (int v = 0; v < RA; v++) r[v] = rn[v];
             const uint64_t tn = t + gridDim.x;
             if (tn < a.numTiles)
-                RegIO<T, NT, 0>::hbmLoad(rn, amps, deposit(tn, a.nonLocalPos, a.numNonLocal) | loadL.tg, loadL);
+                RegIO<T, NT, 0>::hbmLoad(rn, amps, deposit(tn, a.nonLocalPos, a.numNonLocal) | loadL.tg, loadL,
+                                         a.localPos);
             tileBody<T, NT>(amps, a, ops, base, r, tile, row, first, tid);
             // the next tile's first LDS writes must not overtake this tile's reads
             if (a.usesLds) __syncthreads();
@@ -74,7 +75,7 @@ __device__ __forceinline__ void tilePass(T* __restrict__ amps, const TileArgs& a
         for (uint64_t t = blockIdx.x; t < a.numTiles; t += gridDim.x) {
             const uint64_t base = deposit(t, a.nonLocalPos, a.numNonLocal);
             ct<T> r[RA];
-            RegIO<T, NT, 0>::hbmLoad(r, amps, base | loadL.tg, loadL);
+            RegIO<T, NT, 0>::hbmLoad(r, amps, base | loadL.tg, loadL, a.localPos);
             tileBody<T, NT>(amps, a, ops, base, r, tile, row, first, tid);
             if (a.usesLds) __syncthreads();
         }

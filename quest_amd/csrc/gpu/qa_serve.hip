@@ -68,14 +68,14 @@ constexpr int EXP_CHUNK = BLOCK * 16;  // work items per chunk
 
 template <typename T, int V>
 struct RegZero {
-    __device__ __forceinline__ static void run(ct<T>* r, uint64_t at, const RegLayout& L) {
-        r[V] = ((at | offQ<V>(L)) == 0) ? ct<T>{(T)1, (T)0} : ct<T>{(T)0, (T)0};
-        RegZero<T, V + 1>::run(r, at, L);
+    __device__ __forceinline__ static void run(ct<T>* r, uint64_t at, const RegLayout& L, const int* pos) {
+        r[V] = ((at | offQ<V>(L, pos)) == 0) ? ct<T>{(T)1, (T)0} : ct<T>{(T)0, (T)0};
+        RegZero<T, V + 1>::run(r, at, L, pos);
     }
 };
 template <typename T>
 struct RegZero<T, RA> {
-    __device__ __forceinline__ static void run(ct<T>*, uint64_t, const RegLayout&) {}
+    __device__ __forceinline__ static void run(ct<T>*, uint64_t, const RegLayout&, const int*) {}
 };
 
 template <typename T>
@@ -100,8 +100,8 @@ __global__ void __launch_bounds__(512) k_batchPass(const PassDesc* __restrict__ 
         passLayouts(row, first, a, po, tid);
         const RegLayout& L = a.firstDirect ? first : row;
         ct<T> r[RA];
-        if (P.initZero) RegZero<T, 0>::run(r, base | L.tg, L);
-        else RegIO<T, 0, 0>::hbmLoad(r, amps, base | L.tg, L);
+        if (P.initZero) RegZero<T, 0>::run(r, base | L.tg, L, a.localPos);
+        else RegIO<T, 0, 0>::hbmLoad(r, amps, base | L.tg, L, a.localPos);
         tileBody<T, 0>(amps, a, po, base, r, tile, row, first, tid);
         // the next tile's first LDS writes must not overtake this tile's reads
         if (a.usesLds) __syncthreads();

@@ -162,23 +162,24 @@ __device__ __forceinline__ uint32_t swz(uint32_t j) {
     else return j ^ ((j >> 5) & 31u);
 }
 
-// a register layout: tile slots (sb) and global bits (qb) of the register
-// bits, and this thread's tile index (jb) and global offset (tg) with the
-// register bits zero
+// a register layout: the tile slots of the register bits (8 bits each, so
+// a layout costs one scalar register), and this thread's tile index (jb)
+// and global offset (tg) with the register bits zero
 struct RegLayout {
-    uint32_t sb[RB];
-    uint64_t qb[RB];
+    uint32_t slots;
     uint32_t jb;
     uint64_t tg;
 };
 
+__device__ __forceinline__ uint32_t slotAt(const RegLayout& L, int b) { return (L.slots >> (8 * b)) & 0xFFu; }
+
 __device__ __forceinline__ void makeLayout(RegLayout& L, const int* slots, const int* localPos, int k, uint32_t tid) {
     uint32_t gm = 0;
+    L.slots = 0;
 #pragma unroll
     for (int b = 0; b < RB; b++) {
-        L.sb[b] = 1u << slots[b];
-        L.qb[b] = 1ull << localPos[slots[b]];
-        gm |= L.sb[b];
+        L.slots |= (uint32_t)slots[b] << (8 * b);
+        gm |= 1u << slots[b];
     }
     // the thread id deposited over the other slots, ascending
     L.jb = 0;
@@ -197,15 +198,16 @@ __device__ __forceinline__ uint32_t offJ(const RegLayout& L) {
     uint32_t o = 0;
 #pragma unroll
     for (int b = 0; b < RB; b++)
-        if (V & (1 << b)) o |= L.sb[b];
+        if (V & (1 << b)) o |= 1u << slotAt(L, b);
     return o;
 }
+// pos: the pass's tile slot -> global qubit map
 template <int V>
-__device__ __forceinline__ uint64_t offQ(const RegLayout& L) {
+__device__ __forceinline__ uint64_t offQ(const RegLayout& L, const int* pos) {
     uint64_t o = 0;
 #pragma unroll
     for (int b = 0; b < RB; b++)
-        if (V & (1 << b)) o |= L.qb[b];
+        if (V & (1 << b)) o |= 1ull << pos[slotAt(L, b)];
     return o;
 }
 
@@ -264,8 +266,7 @@ struct RegIO {
         RegLayout o = L;
         sj = ldsByte<T>(L.jb);
         asm volatile("" : "+v"(sj));
-#pragma unroll
-        for (int b = 0; b < RB; b++) asm volatile("" : "+s"(o.sb[b]));
+        asm volatile("" : "+s"(o.slots));
         return o;
     }
     __device__ __forceinline__ static void ldsLoad(ct<T>* r, const ct<T>* tile, const RegLayout& L) {
@@ -278,21 +279,23 @@ struct RegIO {
         const RegLayout o = opaque(L, sj);
         ldsStoreAt(r, tile, sj, o);
     }
-    __device__ __forceinline__ static void hbmLoad(ct<T>* r, const T* amps, uint64_t at, const RegLayout& L) {
-        r[V] = ldx<NT>(amps, at | offQ<V>(L));
-        RegIO<T, NT, V + 1>::hbmLoad(r, amps, at, L);
+    __device__ __forceinline__ static void hbmLoad(ct<T>* r, const T* amps, uint64_t at, const RegLayout& L,
+                                                   const int* pos) {
+        r[V] = ldx<NT>(amps, at | offQ<V>(L, pos));
+        RegIO<T, NT, V + 1>::hbmLoad(r, amps, at, L, pos);
     }
-    __device__ __forceinline__ static void hbmStore(const ct<T>* r, T* amps, uint64_t at, const RegLayout& L) {
-        stx<NT>(amps, at | offQ<V>(L), r[V]);
-        RegIO<T, NT, V + 1>::hbmStore(r, amps, at, L);
+    __device__ __forceinline__ static void hbmStore(const ct<T>* r, T* amps, uint64_t at, const RegLayout& L,
+                                                    const int* pos) {
+        stx<NT>(amps, at | offQ<V>(L, pos), r[V]);
+        RegIO<T, NT, V + 1>::hbmStore(r, amps, at, L, pos);
     }
 };
 template <typename T, int NT>
 struct RegIO<T, NT, RA> {
     __device__ __forceinline__ static void ldsLoadAt(ct<T>*, const ct<T>*, uint32_t, const RegLayout&) {}
     __device__ __forceinline__ static void ldsStoreAt(const ct<T>*, ct<T>*, uint32_t, const RegLayout&) {}
-    __device__ __forceinline__ static void hbmLoad(ct<T>*, const T*, uint64_t, const RegLayout&) {}
-    __device__ __forceinline__ static void hbmStore(const ct<T>*, T*, uint64_t, const RegLayout&) {}
+    __device__ __forceinline__ static void hbmLoad(ct<T>*, const T*, uint64_t, const RegLayout&, const int*) {}
+    __device__ __forceinline__ static void hbmStore(const ct<T>*, T*, uint64_t, const RegLayout&, const int*) {}
 };
 
 // register part of an op's control condition; C = false when the op has no
@@ -741,12 +744,12 @@ __device__ __forceinline__ void tileBody(T* __restrict__ amps, const TileArgs& a
         }
     }
     if (a.lastDirect) {
-        RegIO<T, NT, 0>::hbmStore(r, amps, base | cur.tg, cur);
+        RegIO<T, NT, 0>::hbmStore(r, amps, base | cur.tg, cur, a.localPos);
     } else {
         if (inRegs) RegIO<T, NT, 0>::ldsStore(r, tile, cur);
         __syncthreads();
         RegIO<T, NT, 0>::ldsLoad(r, tile, row);
-        RegIO<T, NT, 0>::hbmStore(r, amps, base | row.tg, row);
+        RegIO<T, NT, 0>::hbmStore(r, amps, base | row.tg, row, a.localPos);
     }
 }
 

@@ -318,6 +318,24 @@ def test_fusion_at_26_qubits(api):
         assert abs(np.vdot(got, got).real - 1) < 1e-10
 
 
+def test_benchmark_family_at_28_qubits(api):
+    """Full-size property for the default planner: the headline workload's
+    family (layers of Haar gates, two-qubit phases and 3-control gates, the
+    full QFT, 64 Pauli gadgets on 4-12 qubits, probabilities and Pauli
+    expectations) at 28 qubits with the benchmark's tile shape, default
+    (reordered and merged) against exact fusion (bit-identical to the
+    single-gate kernels): amplitudes within 1e-12 relative L2, results within
+    1e-10, the norm kept."""
+    say("[gpu] fusion 28q benchmark family")
+    c = qc.layered(28, 6, 2024, ctrl3=True, qft=True, gadgets=16, gadget_range=(4, 12), num_expec=4,
+                   probs=True, init_seed=5)
+    c.init = ("plus",)
+    ref, rres = run_amps(api, c, (13, 3, 2048), exact=True)
+    got, gres = run_amps(api, c, (13, 3, 2048), exact=False)
+    check_fused(got, gres, ref, rres, exact=False)
+    assert abs(np.vdot(got, got).real - 1) < 1e-10
+
+
 def test_fusion_config_validation(api):
     from quest_amd.api import QuESTError
     say("[gpu] fusion config validation")

@@ -802,17 +802,20 @@ void fuse::mergePass(const GOp* ops, size_t num, uint64_t localMask, std::vector
     for (const Block& b : open) emit(b);
 }
 
-void fuse::clusterPass(const GOp* ops, size_t num, std::vector<GOp>& out) {
-    std::vector<uint32_t> rem(num), next;
+void fuse::clusterPass(const GOp* ops, size_t num, uint64_t avoid, std::vector<GOp>& out) {
+    std::vector<uint32_t> rem(num), next, taken;
     for (size_t i = 0; i < num; i++) rem[i] = (uint32_t)i;
-    while (!rem.empty()) {
+    // one round over rem: the ops taken (in order) and the rest; `keepOff`
+    // qubits may not be dense targets of the round's ops
+    auto round = [&](uint64_t keepOff) {
         uint64_t R = 0, skipDense = 0, skipTouch = 0;
         bool closed = false;  // an LDS op ended this round's register group
         next.clear();
+        taken.clear();
         for (uint32_t i : rem) {
             const GOp& g = ops[i];
             const uint64_t d = denseMask(g), t = supportMask(g);
-            bool take = (d & skipTouch) == 0 && (t & skipDense) == 0;
+            bool take = (d & skipTouch) == 0 && (t & skipDense) == 0 && (d & keepOff) == 0;
             if (take && d) {
                 if (isLdsOp(g)) {
                     take = !closed && R == 0;
@@ -823,13 +826,22 @@ void fuse::clusterPass(const GOp* ops, size_t num, std::vector<GOp>& out) {
             }
             if (take) {
                 R |= d;
-                out.push_back(g);
+                taken.push_back(i);
             } else {
                 skipDense |= d;
                 skipTouch |= t;
                 next.push_back(i);
             }
         }
+        return R;
+    };
+    bool first = true;
+    while (!rem.empty()) {
+        // the first round keeps off `avoid` when it still applies a dense op,
+        // so the pass can load HBM in its first register layout
+        if (!(first && avoid && round(avoid) != 0)) round(0);
+        first = false;
+        for (uint32_t i : taken) out.push_back(ops[i]);
         rem.swap(next);
     }
 }
@@ -1066,7 +1078,7 @@ int flushQueue(void* amps, Queue& qu) {
         merged.clear();
         mergePass(pass.data(), pass.size(), L, merged);
         pass.clear();
-        clusterPass(merged.data(), merged.size(), pass);
+        clusterPass(merged.data(), merged.size(), (1ull << low) - 1ull, pass);
         rc = launchPass(amps, qu, pass, 0, pass.size(), L, numGates);
         ops.swap(rest);
     }

@@ -400,7 +400,7 @@ bool launchStep(Engine& E, StepBuf& sb, std::string& err) {
             merged.clear();
             mergePass(pass.data(), pass.size(), L, merged);
             pass.clear();
-            clusterPass(merged.data(), merged.size(), pass);
+            clusterPass(merged.data(), merged.size(), (1ull << low) - 1ull, pass);
             planPass(pass.data(), pass.size(), r->n, L, E.lowQ, pd.a, prog, true);
             r->firstPass = false;
             r->ops.swap(rest);

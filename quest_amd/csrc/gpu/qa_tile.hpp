@@ -800,8 +800,11 @@ void mergePass(const GOp* ops, size_t num, uint64_t localMask, std::vector<GOp>&
 // applied together: rounds that each take, in order, every op that commutes
 // with the ops passed over in the round and whose dense targets fit, with
 // those taken so far, in RB qubits (a Pauli op applied in LDS ends a round's
-// dense ops).  Each round then needs one register layout.
-void clusterPass(const GOp* ops, size_t num, std::vector<GOp>& out);
+// dense ops).  Each round then needs one register layout.  The first
+// round leaves the `avoid` qubits (the always-local low qubits) to later
+// rounds when it can still take a dense op without them, so that the pass
+// loads HBM in its first layout instead of through LDS.
+void clusterPass(const GOp* ops, size_t num, uint64_t avoid, std::vector<GOp>& out);
 
 // the program of one pass applying ops[0..num) (dense targets all inside
 // localMask) to a state of numQubits index bits: fills every field of `a`

@@ -307,7 +307,7 @@ int main(int argc, char** argv) {
                 merged.clear();
                 mergePass(pass.data(), pass.size(), L, merged);
                 planned.clear();
-                clusterPass(merged.data(), merged.size(), planned);
+                clusterPass(merged.data(), merged.size(), lowMask, planned);
                 if (planned.size() > pass.size()) {
                     std::printf("case %d: planning added ops (%zu > %zu)\n", c, planned.size(), pass.size());
                     return 1;

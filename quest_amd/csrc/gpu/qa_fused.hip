@@ -191,6 +191,18 @@ unsigned residentBlocks(const void* fn, unsigned block, size_t lds) {
     return r;
 }
 
+bool ldsTileAtZero(const void* fn) {
+    static std::mutex mu;
+    static std::map<const void*, bool> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(fn);
+    if (it != cache.end()) return it->second;
+    hipFuncAttributes attr;
+    const bool ok = hipFuncGetAttributes(&attr, fn) == hipSuccess && attr.sharedSizeBytes == 0;
+    cache[fn] = ok;
+    return ok;
+}
+
 std::vector<TOp> g_prog;  // program of the pass being launched (under g_mu)
 
 // packed MFMA block matrices: staged per pass, uploaded through a ring like
@@ -942,6 +954,34 @@ double opFlops(const GOp& g, int n) {
     return perAmp * std::ldexp(1.0, n - __builtin_popcountll(g.cMask));
 }
 
+void setLdsOffsets(TOp* prog, size_t num, TileArgs& a, int prec) {
+    auto offsets = [&](const int* slots, uint32_t* out) {
+        for (int v = 0; v < RA; v++) {
+            uint32_t j = 0;
+            for (int b = 0; b < RB; b++)
+                if ((v >> b) & 1) j |= 1u << slots[b];
+            out[v] = prec == QA_FP64 ? ldsByte<double>(j) : ldsByte<float>(j);
+        }
+    };
+    int row[RB];
+    for (int b = 0; b < RB; b++) row[b] = a.k - RB + b;
+    offsets(row, a.rowOff);
+    int cur[RB];
+    std::memcpy(cur, groupSlots(prog[0]), sizeof(cur));
+    for (size_t i = 0; i < num; i++) {
+        TOp& t = prog[i];
+        uint32_t* w = reinterpret_cast<uint32_t*>(t.fx);
+        if (t.h.kind == F_GROUP) {
+            offsets(cur, w + RA);
+            std::memcpy(cur, groupSlots(t), sizeof(cur));
+            offsets(cur, w);
+        } else if (t.h.kind == F_PAULI_LDS || t.h.kind == F_BLOCK) {
+            offsets(cur, w + RA);
+        }
+    }
+    offsets(cur, a.lastOff);
+}
+
 // launches one pass applying ops [begin, end) of `ops`, which stand for
 // numGates queued gates
 int launchPass(void* amps, const Queue& qu, const std::vector<GOp>& ops, size_t begin, size_t end, uint64_t localMask,
@@ -983,6 +1023,7 @@ int launchPass(void* amps, const Queue& qu, const std::vector<GOp>& ops, size_t 
     const int k = a.k;
 
     size_t num = prog.size();
+    setLdsOffsets(prog.data(), num, a, qu.prec);
     if (g_ringPos + num > RING_OPS) {
         QA_HIP(hipStreamSynchronize(stream()));
         g_ringPos = 0;
@@ -997,6 +1038,10 @@ int launchPass(void* amps, const Queue& qu, const std::vector<GOp>& ops, size_t 
     const bool nt = std::min(g_cfg.lowQubits, k) >= STREAM_LOW_BIT;
     const bool pf = g_cfg.prefetch;
     const void* fn = tileKernel(qu.prec, nt, pf);
+    if (!ldsTileAtZero(fn)) {
+        setError("fusion: the tile-pass kernel has static LDS; its LDS tile would not start at address 0");
+        return QA_ERR_HIP;
+    }
     const unsigned grid = (unsigned)std::min<uint64_t>(a.numTiles, residentBlocks(fn, block, lds));
     std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
     if (g_stats.timing) {

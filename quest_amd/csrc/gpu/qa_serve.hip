@@ -402,6 +402,7 @@ bool launchStep(Engine& E, StepBuf& sb, std::string& err) {
             pass.clear();
             clusterPass(merged.data(), merged.size(), (1ull << low) - 1ull, pass);
             planPass(pass.data(), pass.size(), r->n, L, E.lowQ, pd.a, prog, true);
+            setLdsOffsets(prog.data() + pd.opsOffset, prog.size() - pd.opsOffset, pd.a, E.prec);
             r->firstPass = false;
             r->ops.swap(rest);
             byK[pd.a.k].push_back(pd);
@@ -468,6 +469,10 @@ bool launchStep(Engine& E, StepBuf& sb, std::string& err) {
         const unsigned block = 1u << (k - RB);
         const size_t ldsBytes = lds ? (E.ampBytes << k) : 0;
         const void* fn = E.prec == QA_FP64 ? passKernel<double>() : passKernel<float>();
+        if (!ldsTileAtZero(fn)) {
+            err = "serve: the pass kernel has static LDS; its LDS tile would not start at address 0";
+            return false;
+        }
         const unsigned grid = (unsigned)std::min<uint64_t>(tiles, residentBlocks(fn, block, ldsBytes));
         const PassDesc* dp = reinterpret_cast<const PassDesc*>(sb.dev) + g.second;
         if (E.prec == QA_FP64)

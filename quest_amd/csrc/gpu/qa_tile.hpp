@@ -61,7 +61,12 @@ static_assert(sizeof(TOpHead) == 64, "op header is one 64-byte scalar load");
 //     parity mask (PARITY) or the maskYZ bits (PAULI) outside the
 //     registers, b[5] = its non-local part
 //   DEPOL1: b[0..2] = c1, c2, c3;  DEPH1: b[0] = factor
-//   GROUP: b[0..1] = the RB tile slots of the register bits (4 ints)
+//   GROUP: b[0..1] = the RB tile slots of the register bits (4 ints);
+//     fx holds, as 32-bit words, the LDS byte offsets of its layout's
+//     registers (words 0..15, loaded after the exchange) and of the layout
+//     before it (words 16..31, stored first)
+//   PAULI_LDS, BLOCK: fx words 16..31 = the LDS byte offsets of the
+//     registers of the layout before them
 struct alignas(64) TOp {
     TOpHead h;
     uint64_t b[8];
@@ -143,6 +148,8 @@ struct TileArgs {
     const float* blocks;    // packed block matrices of the pass's BLOCK ops
     int localPos[MAXQ];     // global positions of the tile slots (ascending)
     int nonLocalPos[MAXQ];  // global positions of the other qubits (ascending)
+    uint32_t rowOff[RA];    // LDS byte offsets of the registers: row layout
+    uint32_t lastOff[RA];   // and the layout of the pass's last group
 };
 
 __device__ __forceinline__ uint64_t deposit(uint64_t v, const int* pos, int num) {
@@ -157,7 +164,7 @@ __device__ __forceinline__ uint64_t deposit(uint64_t v, const int* pos, int num)
 // (complex128: 16 bytes, 16 amplitudes span the 64 four-byte banks; complex64:
 // 8 bytes, so 32 amplitudes do, and 5 bits are XORed with the next 5)
 template <typename T>
-__device__ __forceinline__ uint32_t swz(uint32_t j) {
+__host__ __device__ __forceinline__ uint32_t swz(uint32_t j) {
     if constexpr (sizeof(T) == 8) return j ^ ((j >> 4) & 15u);
     else return j ^ ((j >> 5) & 31u);
 }
@@ -230,54 +237,50 @@ __device__ __forceinline__ void ldsPut(ct<T>* tile, uint32_t i, ct<T> a) {
 // byte offset of tile amplitude j in LDS: the swizzle is linear over XOR,
 // and a register layout's amplitude indices are jb | offJ<V> with disjoint
 // bits, so each thread computes swz(jb) once and every register's offset is
-// one XOR with a uniform constant
+// one XOR with a uniform constant (computed on the host per layout)
 template <typename T>
-__device__ __forceinline__ uint32_t ldsByte(uint32_t j) {
+__host__ __device__ __forceinline__ uint32_t ldsByte(uint32_t j) {
     return swz<T>(j) * (uint32_t)sizeof(typename Amp<T>::V);
 }
+// Register-layout exchanges address LDS by byte offset alone: the tile is
+// the pass kernels' only LDS (dynamic, so at LDS address 0 when the kernel
+// has no static LDS, which the host checks before launching), and an
+// access then needs no add of the array's base
+typedef __attribute__((address_space(3))) unsigned char LdsByte;
 template <typename T>
-__device__ __forceinline__ ct<T> ldsGetB(const ct<T>* tile, uint32_t byteOff) {
-    const typename Amp<T>::V x =
-        *reinterpret_cast<const typename Amp<T>::V*>(reinterpret_cast<const unsigned char*>(tile) + byteOff);
+__device__ __forceinline__ ct<T> ldsGetB(const ct<T>*, uint32_t byteOff) {
+    typedef typename Amp<T>::V V;
+    const V x = *reinterpret_cast<const __attribute__((address_space(3))) V*>((LdsByte*)nullptr + byteOff);
     return ct<T>{x.x, x.y};
 }
 template <typename T>
-__device__ __forceinline__ void ldsPutB(ct<T>* tile, uint32_t byteOff, ct<T> a) {
-    typename Amp<T>::V x;
+__device__ __forceinline__ void ldsPutB(ct<T>*, uint32_t byteOff, ct<T> a) {
+    typedef typename Amp<T>::V V;
+    V x;
     x.x = a.re;
     x.y = a.im;
-    *reinterpret_cast<typename Amp<T>::V*>(reinterpret_cast<unsigned char*>(tile) + byteOff) = x;
+    *reinterpret_cast<__attribute__((address_space(3))) V*>((LdsByte*)nullptr + byteOff) = x;
+}
+
+// the thread's part of a layout's LDS offsets, made opaque at each use so
+// that the per-register offsets are formed there, not hoisted out of the
+// tile loop into sixteen more live registers
+__device__ __forceinline__ uint32_t opaqueV(uint32_t sj) {
+    asm volatile("" : "+v"(sj));
+    return sj;
 }
 
 template <typename T, int NT, int V>
 struct RegIO {
-    __device__ __forceinline__ static void ldsLoadAt(ct<T>* r, const ct<T>* tile, uint32_t sj, const RegLayout& L) {
-        r[V] = ldsGetB(tile, sj ^ ldsByte<T>(offJ<V>(L)));
-        RegIO<T, NT, V + 1>::ldsLoadAt(r, tile, sj, L);
+    // off: the layout's per-register byte offsets (uniform); sj: this
+    // thread's part
+    __device__ __forceinline__ static void ldsLoad(ct<T>* r, const ct<T>* tile, uint32_t sj, const uint32_t* off) {
+        r[V] = ldsGetB(tile, sj ^ off[V]);
+        RegIO<T, NT, V + 1>::ldsLoad(r, tile, sj, off);
     }
-    __device__ __forceinline__ static void ldsStoreAt(const ct<T>* r, ct<T>* tile, uint32_t sj, const RegLayout& L) {
-        ldsPutB(tile, sj ^ ldsByte<T>(offJ<V>(L)), r[V]);
-        RegIO<T, NT, V + 1>::ldsStoreAt(r, tile, sj, L);
-    }
-    // (the thread's base offset and the layout's slots are made opaque so
-    // that the per-register offsets are formed at each use, not hoisted out
-    // of the tile loop into sixteen more live registers)
-    __device__ __forceinline__ static RegLayout opaque(const RegLayout& L, uint32_t& sj) {
-        RegLayout o = L;
-        sj = ldsByte<T>(L.jb);
-        asm volatile("" : "+v"(sj));
-        asm volatile("" : "+s"(o.slots));
-        return o;
-    }
-    __device__ __forceinline__ static void ldsLoad(ct<T>* r, const ct<T>* tile, const RegLayout& L) {
-        uint32_t sj;
-        const RegLayout o = opaque(L, sj);
-        ldsLoadAt(r, tile, sj, o);
-    }
-    __device__ __forceinline__ static void ldsStore(const ct<T>* r, ct<T>* tile, const RegLayout& L) {
-        uint32_t sj;
-        const RegLayout o = opaque(L, sj);
-        ldsStoreAt(r, tile, sj, o);
+    __device__ __forceinline__ static void ldsStore(const ct<T>* r, ct<T>* tile, uint32_t sj, const uint32_t* off) {
+        ldsPutB(tile, sj ^ off[V], r[V]);
+        RegIO<T, NT, V + 1>::ldsStore(r, tile, sj, off);
     }
     __device__ __forceinline__ static void hbmLoad(ct<T>* r, const T* amps, uint64_t at, const RegLayout& L,
                                                    const int* pos) {
@@ -292,8 +295,8 @@ struct RegIO {
 };
 template <typename T, int NT>
 struct RegIO<T, NT, RA> {
-    __device__ __forceinline__ static void ldsLoadAt(ct<T>*, const ct<T>*, uint32_t, const RegLayout&) {}
-    __device__ __forceinline__ static void ldsStoreAt(const ct<T>*, ct<T>*, uint32_t, const RegLayout&) {}
+    __device__ __forceinline__ static void ldsLoad(ct<T>*, const ct<T>*, uint32_t, const uint32_t*) {}
+    __device__ __forceinline__ static void ldsStore(const ct<T>*, ct<T>*, uint32_t, const uint32_t*) {}
     __device__ __forceinline__ static void hbmLoad(ct<T>*, const T*, uint64_t, const RegLayout&, const int*) {}
     __device__ __forceinline__ static void hbmStore(const ct<T>*, T*, uint64_t, const RegLayout&, const int*) {}
 };
@@ -470,33 +473,6 @@ __device__ __forceinline__ void applyOpC(ct<T>* r, const TOp* __restrict__ op, c
                 case 3: regDense1<T, 3, C>(r, h, f); break;
             }
             break;
-        case F_PHASES: {
-            // per-thread factors: A for every amplitude, B[b] for those with
-            // register bit b set; the terms come sorted by that slot, with
-            // the count of each slot in a nibble of rm1
-            const PhaseTerm* pt = reinterpret_cast<const PhaseTerm*>(op->b);
-            ct<T> acc[1 + RB];
-            int k = 0;
-#pragma unroll
-            for (int sl = 0; sl <= RB; sl++) {
-                acc[sl] = ct<T>{(T)1, (T)0};
-                const int ns = (int)((h.rm1 >> (4 * sl)) & 15u);
-                for (int e = 0; e < ns; e++, k++) {
-                    const PhaseTerm t = pt[k];
-                    const bool on = (jb & t.jm) == t.jv && (base & t.gm) == t.gv;
-                    acc[sl] = cmulT(ct<T>{on ? (T)t.fre : (T)1, on ? (T)t.fim : (T)0}, acc[sl]);
-                }
-            }
-#pragma unroll
-            for (int v = 0; v < RA; v++) {
-                ct<T> x = cmulT(acc[0], r[v]);
-#pragma unroll
-                for (int b = 0; b < RB; b++)
-                    if (v & (1 << b)) x = cmulT(acc[1 + b], x);
-                r[v] = x;
-            }
-            break;
-        }
         case F_DENSE1V: {
             // a 2x2 matrix per value of one or two non-local qubits: the
             // tile's value picks it (uniform over the workgroup)
@@ -508,29 +484,6 @@ __device__ __forceinline__ void applyOpC(ct<T>* r, const TOp* __restrict__ op, c
                 case 1: regDense1<T, 1, C>(r, h, fv); break;
                 case 2: regDense1<T, 2, C>(r, h, fv); break;
                 case 3: regDense1<T, 3, C>(r, h, fv); break;
-            }
-            break;
-        }
-        case F_DENSE1N: {
-            // N = rb2 uncontrolled one-qubit matrices on register bits 0..N-1
-            // in that order: straight-line code, one dispatch for N gates
-            const double* fx = op->fx;
-            switch (h.rb2) {
-                case 2:
-                    regDense1<T, 0, false>(r, h, f);
-                    regDense1<T, 1, false>(r, h, fx);
-                    break;
-                case 3:
-                    regDense1<T, 0, false>(r, h, f);
-                    regDense1<T, 1, false>(r, h, fx);
-                    regDense1<T, 2, false>(r, h, fx + 8);
-                    break;
-                case 4:
-                    regDense1<T, 0, false>(r, h, f);
-                    regDense1<T, 1, false>(r, h, fx);
-                    regDense1<T, 2, false>(r, h, fx + 8);
-                    regDense1<T, 3, false>(r, h, fx + 16);
-                    break;
             }
             break;
         }
@@ -546,12 +499,6 @@ __device__ __forceinline__ void applyOpC(ct<T>* r, const TOp* __restrict__ op, c
         }
         case F_SWAP:
             switch (h.rb1 * RB + h.rb2) { QA_PAIR_CASES(regSwap) }
-            break;
-        case F_DEPOL1:
-            switch (h.rb1 * RB + h.rb2) { QA_PAIR_CASES(regDepol1) }
-            break;
-        case F_DEPH1:
-            switch (h.rb1 * RB + h.rb2) { QA_PAIR_CASES(regDeph1) }
             break;
         case F_DIAG1:
             if (h.rb1 < 0) {
@@ -589,14 +536,84 @@ __device__ __forceinline__ void applyOpC(ct<T>* r, const TOp* __restrict__ op, c
     }
 }
 
+// ops that never have controls among the register bits (PHASES, DENSE1N,
+// the channels) are applied here once; the others by applyOpC
 template <typename T>
 __device__ __forceinline__ void applyOp(ct<T>* r, const TOp* __restrict__ op, const OpRegs& o, uint32_t jb,
                                         uint64_t base) {
-    if (o.h.rcMask) applyOpC<T, true>(r, op, o, jb, base);
-    else applyOpC<T, false>(r, op, o, jb, base);
+    const TOpHead& h = o.h;
+    const double* f = o.f;
+    constexpr bool C = false;
+    switch (h.kind) {
+        case F_PHASES: {
+            // per-thread factors: A for every amplitude, B[b] for those with
+            // register bit b set; the terms come sorted by that slot, with
+            // the count of each slot in a nibble of rm1
+            const PhaseTerm* pt = reinterpret_cast<const PhaseTerm*>(op->b);
+            ct<T> acc[1 + RB];
+            int k = 0;
+#pragma unroll
+            for (int sl = 0; sl <= RB; sl++) {
+                acc[sl] = ct<T>{(T)1, (T)0};
+                const int ns = (int)((h.rm1 >> (4 * sl)) & 15u);
+                for (int e = 0; e < ns; e++, k++) {
+                    const PhaseTerm t = pt[k];
+                    const bool on = (jb & t.jm) == t.jv && (base & t.gm) == t.gv;
+                    acc[sl] = cmulT(ct<T>{on ? (T)t.fre : (T)1, on ? (T)t.fim : (T)0}, acc[sl]);
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < RA; v++) {
+                ct<T> x = cmulT(acc[0], r[v]);
+#pragma unroll
+                for (int b = 0; b < RB; b++)
+                    if (v & (1 << b)) x = cmulT(acc[1 + b], x);
+                r[v] = x;
+            }
+            break;
+        }
+        case F_DENSE1N: {
+            // N = rb2 uncontrolled one-qubit matrices on register bits 0..N-1
+            // in that order: straight-line code, one dispatch for N gates
+            const double* fx = op->fx;
+            switch (h.rb2) {
+                case 2:
+                    regDense1<T, 0, false>(r, h, f);
+                    regDense1<T, 1, false>(r, h, fx);
+                    break;
+                case 3:
+                    regDense1<T, 0, false>(r, h, f);
+                    regDense1<T, 1, false>(r, h, fx);
+                    regDense1<T, 2, false>(r, h, fx + 8);
+                    break;
+                case 4:
+                    regDense1<T, 0, false>(r, h, f);
+                    regDense1<T, 1, false>(r, h, fx);
+                    regDense1<T, 2, false>(r, h, fx + 8);
+                    regDense1<T, 3, false>(r, h, fx + 16);
+                    break;
+            }
+            break;
+        }
+        case F_DEPOL1:
+            switch (h.rb1 * RB + h.rb2) { QA_PAIR_CASES(regDepol1) }
+            break;
+        case F_DEPH1:
+            switch (h.rb1 * RB + h.rb2) { QA_PAIR_CASES(regDeph1) }
+            break;
+        default:
+            if (h.rcMask) applyOpC<T, true>(r, op, o, jb, base);
+            else applyOpC<T, false>(r, op, o, jb, base);
+    }
 }
 
-__device__ __forceinline__ const int* groupSlots(const TOp& op) { return reinterpret_cast<const int*>(op.b); }
+__host__ __device__ __forceinline__ const int* groupSlots(const TOp& op) { return reinterpret_cast<const int*>(op.b); }
+__host__ __device__ __forceinline__ const uint32_t* loadOffsets(const TOp* op) {
+    return reinterpret_cast<const uint32_t*>(op->fx);
+}
+__host__ __device__ __forceinline__ const uint32_t* storeOffsets(const TOp* op) {
+    return reinterpret_cast<const uint32_t*>(op->fx) + RA;
+}
 
 // A Pauli tensor / gadget with more X/Y qubits than register bits, applied
 // to the tile in LDS: every thread takes pairs (jA, jA ^ xy) with the top
@@ -705,15 +722,17 @@ __device__ __forceinline__ void tileBody(T* __restrict__ amps, const TileArgs& a
                                          uint64_t base, ct<T>* r, ct<T>* tile, const RegLayout& row,
                                          const RegLayout& first, uint32_t tid) {
     RegLayout cur = first;
+    uint32_t sj = ldsByte<T>(cur.jb);  // this thread's part of cur's LDS offsets
+    const uint32_t sjRow = ldsByte<T>(row.jb);
     bool inRegs = a.firstDirect;  // registers hold `cur` (else the row layout, not yet in LDS)
-    if (!a.firstDirect) RegIO<T, NT, 0>::ldsStore(r, tile, row);
+    if (!a.firstDirect) RegIO<T, NT, 0>::ldsStore(r, tile, opaqueV(sjRow), a.rowOff);
     for (int o = a.firstDirect ? 1 : 0; o < a.numOps; o++) {
         u32x16 ch, cb;
         fetchOp(ops + o, ch, cb);
         OpRegs op;
         unpackOp(ch, cb, op);
         if (op.h.kind == F_PAULI_LDS) {
-            if (inRegs) RegIO<T, NT, 0>::ldsStore(r, tile, cur);
+            if (inRegs) RegIO<T, NT, 0>::ldsStore(r, tile, opaqueV(sj), storeOffsets(ops + o));
             __syncthreads();
             if ((base & op.h.gcMask) == op.h.gcVal)
                 ldsPauli<T>(tile, a.k, op, ((uint64_t)cb[13] << 32) | cb[12], base, tid, blockDim.x);
@@ -721,7 +740,7 @@ __device__ __forceinline__ void tileBody(T* __restrict__ amps, const TileArgs& a
             inRegs = false;
         } else if (op.h.kind == F_BLOCK) {
             if constexpr (sizeof(T) == 4) {
-                if (inRegs) RegIO<T, NT, 0>::ldsStore(r, tile, cur);
+                if (inRegs) RegIO<T, NT, 0>::ldsStore(r, tile, opaqueV(sj), storeOffsets(ops + o));
                 __syncthreads();
                 const int bslots[BQ] = {(int)cb[0], (int)cb[1], (int)cb[2], (int)cb[3], (int)cb[4]};
                 runBlock32(reinterpret_cast<ct<float>*>(tile), a.k, bslots,
@@ -733,10 +752,11 @@ __device__ __forceinline__ void tileBody(T* __restrict__ amps, const TileArgs& a
             int slots[RB];
 #pragma unroll
             for (int b = 0; b < RB; b++) slots[b] = (int)cb[b];
-            if (inRegs) RegIO<T, NT, 0>::ldsStore(r, tile, cur);
+            if (inRegs) RegIO<T, NT, 0>::ldsStore(r, tile, opaqueV(sj), storeOffsets(ops + o));
             __syncthreads();
             makeLayout(cur, slots, a.localPos, a.k, tid);
-            RegIO<T, NT, 0>::ldsLoad(r, tile, cur);
+            sj = ldsByte<T>(cur.jb);
+            RegIO<T, NT, 0>::ldsLoad(r, tile, opaqueV(sj), loadOffsets(ops + o));
             inRegs = true;
         } else if ((base & op.h.gcMask) == op.h.gcVal &&   // uniform over the workgroup
                    (cur.jb & op.h.jcMask) == op.h.jcVal) {  // uniform over the thread's registers
@@ -746,9 +766,9 @@ __device__ __forceinline__ void tileBody(T* __restrict__ amps, const TileArgs& a
     if (a.lastDirect) {
         RegIO<T, NT, 0>::hbmStore(r, amps, base | cur.tg, cur, a.localPos);
     } else {
-        if (inRegs) RegIO<T, NT, 0>::ldsStore(r, tile, cur);
+        if (inRegs) RegIO<T, NT, 0>::ldsStore(r, tile, opaqueV(sj), a.lastOff);
         __syncthreads();
-        RegIO<T, NT, 0>::ldsLoad(r, tile, row);
+        RegIO<T, NT, 0>::ldsLoad(r, tile, opaqueV(sjRow), a.rowOff);
         RegIO<T, NT, 0>::hbmStore(r, amps, base | row.tg, row, a.localPos);
     }
 }
@@ -758,6 +778,10 @@ __device__ __forceinline__ void tileBody(T* __restrict__ amps, const TileArgs& a
 // resident workgroups of a kernel on the whole current device, for
 // persistent grids (cached per kernel, block size and LDS bytes)
 unsigned residentBlocks(const void* fn, unsigned block, size_t lds);
+
+// true when a pass kernel has no static LDS, so that its dynamic LDS tile
+// starts at LDS address 0 (the register-layout exchanges rely on it)
+bool ldsTileAtZero(const void* fn);
 
 // qubits whose amplitude pairs an op mixes (all must be local in its pass;
 // diagonal ops may involve any qubit)
@@ -808,9 +832,9 @@ void clusterPass(const GOp* ops, size_t num, uint64_t avoid, std::vector<GOp>& o
 
 // the program of one pass applying ops[0..num) (dense targets all inside
 // localMask) to a state of numQubits index bits: fills every field of `a`
-// and appends the ops, with their GROUP ops, to `prog`; with `phases`, runs
-// of diagonal gates become PHASES ops (their factors are then multiplied
-// before they reach the amplitudes, which changes rounding)
+// but the LDS offsets and appends the ops, with their GROUP ops, to `prog`;
+// with `phases`, runs of diagonal gates become PHASES ops (their factors are
+// then multiplied before they reach the amplitudes, which changes rounding)
 void planPass(const GOp* ops, size_t num, int numQubits, uint64_t localMask, int lowQubits, TileArgs& a,
               std::vector<TOp>& prog, bool phases = false);
 
@@ -820,6 +844,11 @@ void planPass(const GOp* ops, size_t num, int numQubits, uint64_t localMask, int
 // BLOCK op holds its offset there (a.blocks is left for the caller)
 void planPassBlocks(const GOp* ops, size_t num, int numQubits, uint64_t localMask, int lowQubits, int blockMinGates,
                     TileArgs& a, std::vector<TOp>& prog, std::vector<float>& blocks);
+
+// the LDS byte offsets of the register layouts of a planned pass (prog[0..num),
+// whose first op is a GROUP), for amplitudes of precision prec: the fx words
+// of its GROUP, PAULI_LDS and BLOCK ops, and a.rowOff / a.lastOff
+void setLdsOffsets(TOp* prog, size_t num, TileArgs& a, int prec);
 
 }  // namespace fuse
 }  // namespace qa

@@ -398,6 +398,11 @@ void applyMultiControlledPhaseGadget(Qureg qureg, int* controls, int numControls
                                      qreal angle);
 void applyMultiStateControlledPhaseGadget(Qureg qureg, int* controls, int* states, int numControls,
                                           int* targets, int numTargets, qreal angle);
+/* Pauli string sums (operations.cpp:1112-1197): qureg <- sum * qureg through
+   a workspace of the same shape (on a density matrix, left multiplication),
+   and exp(i angle sum) by a Trotter-Suzuki product of Pauli gadgets */
+void multiplyPauliStrSum(Qureg qureg, PauliStrSum sum, Qureg workspace);
+void applyTrotterizedPauliStrSumGadget(Qureg qureg, PauliStrSum sum, qreal angle, int order, int reps);
 void applyPhaseFlip(Qureg qureg, int target);
 void applyPhaseShift(Qureg qureg, int target, qreal angle);
 void applyTwoQubitPhaseFlip(Qureg qureg, int target1, int target2);

@@ -233,6 +233,47 @@ class State:
         phase *= 1 if bin(my).count("1") % 2 == 1 else -1
         self._pauli_gadget(self._bra(ctrls), mx << self.n, my << self.n, mz << self.n, phase)
 
+    # ---- Pauli string sums; terms are (chars, targets, complex coefficient)
+    def multiply_pauli_sum(self, terms):
+        """operations.cpp:1112-1131: a workspace copy, each string applied to
+        it, added in with its coefficient and undone; strings act on the
+        index bits (left multiplication of a density matrix)."""
+        ws = State(self.n, self.dens)
+        ws.a = self.a.copy()
+        self.a = np.zeros_like(self.a)
+        for chars, targs, coeff in terms:
+            mx, my, mz = masks(chars, targs)
+            ws._pauli_tensor([], mx, my, mz, 1)
+            self.a = self.a + complex(coeff) * ws.a
+            ws._pauli_tensor([], mx, my, mz, 1)
+
+    def _first_order_trotter(self, terms, angle, reverse):
+        """operations.cpp:1133-1142"""
+        order = range(len(terms) - 1, -1, -1) if reverse else range(len(terms))
+        for j in order:
+            chars, targs, coeff = terms[j]
+            self.pauli_gadget([], chars, targs, 2 * angle * complex(coeff).real)
+
+    def trotter(self, terms, angle, order, reps):
+        """operations.cpp:1144-1197.  Orders above 2 follow the reference's
+        code (operations.cpp:1156-1165): five first-order sweeps in reversed
+        term order with angles p, p, 1-4p, p, p (times angle), not the
+        recursive formula its documentation gives."""
+        if angle == 0:
+            return
+        step = angle / reps
+        for _ in range(reps):
+            if order == 1:
+                self._first_order_trotter(terms, step, False)
+            elif order == 2:
+                self._first_order_trotter(terms, step / 2, False)
+                self._first_order_trotter(terms, step / 2, True)
+            else:
+                p = 1.0 / (4 - 4 ** (1.0 / (order - 1)))
+                a, b = p * step, (1 - 4 * p) * step
+                for x in (a, a, b, a, a):
+                    self._first_order_trotter(terms, x, True)
+
     def phase_gadget(self, ctrls, targs, angle):
         phase = -angle / 2
         mask = sum(1 << t for t in targs)

@@ -227,6 +227,8 @@ _SIGS = {
     "applyPauliStr": (None, [Qureg, PauliStr]),
     "applyMultiControlledPauliStr": (None, [Qureg, IntP, cint, PauliStr]),
     "applyPauliGadget": (None, [Qureg, PauliStr, qreal]),
+    "multiplyPauliStrSum": (None, [Qureg, PauliStrSum, Qureg]),
+    "applyTrotterizedPauliStrSumGadget": (None, [Qureg, PauliStrSum, qreal, cint, cint]),
     "applyMultiControlledPauliGadget": (None, [Qureg, IntP, cint, PauliStr, qreal]),
     "applyPhaseGadget": (None, [Qureg, IntP, cint, qreal]),
     "applyQuantumFourierTransform": (None, [Qureg, IntP, cint]),

@@ -620,6 +620,89 @@ void applyMultiControlledPauliGadget(Qureg q, int* c, int nc, PauliStr str, qrea
     applyMultiStateControlledPauliGadget(q, c, nullptr, nc, str, angle);
 }
 
+// ---- Pauli string sums (operations.cpp:1112-1197) ---------------------------------------
+
+namespace {
+
+void validatePauliSum(const Qureg& q, const PauliStrSum& sum) {
+    validateQureg(q);
+    if (sum.numTerms < 1 || !sum.strings || !sum.coeffs) fail("Invalid PauliStrSum.");
+    for (qindex t = 0; t < sum.numTerms; t++)
+        if (pauliMaxIndex(sum.strings[t]) >= q.numQubits)
+            fail("The PauliStrSum targets qubits beyond the " + std::to_string(q.numQubits) + "-qubit Qureg.");
+}
+
+// operations.cpp:1133-1142: the terms in order (or reversed), each as the
+// gadget exp(i angle c_j sigma_j) (gadget angle 2 angle Re c_j)
+void firstOrderTrotter(const Qureg& q, const PauliStrSum& sum, double angle, bool reverse) {
+    for (qindex i = 0; i < sum.numTerms; i++) {
+        const qindex j = reverse ? sum.numTerms - i - 1 : i;
+        pauliGadgetImpl(q, nullptr, nullptr, 0, sum.strings[j], 2 * angle * sum.coeffs[j].real());
+    }
+}
+
+// operations.cpp:1144-1167.  Orders above 2 apply, as the reference does,
+// five first-order sweeps in reversed term order with angles p, p, 1-4p,
+// p, p (times angle), p = 1 / (4 - 4^(1/(order-1)))
+void higherOrderTrotter(const Qureg& q, const PauliStrSum& sum, double angle, int order) {
+    if (order == 1) {
+        firstOrderTrotter(q, sum, angle, false);
+    } else if (order == 2) {
+        firstOrderTrotter(q, sum, angle / 2, false);
+        firstOrderTrotter(q, sum, angle / 2, true);
+    } else {
+        const double p = 1. / (4 - std::pow(4, 1. / (order - 1)));
+        const double a = p * angle, b = (1 - 4 * p) * angle;
+        const bool reverse = (order - 2) != 0;
+        firstOrderTrotter(q, sum, a, reverse);
+        firstOrderTrotter(q, sum, a, reverse);
+        firstOrderTrotter(q, sum, b, reverse);
+        firstOrderTrotter(q, sum, a, reverse);
+        firstOrderTrotter(q, sum, a, reverse);
+    }
+}
+
+}  // namespace
+
+// operations.cpp:1112-1131: the workspace takes a copy of qureg, qureg is
+// zeroed, and each term is applied to the workspace, added into qureg with
+// its coefficient and undone (Pauli strings are their own inverses); the
+// strings act on the state's index bits, so on a density matrix this is
+// left multiplication
+void multiplyPauliStrSum(Qureg q, PauliStrSum sum, Qureg ws) {
+    QAH_API({
+        validatePauliSum(q, sum);
+        validateQureg(ws);
+        if (ws.numQubits != q.numQubits || ws.isDensityMatrix != q.isDensityMatrix || precOf(ws) != precOf(q))
+            fail("The workspace Qureg is incompatible with the Qureg (its qubits, type or precision differ).");
+        if (dev(ws) == dev(q)) fail("The workspace must be a different Qureg.");
+        const int p = precOf(q), nb = idxBits(q);
+        check(qa_statevec_setQuregToSuperposition(p, nb, 0, 0, dev(ws), 1, 0, dev(q), 0, 0, dev(q)));
+        check(qa_statevec_initUniformState(dev(q), p, nb, 0, 0));
+        for (qindex t = 0; t < sum.numTerms; t++) {
+            ctrlPauliTensor(ws, {}, {}, sum.strings[t], 1);
+            check(qa_statevec_setQuregToSuperposition(p, nb, 1, 0, dev(q), sum.coeffs[t].real(), sum.coeffs[t].imag(),
+                                                      dev(ws), 0, 0, dev(ws)));
+            ctrlPauliTensor(ws, {}, {}, sum.strings[t], 1);
+        }
+    });
+}
+
+// operations.cpp:1169-1197
+void applyTrotterizedPauliStrSumGadget(Qureg q, PauliStrSum sum, qreal angle, int order, int reps) {
+    QAH_API({
+        validatePauliSum(q, sum);
+        if (validating())
+            for (qindex t = 0; t < sum.numTerms; t++)
+                if (std::abs(sum.coeffs[t].imag()) > epsilon()) fail("The PauliStrSum is not Hermitian.");
+        if (order <= 0 || (order % 2 != 0 && order != 1))
+            fail("The Trotter order (" + std::to_string(order) + ") must be 1, or positive and even.");
+        if (reps <= 0) fail("The number of Trotter repetitions (" + std::to_string(reps) + ") must be positive.");
+        if (angle == 0) return;
+        for (int r = 0; r < reps; r++) higherOrderTrotter(q, sum, angle / reps, order);
+    });
+}
+
 void multiplyPhaseGadget(Qureg q, int* t, int nt, qreal angle) {
     QAH_API({
         validateTargets(q, t, nt);

@@ -109,3 +109,101 @@ def test_grouped_terms_equal_single_string_reductions(api, prec):
         assert abs(complex(out[0], out[1]) - total) <= 1e-12 * max(1.0, abs(total))
     finally:
         api.destroyQureg(q)
+
+
+def sum_terms(n: int, seed: int, complex_coeffs: bool):
+    """Strings of 1..n random Paulis with random coefficients."""
+    rng = np.random.default_rng(seed)
+    terms = []
+    for _ in range(9):
+        k = int(rng.integers(1, n + 1))
+        targs = [int(t) for t in rng.choice(n, size=k, replace=False)]
+        chars = "".join(rng.choice(list("XYZ"), size=k))
+        c = rng.normal() + (1j * rng.normal() if complex_coeffs else 0)
+        terms.append((chars, targs, complex(c)))
+    return terms
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.parametrize("dens,prec,tol", [(False, 2, 1e-12), (True, 2, 1e-12), (False, 1, 1e-5)])
+@pytest.mark.parametrize("order,reps", [(1, 2), (2, 1), (4, 1), (6, 1)])
+def test_trotterized_gadget_matches_oracle(api, dens, prec, tol, order, reps):
+    """applyTrotterizedPauliStrSumGadget (operations.cpp:1169-1197) against
+    the oracle's restatement, which is pinned to products of matrix
+    exponentials on the CPU; the gadgets run through the fused tile passes."""
+    from oracle import qoracle as O
+    n = 5 if dens else 11
+    say(f"[gpu] trotter dens={dens} prec={prec} order={order} reps={reps}")
+    terms = sum_terms(n, 40 + order, False)
+    init = qc.random_state(n, 9, dens)
+    st = O.State(n, dens)
+    st.a[:] = init
+    st.trotter(terms, 0.41, order, reps)
+    q = api.createCustomPrecisionQureg(n, int(dens), prec)
+    s = make_sum(api, terms, [c for _, _, c in terms])
+    try:
+        api.set_amps(q, init)
+        api.applyTrotterizedPauliStrSumGadget(q, s, 0.41, order, reps)
+        got = api.get_amps(q)
+    finally:
+        api.destroyPauliStrSum(s)
+        api.destroyQureg(q)
+    assert rel_l2(got, st.a) <= tol
+
+
+@pytest.mark.parametrize("dens,prec,tol", [(False, 2, 1e-12), (True, 2, 1e-12), (False, 1, 1e-5)])
+def test_multiply_pauli_sum_matches_oracle(api, dens, prec, tol):
+    """multiplyPauliStrSum (operations.cpp:1112-1131): qureg <- H qureg with
+    complex coefficients, left multiplication on a density matrix."""
+    from oracle import qoracle as O
+    n = 5 if dens else 12
+    say(f"[gpu] multiply pauli sum dens={dens} prec={prec}")
+    terms = sum_terms(n, 77, True)
+    init = qc.random_state(n, 5, dens)
+    st = O.State(n, dens)
+    st.a[:] = init
+    st.multiply_pauli_sum(terms)
+    q = api.createCustomPrecisionQureg(n, int(dens), prec)
+    ws = api.createCustomPrecisionQureg(n, int(dens), prec)
+    s = make_sum(api, terms, [c for _, _, c in terms])
+    try:
+        api.set_amps(q, init)
+        api.multiplyPauliStrSum(q, s, ws)
+        got = api.get_amps(q)
+    finally:
+        api.destroyPauliStrSum(s)
+        api.destroyQureg(ws)
+        api.destroyQureg(q)
+    assert rel_l2(got, st.a) <= tol
+
+
+def test_pauli_sum_operation_errors(api):
+    """validation.cpp:3706-3711 (Trotter order 1 or even, reps > 0),
+    3272-3280 (Hermitian sum), 3800-3808 (compatible workspace)."""
+    say("[gpu] pauli sum validation")
+    q = api.createQureg(4)
+    w = api.createQureg(5)
+    herm = make_sum(api, [("XZ", [0, 2], 0.5), ("Y", [1], -0.25)])
+    nonh = make_sum(api, [("XZ", [0, 2], 0.5)], [0.5 + 0.5j])
+    wide = make_sum(api, [("X", [6], 1.0)])
+    try:
+        with pytest.raises(api.QuESTError, match="order"):
+            api.applyTrotterizedPauliStrSumGadget(q, herm, 0.3, 3, 1)
+        with pytest.raises(api.QuESTError, match="order"):
+            api.applyTrotterizedPauliStrSumGadget(q, herm, 0.3, 0, 1)
+        with pytest.raises(api.QuESTError, match="repetitions"):
+            api.applyTrotterizedPauliStrSumGadget(q, herm, 0.3, 2, 0)
+        with pytest.raises(api.QuESTError, match="Hermitian"):
+            api.applyTrotterizedPauliStrSumGadget(q, nonh, 0.3, 2, 1)
+        with pytest.raises(api.QuESTError, match="beyond"):
+            api.applyTrotterizedPauliStrSumGadget(q, wide, 0.3, 2, 1)
+        with pytest.raises(api.QuESTError, match="workspace"):
+            api.multiplyPauliStrSum(q, herm, w)
+    finally:
+        for s in (herm, nonh, wide):
+            api.destroyPauliStrSum(s)
+        api.destroyQureg(w)
+        api.destroyQureg(q)

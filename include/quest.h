@@ -325,8 +325,22 @@ void applyMultiControlledDiagMatr(Qureg qureg, int* controls, int numControls, i
                                   DiagMatr matrix);
 void applyMultiStateControlledDiagMatr(Qureg qureg, int* controls, int* states, int numControls, int* targets,
                                        int numTargets, DiagMatr matrix);
+/* DiagMatr powers (operations.cpp:405-500): elements raised to a complex
+   exponent (pow(qcomp, qcomp)) before use; the apply forms require a
+   unitary matrix and an (approximately) real exponent */
+void multiplyDiagMatrPower(Qureg qureg, int* targets, int numTargets, DiagMatr matrix, qcomp exponent);
+void applyDiagMatrPower(Qureg qureg, int* targets, int numTargets, DiagMatr matrix, qcomp exponent);
+void applyControlledDiagMatrPower(Qureg qureg, int control, int* targets, int numTargets, DiagMatr matrix,
+                                  qcomp exponent);
+void applyMultiControlledDiagMatrPower(Qureg qureg, int* controls, int numControls, int* targets, int numTargets,
+                                       DiagMatr matrix, qcomp exponent);
+void applyMultiStateControlledDiagMatrPower(Qureg qureg, int* controls, int* states, int numControls, int* targets,
+                                            int numTargets, DiagMatr matrix, qcomp exponent);
 void multiplyFullStateDiagMatr(Qureg qureg, FullStateDiagMatr matrix);
 void applyFullStateDiagMatr(Qureg qureg, FullStateDiagMatr matrix);
+/* operations.cpp:545-576 */
+void multiplyFullStateDiagMatrPower(Qureg qureg, FullStateDiagMatr matrix, qcomp exponent);
+void applyFullStateDiagMatrPower(Qureg qureg, FullStateDiagMatr matrix, qcomp exponent);
 void applyS(Qureg qureg, int target);
 void applyControlledS(Qureg qureg, int control, int target);
 void applyMultiControlledS(Qureg qureg, int* controls, int numControls, int target);
@@ -346,6 +360,11 @@ void applyMultiControlledSwap(Qureg qureg, int* controls, int numControls, int q
 void applyMultiStateControlledSwap(Qureg qureg, int* controls, int* states, int numControls, int qubit1,
                                    int qubit2);
 void applySqrtSwap(Qureg qureg, int qubit1, int qubit2);
+/* operations.cpp:825-860 */
+void applyControlledSqrtSwap(Qureg qureg, int control, int qubit1, int qubit2);
+void applyMultiControlledSqrtSwap(Qureg qureg, int* controls, int numControls, int qubit1, int qubit2);
+void applyMultiStateControlledSqrtSwap(Qureg qureg, int* controls, int* states, int numControls, int qubit1,
+                                       int qubit2);
 void applyPauliX(Qureg qureg, int target);
 void applyPauliY(Qureg qureg, int target);
 void applyPauliZ(Qureg qureg, int target);
@@ -455,6 +474,9 @@ Qureg calcReducedDensityMatrix(Qureg qureg, int* retainQubits, int numRetainQubi
 void _wrap_calcInnerProduct(Qureg bra, Qureg ket, qcomp* out);
 void _wrap_calcExpecNonHermitianPauliStrSum(qcomp* out, Qureg qureg, PauliStrSum sum);
 void _wrap_calcExpecNonHermitianFullStateDiagMatr(qcomp* out, Qureg qureg, FullStateDiagMatr matr);
+/* calculations.cpp:98-115 */
+void _wrap_calcExpecNonHermitianFullStateDiagMatrPower(qcomp* out, Qureg qureg, FullStateDiagMatr matr,
+                                                       qcomp exponent);
 
 #ifdef __cplusplus
 }  /* extern "C" */
@@ -467,6 +489,7 @@ qcomp getDensityQuregAmp(Qureg qureg, qindex row, qindex column);
 qcomp calcInnerProduct(Qureg qureg1, Qureg qureg2);
 qcomp calcExpecNonHermitianPauliStrSum(Qureg qureg, PauliStrSum sum);
 qcomp calcExpecNonHermitianFullStateDiagMatr(Qureg qureg, FullStateDiagMatr matr);
+qcomp calcExpecNonHermitianFullStateDiagMatrPower(Qureg qureg, FullStateDiagMatr matr, qcomp exponent);
 #else
 static inline qcomp getQuregAmp(Qureg qureg, qindex index) {
     qcomp out;
@@ -491,6 +514,12 @@ static inline qcomp calcExpecNonHermitianPauliStrSum(Qureg qureg, PauliStrSum su
 static inline qcomp calcExpecNonHermitianFullStateDiagMatr(Qureg qureg, FullStateDiagMatr matr) {
     qcomp out;
     _wrap_calcExpecNonHermitianFullStateDiagMatr(&out, qureg, matr);
+    return out;
+}
+static inline qcomp calcExpecNonHermitianFullStateDiagMatrPower(Qureg qureg, FullStateDiagMatr matr,
+                                                               qcomp exponent) {
+    qcomp out;
+    _wrap_calcExpecNonHermitianFullStateDiagMatrPower(&out, qureg, matr, exponent);
     return out;
 }
 #endif

@@ -171,14 +171,19 @@ class State:
         if self.dens:
             apply(self._bra(ctrls), self._bra(targs), True)
 
-    def diag_n(self, ctrls, targs, d):
-        """applyDiagMatr: cpu_statevec_anyCtrlAnyTargDiagMatr_sub, conj on the bra pass"""
+    def diag_n(self, ctrls, targs, d, power=None, states=None, multiply=False):
+        """applyDiagMatr / applyDiagMatrPower (operations.cpp:468-500):
+        cpu_statevec_anyCtrlAnyTargDiagMatr_sub with the optional complex
+        power, conj (after the power) on the bra pass; multiply = ket only"""
         d = _cmat(d).reshape(-1)
-        lib().qo_diagN(_p(self.a), self.bits, _ip(ctrls), None, len(ctrls), _ip(targs), len(targs), _p(d), 0, 0,
-                       0.0, 0.0)
-        if self.dens:
-            lib().qo_diagN(_p(self.a), self.bits, _ip(self._bra(ctrls)), None, len(ctrls), _ip(self._bra(targs)),
-                           len(targs), _p(d), 1, 0, 0.0, 0.0)
+        hp = 0 if power is None else 1
+        e = complex(0 if power is None else power)
+        s = list(states) if states is not None else None
+        lib().qo_diagN(_p(self.a), self.bits, _ip(ctrls), _ip(s) if s else None, len(ctrls), _ip(targs), len(targs),
+                       _p(d), 0, hp, e.real, e.imag)
+        if self.dens and not multiply:
+            lib().qo_diagN(_p(self.a), self.bits, _ip(self._bra(ctrls)), _ip(s) if s else None, len(ctrls),
+                           _ip(self._bra(targs)), len(targs), _p(d), 1, hp, e.real, e.imag)
 
     def swap(self, ctrls, a, b):
         if a > b:

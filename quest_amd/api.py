@@ -213,6 +213,18 @@ _SIGS = {
     "applyDiagMatr2": (None, [Qureg, cint, cint, DiagMatr2]),
     "applyDiagMatr": (None, [Qureg, IntP, cint, DiagMatr]),
     "applyFullStateDiagMatr": (None, [Qureg, FullStateDiagMatr]),
+    "multiplyFullStateDiagMatr": (None, [Qureg, FullStateDiagMatr]),
+    "multiplyDiagMatrPower": (None, [Qureg, IntP, cint, DiagMatr, qcomp]),
+    "applyDiagMatrPower": (None, [Qureg, IntP, cint, DiagMatr, qcomp]),
+    "applyControlledDiagMatrPower": (None, [Qureg, cint, IntP, cint, DiagMatr, qcomp]),
+    "applyMultiControlledDiagMatrPower": (None, [Qureg, IntP, cint, IntP, cint, DiagMatr, qcomp]),
+    "applyMultiStateControlledDiagMatrPower": (None, [Qureg, IntP, IntP, cint, IntP, cint, DiagMatr, qcomp]),
+    "multiplyFullStateDiagMatrPower": (None, [Qureg, FullStateDiagMatr, qcomp]),
+    "applyFullStateDiagMatrPower": (None, [Qureg, FullStateDiagMatr, qcomp]),
+    "applySqrtSwap": (None, [Qureg, cint, cint]),
+    "applyControlledSqrtSwap": (None, [Qureg, cint, cint, cint]),
+    "applyMultiControlledSqrtSwap": (None, [Qureg, IntP, cint, cint, cint]),
+    "applyMultiStateControlledSqrtSwap": (None, [Qureg, IntP, IntP, cint, cint, cint]),
     "applyHadamard": (None, [Qureg, cint]), "applyPauliX": (None, [Qureg, cint]),
     "applyPauliY": (None, [Qureg, cint]), "applyPauliZ": (None, [Qureg, cint]),
     "applyS": (None, [Qureg, cint]), "applyT": (None, [Qureg, cint]),
@@ -252,6 +264,10 @@ _SIGS = {
     "calcTotalProb": (qreal, [Qureg]), "calcPurity": (qreal, [Qureg]),
     "calcFidelity": (qreal, [Qureg, Qureg]), "calcDistance": (qreal, [Qureg, Qureg]),
     "_wrap_calcInnerProduct": (None, [Qureg, Qureg, C.POINTER(qcomp)]),
+    "calcExpecFullStateDiagMatr": (qreal, [Qureg, FullStateDiagMatr]),
+    "calcExpecFullStateDiagMatrPower": (qreal, [Qureg, FullStateDiagMatr, qreal]),
+    "_wrap_calcExpecNonHermitianFullStateDiagMatr": (None, [C.POINTER(qcomp), Qureg, FullStateDiagMatr]),
+    "_wrap_calcExpecNonHermitianFullStateDiagMatrPower": (None, [C.POINTER(qcomp), Qureg, FullStateDiagMatr, qcomp]),
     "calcPartialTrace": (Qureg, [Qureg, IntP, cint]),
     "getQuESTStream": (ptr, []),
     "setQuESTFusion": (None, [cint, cint, cint, cint]),
@@ -397,6 +413,24 @@ def inner_product(a: Qureg, b: Qureg) -> complex:
     out = qcomp()
     _err.last = None
     _raw["_wrap_calcInnerProduct"](a, b, C.byref(out))
+    _raise_pending()
+    return out.value()
+
+
+def make_fullstate_diag(n: int, d) -> FullStateDiagMatr:
+    d = np.ascontiguousarray(np.asarray(d, dtype=np.complex128))
+    m = createFullStateDiagMatr(n)
+    setFullStateDiagMatr(m, 0, C.cast(d.ctypes.data, C.POINTER(qcomp)), d.size)
+    return m
+
+
+def expec_nonhermitian_fullstate_diag(q: Qureg, m: FullStateDiagMatr, exponent=None) -> complex:
+    out = qcomp()
+    _err.last = None
+    if exponent is None:
+        _raw["_wrap_calcExpecNonHermitianFullStateDiagMatr"](C.byref(out), q, m)
+    else:
+        _raw["_wrap_calcExpecNonHermitianFullStateDiagMatrPower"](C.byref(out), q, m, qcomp.of(exponent))
     _raise_pending()
     return out.value()
 

@@ -245,6 +245,23 @@ void _wrap_calcExpecNonHermitianFullStateDiagMatr(qcomp* out, Qureg q, FullState
     });
 }
 
+// calculations.cpp:98-115: sum of d_i^expo |a_i|^2 (rho_ii on a density
+// matrix) with the complex power
+void _wrap_calcExpecNonHermitianFullStateDiagMatrPower(qcomp* out, Qureg q, FullStateDiagMatr m, qcomp expo) {
+    QAH_API({
+        validateQureg(q);
+        if (m.numQubits != q.numQubits) fail("The FullStateDiagMatr and Qureg have different numbers of qubits.");
+        double v[2];
+        if (q.isDensityMatrix)
+            check(qa_densmatr_calcExpecFullStateDiagMatr(dev(q), precOf(q), q.numQubits, m.gpuElems, 1, expo.real(),
+                                                         expo.imag(), v));
+        else
+            check(qa_statevec_calcExpecFullStateDiagMatr(dev(q), precOf(q), q.numQubits, m.gpuElems, 1, expo.real(),
+                                                         expo.imag(), v));
+        *out = qcomp(v[0], v[1]);
+    });
+}
+
 qreal calcExpecFullStateDiagMatr(Qureg q, FullStateDiagMatr m) {
     qcomp v = 0;
     _wrap_calcExpecNonHermitianFullStateDiagMatr(&v, q, m);
@@ -423,6 +440,11 @@ qcomp calcInnerProduct(Qureg a, Qureg b) {
 qcomp calcExpecNonHermitianPauliStrSum(Qureg q, PauliStrSum sum) {
     qcomp out = 0;
     _wrap_calcExpecNonHermitianPauliStrSum(&out, q, sum);
+    return out;
+}
+qcomp calcExpecNonHermitianFullStateDiagMatrPower(Qureg q, FullStateDiagMatr m, qcomp expo) {
+    qcomp out = 0;
+    _wrap_calcExpecNonHermitianFullStateDiagMatrPower(&out, q, m, expo);
     return out;
 }
 qcomp calcExpecNonHermitianFullStateDiagMatr(Qureg q, FullStateDiagMatr m) {

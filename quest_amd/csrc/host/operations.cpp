@@ -409,6 +409,107 @@ void applyMultiStateControlledDiagMatr(Qureg q, int* c, int* s, int nc, int* t, 
     QAH_API({ applyUnitary(q, c, s, nc, t, nt, m); });
 }
 
+// ---- DiagMatr powers (operations.cpp:405-500, 545-576) --------------------------------
+
+namespace {
+
+// validation.cpp:2408-2431: a strictly real, negative exponent needs every
+// element of (approximately) non-zero magnitude
+void requireNonDiverging(const qcomp* elems, qindex num, qcomp expo) {
+    if (!validating() || expo.imag() != 0 || expo.real() >= 0) return;
+    for (qindex i = 0; i < num; i++)
+        if (std::abs(elems[i]) <= epsilon())
+            fail("The diagonal matrix has an element of (approximately) zero magnitude, which diverges under the "
+                 "real, negative exponent.");
+}
+
+// validation.cpp:2367-2379
+void requireRealExponent(qcomp expo) {
+    if (validating() && std::abs(expo.imag()) > epsilon())
+        fail("The exponent of a unitary diagonal matrix must be (approximately) real.");
+}
+
+void requireSynced(const DiagMatr& m) {
+    if (!*m.wasGpuSynced) fail("The DiagMatr has not been synced to the GPU.");
+}
+
+void requireFullState(const Qureg& q, const FullStateDiagMatr& m) {
+    validateQureg(q);
+    if (m.numQubits != q.numQubits) fail("The FullStateDiagMatr and Qureg have different numbers of qubits.");
+    if (!*m.wasGpuSynced) fail("The FullStateDiagMatr has not been synced to the GPU.");
+}
+
+void requireUnitaryFullState(FullStateDiagMatr& m) {
+    if (!validating()) return;
+    if (*m.isApproxUnitary == -1) *m.isApproxUnitary = isUnitaryDiag(m.cpuElems, m.numElems) ? 1 : 0;
+    if (!*m.isApproxUnitary) fail("The given matrix was not (approximately) unitary.");
+}
+
+}  // namespace
+
+void multiplyDiagMatrPower(Qureg q, int* t, int nt, DiagMatr m, qcomp expo) {
+    QAH_API({
+        validateTargets(q, t, nt);
+        if (m.numQubits != nt) fail("The matrix size does not match the number of target qubits.");
+        requireSynced(m);
+        requireNonDiverging(m.cpuElems, m.numElems, expo);
+        applyDiagPow(q, {}, {}, vec(t, nt), m, expo, true, false);
+    });
+}
+// d^expo on the kets, conj(d^expo) on the bras of a density matrix
+void applyMultiStateControlledDiagMatrPower(Qureg q, int* c, int* s, int nc, int* t, int nt, DiagMatr m,
+                                            qcomp expo) {
+    QAH_API({
+        validateCtrlsAndTargs(q, c, nc, t, nt);
+        validateCtrlStates(s, nc);
+        if (m.numQubits != nt) fail("The matrix size does not match the number of target qubits.");
+        requireSynced(m);
+        requireUnitary(m);
+        requireNonDiverging(m.cpuElems, m.numElems, expo);
+        requireRealExponent(expo);
+        auto cv = vec(c, nc), sv = s ? vec(s, nc) : vector<int>(), tv = vec(t, nt);
+        applyDiagPow(q, cv, sv, tv, m, expo, true, false);
+        if (!q.isDensityMatrix) return;
+        applyDiagPow(q, braOf(cv, q), sv, braOf(tv, q), m, expo, true, true);
+    });
+}
+void applyDiagMatrPower(Qureg q, int* t, int nt, DiagMatr m, qcomp expo) {
+    applyMultiStateControlledDiagMatrPower(q, nullptr, nullptr, 0, t, nt, m, expo);
+}
+void applyControlledDiagMatrPower(Qureg q, int c, int* t, int nt, DiagMatr m, qcomp expo) {
+    applyMultiStateControlledDiagMatrPower(q, &c, nullptr, 1, t, nt, m, expo);
+}
+void applyMultiControlledDiagMatrPower(Qureg q, int* c, int nc, int* t, int nt, DiagMatr m, qcomp expo) {
+    applyMultiStateControlledDiagMatrPower(q, c, nullptr, nc, t, nt, m, expo);
+}
+
+void multiplyFullStateDiagMatrPower(Qureg q, FullStateDiagMatr m, qcomp expo) {
+    QAH_API({
+        requireFullState(q, m);
+        requireNonDiverging(m.cpuElems, m.numElems, expo);
+        if (q.isDensityMatrix)
+            check(qa_densmatr_allTargDiagMatr(dev(q), precOf(q), q.numQubits, m.gpuElems, 1, expo.real(), expo.imag(),
+                                              1));
+        else
+            check(qa_statevec_allTargDiagMatr(dev(q), precOf(q), q.numQubits, m.gpuElems, 1, expo.real(),
+                                              expo.imag()));
+    });
+}
+void applyFullStateDiagMatrPower(Qureg q, FullStateDiagMatr m, qcomp expo) {
+    QAH_API({
+        requireFullState(q, m);
+        requireUnitaryFullState(m);
+        requireRealExponent(expo);
+        requireNonDiverging(m.cpuElems, m.numElems, expo);
+        if (q.isDensityMatrix)
+            check(qa_densmatr_allTargDiagMatr(dev(q), precOf(q), q.numQubits, m.gpuElems, 1, expo.real(), expo.imag(),
+                                              0));
+        else
+            check(qa_statevec_allTargDiagMatr(dev(q), precOf(q), q.numQubits, m.gpuElems, 1, expo.real(),
+                                              expo.imag()));
+    });
+}
+
 void multiplyFullStateDiagMatr(Qureg q, FullStateDiagMatr m) {
     QAH_API({
         validateQureg(q);
@@ -477,7 +578,8 @@ void applySwap(Qureg q, int a, int b) { applyMultiStateControlledSwap(q, nullptr
 void applyControlledSwap(Qureg q, int c, int a, int b) { applyMultiStateControlledSwap(q, &c, nullptr, 1, a, b); }
 void applyMultiControlledSwap(Qureg q, int* c, int nc, int a, int b) { applyMultiStateControlledSwap(q, c, nullptr, nc, a, b); }
 
-void applySqrtSwap(Qureg q, int a, int b) {
+// operations.cpp:841-860: the dense two-qubit root of SWAP
+void applyMultiStateControlledSqrtSwap(Qureg q, int* c, int* s, int nc, int a, int b) {
     QAH_API({
         CompMatr2 m;
         m.numQubits = 2;
@@ -488,8 +590,15 @@ void applySqrtSwap(Qureg q, int a, int b) {
         m.elems[1][1] = m.elems[2][2] = qcomp(.5, .5);
         m.elems[1][2] = m.elems[2][1] = qcomp(.5, -.5);
         int t[2] = {a, b};
-        applyUnitary(q, nullptr, nullptr, 0, t, 2, m);
+        applyUnitary(q, c, s, nc, t, 2, m);
     });
+}
+void applyControlledSqrtSwap(Qureg q, int c, int a, int b) { applyMultiStateControlledSqrtSwap(q, &c, nullptr, 1, a, b); }
+void applyMultiControlledSqrtSwap(Qureg q, int* c, int nc, int a, int b) {
+    applyMultiStateControlledSqrtSwap(q, c, nullptr, nc, a, b);
+}
+void applySqrtSwap(Qureg q, int a, int b) {
+    QAH_API({ applyMultiStateControlledSqrtSwap(q, nullptr, nullptr, 0, a, b); });
 }
 
 // ---- Paulis -------------------------------------------------------------------------

@@ -1,0 +1,216 @@
+"""Diagonal operators through quest.h on the GPU.
+
+  - DiagMatr powers (operations.cpp:405-500): against the oracle's powered
+    diagonal (pinned to full operators in tests/test_oracle_diag.py), with
+    controls and control states, on statevectors and density matrices, for
+    real and complex exponents, apply and multiply forms;
+  - FullStateDiagMatr apply / multiply / expectation, with and without
+    powers (operations.cpp:536-576, calculations.cpp:85-190): against closed
+    forms in numpy (a_i d_i^p; rho_rc d_r^p conj(d_c^p), or d_r^p rho_rc when
+    multiplying; sum_i d_i^p |a_i|^2 and sum_i d_i^p rho_ii);
+  - controlled square-root-of-SWAP (operations.cpp:825-860) against the
+    oracle's dense two-qubit unitary;
+  - the validation of unitarity, real exponents and divergent powers.
+Tolerances: 1e-12 relative L2 on amplitudes, 1e-10 on scalars (complex128).
+"""
+
+import sys
+
+import numpy as np
+import pytest
+
+from quest_amd import circuits as qc
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def phases(k, seed):
+    rng = np.random.default_rng(seed)
+    return np.exp(1j * rng.uniform(0, 2 * np.pi, size=1 << k))
+
+
+@pytest.mark.parametrize("dens", [False, True])
+@pytest.mark.parametrize("power", [0.5, -1.25, 3.0])
+@pytest.mark.parametrize("nctrl", [0, 1, 2])
+def test_apply_diag_matr_power_matches_oracle(api, dens, power, nctrl):
+    from oracle import qoracle as O
+    n = 5 if dens else 11
+    say(f"[gpu] diag power dens={dens} p={power} nctrl={nctrl}")
+    targs = [3, 0, 4][: 1 + (nctrl % 2) + 1]
+    ctrls = [1, 2][:nctrl]
+    states = [0, 1][:nctrl]
+    d = phases(len(targs), 3 + nctrl)
+    init = qc.random_state(n, 8, dens)
+    st = O.State(n, dens)
+    st.a[:] = init
+    st.diag_n(ctrls, targs, d, power=power, states=states)
+    q = api.createQureg(n) if not dens else api.createDensityQureg(n)
+    m = api.make_diagmatr(d)
+    try:
+        api.set_amps(q, init)
+        e = api.qcomp.of(power)
+        if nctrl == 0:
+            api.applyDiagMatrPower(q, api.ints(targs), len(targs), m, e)
+        else:
+            api.applyMultiStateControlledDiagMatrPower(q, api.ints(ctrls), api.ints(states), nctrl, api.ints(targs),
+                                                       len(targs), m, e)
+        got = api.get_amps(q)
+    finally:
+        api.destroyDiagMatr(m)
+        api.destroyQureg(q)
+    assert rel_l2(got, st.a) <= 1e-12
+
+
+@pytest.mark.parametrize("dens", [False, True])
+def test_multiply_diag_matr_power_complex_exponent(api, dens):
+    """multiplyDiagMatrPower: any (non-zero) diagonal, complex exponent, kets
+    only (left multiplication of a density matrix)."""
+    from oracle import qoracle as O
+    n = 5 if dens else 10
+    say(f"[gpu] multiply diag power dens={dens}")
+    rng = np.random.default_rng(21)
+    d = rng.normal(size=4) + 1j * rng.normal(size=4)
+    targs = [4, 1]
+    init = qc.random_state(n, 2, dens)
+    st = O.State(n, dens)
+    st.a[:] = init
+    st.diag_n([], targs, d, power=0.3 + 0.2j, multiply=True)
+    q = api.createQureg(n) if not dens else api.createDensityQureg(n)
+    m = api.make_diagmatr(d)
+    try:
+        api.set_amps(q, init)
+        api.multiplyDiagMatrPower(q, api.ints(targs), 2, m, api.qcomp.of(0.3 + 0.2j))
+        got = api.get_amps(q)
+    finally:
+        api.destroyDiagMatr(m)
+        api.destroyQureg(q)
+    assert rel_l2(got, st.a) <= 1e-12
+
+
+def fullstate_expected(init, dens, n, d, mode):
+    """mode: ('apply'|'multiply', power or None)."""
+    kind, p = mode
+    dp = d if p is None else d ** p
+    if not dens:
+        return init * dp
+    rho = init.reshape(1 << n, 1 << n, order="F")
+    out = dp[:, None] * rho if kind == "multiply" else dp[:, None] * rho * dp.conj()[None, :]
+    return out.reshape(-1, order="F")
+
+
+@pytest.mark.parametrize("dens", [False, True])
+@pytest.mark.parametrize("mode", [("apply", None), ("apply", 0.7), ("multiply", None), ("multiply", -0.5 + 0.25j)])
+def test_fullstate_diag_matches_closed_form(api, dens, mode):
+    n = 5 if dens else 12
+    say(f"[gpu] full-state diag dens={dens} mode={mode}")
+    rng = np.random.default_rng(5)
+    if mode[0] == "apply":
+        d = np.exp(1j * rng.uniform(0, 2 * np.pi, size=1 << n))
+    else:
+        d = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    init = qc.random_state(n, 6, dens)
+    want = fullstate_expected(init, dens, n, d, mode)
+    q = api.createQureg(n) if not dens else api.createDensityQureg(n)
+    m = api.make_fullstate_diag(n, d)
+    try:
+        api.set_amps(q, init)
+        kind, p = mode
+        if kind == "apply" and p is None:
+            api.applyFullStateDiagMatr(q, m)
+        elif kind == "apply":
+            api.applyFullStateDiagMatrPower(q, m, api.qcomp.of(p))
+        elif p is None:
+            api.multiplyFullStateDiagMatr(q, m)
+        else:
+            api.multiplyFullStateDiagMatrPower(q, m, api.qcomp.of(p))
+        got = api.get_amps(q)
+    finally:
+        api.destroyFullStateDiagMatr(m)
+        api.destroyQureg(q)
+    assert rel_l2(got, want) <= 1e-12
+
+
+@pytest.mark.parametrize("dens", [False, True])
+def test_fullstate_diag_expectations(api, dens):
+    n = 5 if dens else 12
+    say(f"[gpu] full-state diag expectations dens={dens}")
+    rng = np.random.default_rng(8)
+    init = qc.random_state(n, 9, dens)
+    weights = np.abs(init) ** 2 if not dens else np.real(np.diag(init.reshape(1 << n, 1 << n, order="F")))
+    dr = rng.uniform(0.2, 2.0, size=1 << n)  # real, positive: Hermitian, real powers stay real
+    dc = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    q = api.createQureg(n) if not dens else api.createDensityQureg(n)
+    mr = api.make_fullstate_diag(n, dr.astype(complex))
+    mc = api.make_fullstate_diag(n, dc)
+    try:
+        api.set_amps(q, init)
+        assert abs(api.calcExpecFullStateDiagMatr(q, mr) - np.sum(dr * weights)) <= 1e-10
+        assert abs(api.calcExpecFullStateDiagMatrPower(q, mr, 1.5) - np.sum(dr ** 1.5 * weights)) <= 1e-10
+        got = api.expec_nonhermitian_fullstate_diag(q, mc)
+        assert abs(got - np.sum(dc * weights)) <= 1e-10
+        got = api.expec_nonhermitian_fullstate_diag(q, mc, 0.4 - 0.3j)
+        assert abs(got - np.sum(dc ** (0.4 - 0.3j) * weights)) <= 1e-10
+    finally:
+        api.destroyFullStateDiagMatr(mr)
+        api.destroyFullStateDiagMatr(mc)
+        api.destroyQureg(q)
+
+
+SQRT_SWAP = np.array([[1, 0, 0, 0], [0, 0.5 + 0.5j, 0.5 - 0.5j, 0], [0, 0.5 - 0.5j, 0.5 + 0.5j, 0], [0, 0, 0, 1]])
+
+
+@pytest.mark.parametrize("dens", [False, True])
+@pytest.mark.parametrize("nctrl", [0, 1, 2])
+def test_controlled_sqrt_swap_matches_oracle(api, dens, nctrl):
+    from oracle import qoracle as O
+    n = 5 if dens else 10
+    say(f"[gpu] sqrt swap dens={dens} nctrl={nctrl}")
+    ctrls, states = [0, 3][:nctrl], [1, 0][:nctrl]
+    init = qc.random_state(n, 12, dens)
+    st = O.State(n, dens)
+    st.a[:] = init
+    st.unitary(ctrls, [4, 1], SQRT_SWAP, states=states)
+    q = api.createQureg(n) if not dens else api.createDensityQureg(n)
+    try:
+        api.set_amps(q, init)
+        if nctrl == 0:
+            api.applySqrtSwap(q, 4, 1)
+        elif nctrl == 1:
+            api.applyMultiStateControlledSqrtSwap(q, api.ints(ctrls), api.ints(states), 1, 4, 1)
+        else:
+            api.applyMultiStateControlledSqrtSwap(q, api.ints(ctrls), api.ints(states), 2, 4, 1)
+        got = api.get_amps(q)
+    finally:
+        api.destroyQureg(q)
+    assert rel_l2(got, st.a) <= 1e-12
+
+
+def test_diag_power_validation(api):
+    """validation.cpp:2367-2379 (real exponent for unitaries), 2408-2431
+    (no zero element under a real negative exponent), unitarity."""
+    say("[gpu] diag power validation")
+    q = api.createQureg(4)
+    unit = api.make_diagmatr(np.exp(1j * np.array([0.1, 0.2])))
+    nonu = api.make_diagmatr(np.array([2.0, 0.5]))
+    zero = api.make_diagmatr(np.array([0.0, 1.0]))
+    try:
+        with pytest.raises(api.QuESTError, match="real"):
+            api.applyDiagMatrPower(q, api.ints([0]), 1, unit, api.qcomp.of(0.5 + 0.5j))
+        with pytest.raises(api.QuESTError, match="unitary"):
+            api.applyDiagMatrPower(q, api.ints([0]), 1, nonu, api.qcomp.of(0.5))
+        with pytest.raises(api.QuESTError, match="zero"):
+            api.multiplyDiagMatrPower(q, api.ints([0]), 1, zero, api.qcomp.of(-1.0))
+        api.multiplyDiagMatrPower(q, api.ints([0]), 1, zero, api.qcomp.of(2.0))
+    finally:
+        for m in (unit, nonu, zero):
+            api.destroyDiagMatr(m)
+        api.destroyQureg(q)

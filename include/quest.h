@@ -244,6 +244,7 @@ void setDensityQuregFlatAmps(Qureg qureg, qindex startInd, qcomp* amps, qindex n
 void setQuregToClone(Qureg targetQureg, Qureg copyQureg);
 void setQuregToSuperposition(qcomp facOut, Qureg out, qcomp fac1, Qureg qureg1, qcomp fac2, Qureg qureg2);
 qreal setQuregToRenormalized(Qureg qureg);
+void setQuregToPauliStrSum(Qureg qureg, PauliStrSum sum); /* initialisations.cpp:216-227 */
 void setQuregToPartialTrace(Qureg out, Qureg in, int* traceOutQubits, int numTraceQubits);
 void setQuregToReducedDensityMatrix(Qureg out, Qureg in, int* retainQubits, int numRetainQubits);
 
@@ -263,6 +264,9 @@ void syncFullStateDiagMatr(FullStateDiagMatr matr);
 void setCompMatr(CompMatr matr, qcomp** vals);
 void setDiagMatr(DiagMatr out, qcomp* in);
 void setFullStateDiagMatr(FullStateDiagMatr out, qindex startInd, qcomp* in, qindex numElems);
+/* matrices.cpp:635-665: the diagonal of a sum of I/Z Pauli strings */
+void setFullStateDiagMatrFromPauliStrSum(FullStateDiagMatr out, PauliStrSum in);
+FullStateDiagMatr createFullStateDiagMatrFromPauliStrSum(PauliStrSum in);
 
 /* --------------------------------------------------------------- paulis */
 PauliStr getPauliStr(const char* paulis, int* indices, int numPaulis);

@@ -313,6 +313,17 @@ int qa_statevec_multiQubitProjector(void* amps, int prec, int numQubits,
 int qa_densmatr_multiQubitProjector(void* amps, int prec, int numQubitsN,
                                     const int* qubits, const int* outcomes, int num, double prob);
 int qa_statevec_initUniformState(void* amps, int prec, int numQubits, double re, double im);
+/* gpu_densmatr_setAmpsToPauliStrSum_sub (cpu_subroutines.cpp:63-92,
+   fastmath.hpp:124-196): rho_rc <- sum_j c_j <r|sigma_j|c>, terms as X, Y,
+   Z masks (host arrays) and complex coefficients {re, im} (host) */
+int qa_densmatr_setAmpsToPauliStrSum(void* amps, int prec, int numQubitsN, int64_t numTerms,
+                                     const uint64_t* maskX, const uint64_t* maskY, const uint64_t* maskZ,
+                                     const double* coeffs);
+/* gpu_fullstatediagmatr_setElemsToPauliStrSum (cpu_subroutines.cpp:95-116):
+   devElems = device complex128[2^numQubits] <- the diagonal of a sum of
+   Z strings (masks and coefficients on the host) */
+int qa_fullstatediagmatr_setElemsToPauliStrSum(void* devElems, int numQubits, int64_t numTerms,
+                                               const uint64_t* maskZ, const double* coeffs);
 int qa_statevec_initDebugState(void* amps, int prec, int numQubits);
 /* unnormalised Haar-distributed amplitudes from a counter-based generator
    keyed by (seed, amplitude index): reproducible for a given seed,

@@ -189,6 +189,9 @@ _SIGS = {
     "createFullStateDiagMatr": (FullStateDiagMatr, [cint]),
     "destroyFullStateDiagMatr": (None, [FullStateDiagMatr]),
     "setFullStateDiagMatr": (None, [FullStateDiagMatr, qindex, C.POINTER(qcomp), qindex]),
+    "setFullStateDiagMatrFromPauliStrSum": (None, [FullStateDiagMatr, PauliStrSum]),
+    "createFullStateDiagMatrFromPauliStrSum": (FullStateDiagMatr, [PauliStrSum]),
+    "setQuregToPauliStrSum": (None, [Qureg, PauliStrSum]),
     "getPauliStr": (PauliStr, [C.c_char_p, IntP, cint]),
     "createPauliStrSum": (PauliStrSum, [C.POINTER(PauliStr), C.POINTER(qcomp), qindex]),
     "createInlinePauliStrSum": (PauliStrSum, [C.c_char_p]),

@@ -350,6 +350,27 @@ void setQuregToSuperposition(qcomp facOut, Qureg out, qcomp fac1, Qureg q1, qcom
     });
 }
 
+// initialisations.cpp:216-227: rho <- the matrix of the sum (which may be
+// non-Hermitian)
+void setQuregToPauliStrSum(Qureg qureg, PauliStrSum sum) {
+    QAH_API({
+        validateQureg(qureg);
+        if (!qureg.isDensityMatrix) fail("The Qureg must be a density matrix.");
+        if (sum.numTerms < 1 || !sum.strings || !sum.coeffs) fail("Invalid PauliStrSum.");
+        std::vector<uint64_t> mx((size_t)sum.numTerms), my((size_t)sum.numTerms), mz((size_t)sum.numTerms);
+        std::vector<double> c(2 * (size_t)sum.numTerms);
+        for (qindex t = 0; t < sum.numTerms; t++) {
+            if (pauliMaxIndex(sum.strings[t]) >= qureg.numQubits)
+                fail("The PauliStrSum targets qubits beyond the " + std::to_string(qureg.numQubits) + "-qubit Qureg.");
+            pauliMasks(sum.strings[t], mx[t], my[t], mz[t]);
+            c[2 * t] = sum.coeffs[t].real();
+            c[2 * t + 1] = sum.coeffs[t].imag();
+        }
+        check(qa_densmatr_setAmpsToPauliStrSum(dev(qureg), precOf(qureg), qureg.numQubits, sum.numTerms, mx.data(),
+                                               my.data(), mz.data(), c.data()));
+    });
+}
+
 qreal setQuregToRenormalized(Qureg qureg) {
     QAH_API({
         validateQureg(qureg);

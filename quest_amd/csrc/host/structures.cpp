@@ -3,6 +3,7 @@
 // channels.cpp:176-313, utilities.cpp:771-801 for the superoperator).
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 #include <string>
 
@@ -235,6 +236,43 @@ void setFullStateDiagMatr(FullStateDiagMatr m, qindex start, qcomp* in, qindex n
         *m.wasGpuSynced = 1;
         *m.isApproxUnitary = UNKNOWN;
         *m.isApproxHermitian = UNKNOWN;
+    });
+}
+
+// matrices.cpp:635-665: the diagonal of a sum of I/Z strings, computed on
+// the GPU and mirrored to the host elements
+void setFullStateDiagMatrFromPauliStrSum(FullStateDiagMatr m, PauliStrSum sum) {
+    QAH_API({
+        if (sum.numTerms < 1 || !sum.strings || !sum.coeffs) fail("Invalid PauliStrSum.");
+        std::vector<uint64_t> mz((size_t)sum.numTerms);
+        std::vector<double> c(2 * (size_t)sum.numTerms);
+        for (qindex t = 0; t < sum.numTerms; t++) {
+            uint64_t x, y, z;
+            pauliMasks(sum.strings[t], x, y, z);
+            if (x | y) fail("The PauliStrSum contains X or Y Paulis, so it cannot initialise a diagonal matrix.");
+            if (pauliMaxIndex(sum.strings[t]) >= m.numQubits)
+                fail("The PauliStrSum targets qubits beyond the " + std::to_string(m.numQubits) +
+                     "-qubit FullStateDiagMatr.");
+            mz[t] = z;
+            c[2 * t] = sum.coeffs[t].real();
+            c[2 * t + 1] = sum.coeffs[t].imag();
+        }
+        check(qa_fullstatediagmatr_setElemsToPauliStrSum(m.gpuElems, m.numQubits, sum.numTerms, mz.data(), c.data()));
+        check(qa_memcpyDtoH(m.cpuElems, m.gpuElems, (size_t)m.numElems * 16));
+        *m.wasGpuSynced = 1;
+        *m.isApproxUnitary = UNKNOWN;
+        *m.isApproxHermitian = UNKNOWN;
+    });
+}
+
+FullStateDiagMatr createFullStateDiagMatrFromPauliStrSum(PauliStrSum sum) {
+    QAH_API({
+        if (sum.numTerms < 1 || !sum.strings || !sum.coeffs) fail("Invalid PauliStrSum.");
+        int top = -1;
+        for (qindex t = 0; t < sum.numTerms; t++) top = std::max(top, pauliMaxIndex(sum.strings[t]));
+        FullStateDiagMatr m = createFullStateDiagMatr(top + 1);
+        setFullStateDiagMatrFromPauliStrSum(m, sum);
+        return m;
     });
 }
 

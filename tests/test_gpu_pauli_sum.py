@@ -207,3 +207,77 @@ def test_pauli_sum_operation_errors(api):
             api.destroyPauliStrSum(s)
         api.destroyQureg(w)
         api.destroyQureg(q)
+
+
+PAULI = {"I": np.eye(2), "X": np.array([[0, 1], [1, 0]]), "Y": np.array([[0, -1j], [1j, 0]]),
+         "Z": np.diag([1.0, -1.0])}
+
+
+def pauli_op(n, chars, targs):
+    per = {t: c for c, t in zip(chars, targs)}
+    out = np.array([[1.0 + 0j]])
+    for q in range(n - 1, -1, -1):
+        out = np.kron(out, PAULI[per.get(q, "I")])
+    return out
+
+
+@pytest.mark.parametrize("prec,tol", [(2, 1e-13), (1, 1e-5)])
+def test_set_qureg_to_pauli_str_sum(api, prec, tol):
+    """setQuregToPauliStrSum (initialisations.cpp:216-227): the density
+    matrix becomes sum_j c_j sigma_j, against Kronecker products in numpy
+    (column-major flattening, complex coefficients)."""
+    n = 4
+    say(f"[gpu] setQuregToPauliStrSum prec={prec}")
+    terms = sum_terms(n, 31, True)
+    want = sum(c * pauli_op(n, ch, t) for ch, t, c in terms).reshape(-1, order="F")
+    q = api.createCustomPrecisionQureg(n, 1, prec)
+    s = make_sum(api, terms, [c for _, _, c in terms])
+    try:
+        api.setQuregToPauliStrSum(q, s)
+        got = api.get_amps(q)
+    finally:
+        api.destroyPauliStrSum(s)
+        api.destroyQureg(q)
+    assert np.max(np.abs(got - want)) <= tol * max(1.0, np.max(np.abs(want)))
+
+
+def test_full_state_diag_from_z_sum(api):
+    """createFullStateDiagMatrFromPauliStrSum / setFullStateDiagMatrFromPauliStrSum
+    (matrices.cpp:635-665): the diagonal of a sum of I/Z strings, checked
+    against numpy and through applying it (multiplyFullStateDiagMatr)."""
+    n = 9
+    say("[gpu] full-state diagonal from a Z sum")
+    rng = np.random.default_rng(4)
+    terms = []
+    for _ in range(6):
+        k = int(rng.integers(1, n + 1))
+        targs = [int(t) for t in rng.choice(n, size=k, replace=False)]
+        terms.append(("Z" * k, targs, complex(rng.normal(), rng.normal())))
+    terms.append(("Z", [n - 1], 0.5 + 0j))
+    want = np.diag(sum(c * pauli_op(n, ch, t) for ch, t, c in terms))
+    s = make_sum(api, terms, [c for _, _, c in terms])
+    m = api.createFullStateDiagMatrFromPauliStrSum(s)
+    q = api.createQureg(n)
+    xs = make_sum(api, [("XZ", [0, 1], 1.0)])
+    try:
+        assert m.numQubits == n
+        elems = np.ctypeslib.as_array(C.cast(m.cpuElems, C.POINTER(C.c_double)), shape=(2 << n,))
+        got = elems[0::2] + 1j * elems[1::2]
+        assert np.max(np.abs(got - want)) <= 1e-13
+        init = qc.random_state(n, 3)
+        api.set_amps(q, init)
+        api.multiplyFullStateDiagMatr(q, m)
+        assert rel_l2(api.get_amps(q), init * want) <= 1e-13
+        with pytest.raises(api.QuESTError, match="X or Y"):
+            api.setFullStateDiagMatrFromPauliStrSum(m, xs)
+        sv = api.createQureg(3)
+        try:
+            with pytest.raises(api.QuESTError, match="density matrix"):
+                api.setQuregToPauliStrSum(sv, xs)
+        finally:
+            api.destroyQureg(sv)
+    finally:
+        api.destroyPauliStrSum(xs)
+        api.destroyPauliStrSum(s)
+        api.destroyFullStateDiagMatr(m)
+        api.destroyQureg(q)

@@ -264,6 +264,14 @@ void syncFullStateDiagMatr(FullStateDiagMatr matr);
 void setCompMatr(CompMatr matr, qcomp** vals);
 void setDiagMatr(DiagMatr out, qcomp* in);
 void setFullStateDiagMatr(FullStateDiagMatr out, qindex startInd, qcomp* in, qindex numElems);
+/* matrices.cpp:668-730: diagonals as functions of integer variables (each
+   variable the next numQubitsPerVar[v] index bits, lowest first) */
+void setDiagMatrFromMultiVarFunc(DiagMatr out, qcomp (*func)(qindex*), int* numQubitsPerVar, int numVars,
+                                 int areSigned);
+void setDiagMatrFromMultiDimLists(DiagMatr out, void* lists, int* numQubitsPerDim, int numDims);
+void setFullStateDiagMatrFromMultiVarFunc(FullStateDiagMatr out, qcomp (*func)(qindex*), int* numQubitsPerVar,
+                                          int numVars, int areSigned);
+void setFullStateDiagMatrFromMultiDimLists(FullStateDiagMatr out, void* lists, int* numQubitsPerDim, int numDims);
 /* matrices.cpp:635-665: the diagonal of a sum of I/Z Pauli strings */
 void setFullStateDiagMatrFromPauliStrSum(FullStateDiagMatr out, PauliStrSum in);
 FullStateDiagMatr createFullStateDiagMatrFromPauliStrSum(PauliStrSum in);

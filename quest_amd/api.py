@@ -190,6 +190,10 @@ _SIGS = {
     "destroyFullStateDiagMatr": (None, [FullStateDiagMatr]),
     "setFullStateDiagMatr": (None, [FullStateDiagMatr, qindex, C.POINTER(qcomp), qindex]),
     "setFullStateDiagMatrFromPauliStrSum": (None, [FullStateDiagMatr, PauliStrSum]),
+    "setDiagMatrFromMultiVarFunc": (None, [DiagMatr, C.c_void_p, IntP, cint, cint]),
+    "setDiagMatrFromMultiDimLists": (None, [DiagMatr, C.c_void_p, IntP, cint]),
+    "setFullStateDiagMatrFromMultiVarFunc": (None, [FullStateDiagMatr, C.c_void_p, IntP, cint, cint]),
+    "setFullStateDiagMatrFromMultiDimLists": (None, [FullStateDiagMatr, C.c_void_p, IntP, cint]),
     "createFullStateDiagMatrFromPauliStrSum": (FullStateDiagMatr, [PauliStrSum]),
     "setQuregToPauliStrSum": (None, [Qureg, PauliStrSum]),
     "getPauliStr": (PauliStr, [C.c_char_p, IntP, cint]),
@@ -217,6 +221,7 @@ _SIGS = {
     "applyDiagMatr": (None, [Qureg, IntP, cint, DiagMatr]),
     "applyFullStateDiagMatr": (None, [Qureg, FullStateDiagMatr]),
     "multiplyFullStateDiagMatr": (None, [Qureg, FullStateDiagMatr]),
+    "multiplyDiagMatr": (None, [Qureg, IntP, cint, DiagMatr]),
     "multiplyDiagMatrPower": (None, [Qureg, IntP, cint, DiagMatr, qcomp]),
     "applyDiagMatrPower": (None, [Qureg, IntP, cint, DiagMatr, qcomp]),
     "applyControlledDiagMatrPower": (None, [Qureg, cint, IntP, cint, DiagMatr, qcomp]),

@@ -63,6 +63,54 @@ void buildSuperOp(SuperOp op, qcomp*** ks, int numOps, qindex d) {
 
 }  // namespace
 
+namespace {
+
+// validation.cpp:2170-2188: at least one variable, each of at least one
+// qubit, together the matrix's qubits
+void validateVars(int matrQubits, const int* numQubitsPerVar, int numVars) {
+    if (numVars <= 0 || !numQubitsPerVar) fail("The number of variables must be positive.");
+    int total = 0;
+    for (int v = 0; v < numVars; v++) {
+        if (numQubitsPerVar[v] <= 0)
+            fail("Variable " + std::to_string(v) + " must have a positive number of qubits.");
+        total += numQubitsPerVar[v];
+    }
+    if (total != matrQubits)
+        fail("The variables' qubits (" + std::to_string(total) + ") differ from the matrix's (" +
+             std::to_string(matrQubits) + ").");
+}
+
+// fastmath.hpp:100-115: variable n takes the next numQubitsPerVar[n] index
+// bits, lowest first; with areSigned, a variable whose top bit is set has
+// 2^(bits-1) subtracted, as the reference computes it
+void varValues(qindex index, const int* numQubitsPerVar, int numVars, bool areSigned, qindex* out) {
+    qindex rest = index;
+    for (int n = 0; n < numVars; n++) {
+        out[n] = rest & ((1LL << numQubitsPerVar[n]) - 1);
+        rest >>= numQubitsPerVar[n];
+    }
+    if (areSigned)
+        for (int n = 0; n < numVars; n++)
+            if ((out[n] >> (numQubitsPerVar[n] - 1)) & 1) out[n] -= 1LL << (numQubitsPerVar[n] - 1);
+}
+
+// utilities.cpp:307-315: lists[i0][i1]...[i_{n-1}]
+qcomp nestedElem(void* lists, const qindex* inds, int n) {
+    if (n == 1) return static_cast<qcomp*>(lists)[inds[0]];
+    return nestedElem(static_cast<void**>(lists)[inds[0]], inds + 1, n - 1);
+}
+
+template <class F>
+void fillDiag(qcomp* elems, qindex numElems, const int* numQubitsPerVar, int numVars, bool areSigned, F&& elemOf) {
+    std::vector<qindex> vals((size_t)numVars);
+    for (qindex i = 0; i < numElems; i++) {
+        varValues(i, numQubitsPerVar, numVars, areSigned, vals.data());
+        elems[i] = elemOf(vals.data());
+    }
+}
+
+}  // namespace
+
 extern "C" {
 
 void _validateNewNestedElemsPtrNotNull(qcomp** ptrs, int numQubits, const char* caller) {
@@ -236,6 +284,49 @@ void setFullStateDiagMatr(FullStateDiagMatr m, qindex start, qcomp* in, qindex n
         *m.wasGpuSynced = 1;
         *m.isApproxUnitary = UNKNOWN;
         *m.isApproxHermitian = UNKNOWN;
+    });
+}
+
+// ---- diagonal matrices from functions of integer variables (matrices.cpp:668-730)
+
+// the callback is called once per element, in index order, on the host
+void setDiagMatrFromMultiVarFunc(DiagMatr out, qcomp (*f)(qindex*), int* numQubitsPerVar, int numVars,
+                                 int areSigned) {
+    QAH_API({
+        validateVars(out.numQubits, numQubitsPerVar, numVars);
+        if (areSigned != 0 && areSigned != 1) fail("areSigned must be 0 or 1.");
+        if (!f) fail("NULL callback function.");
+        fillDiag(out.cpuElems, out.numElems, numQubitsPerVar, numVars, areSigned, [&](qindex* v) { return f(v); });
+        syncDiagMatr(out);
+    });
+}
+void setFullStateDiagMatrFromMultiVarFunc(FullStateDiagMatr out, qcomp (*f)(qindex*), int* numQubitsPerVar,
+                                          int numVars, int areSigned) {
+    QAH_API({
+        validateVars(out.numQubits, numQubitsPerVar, numVars);
+        if (areSigned != 0 && areSigned != 1) fail("areSigned must be 0 or 1.");
+        if (!f) fail("NULL callback function.");
+        fillDiag(out.cpuElems, out.numElems, numQubitsPerVar, numVars, areSigned, [&](qindex* v) { return f(v); });
+        syncFullStateDiagMatr(out);
+    });
+}
+// lists nested numDims deep, indexed by the variables' (unsigned) values
+void setDiagMatrFromMultiDimLists(DiagMatr out, void* lists, int* numQubitsPerDim, int numDims) {
+    QAH_API({
+        validateVars(out.numQubits, numQubitsPerDim, numDims);
+        if (!lists) fail("NULL lists.");
+        fillDiag(out.cpuElems, out.numElems, numQubitsPerDim, numDims, false,
+                 [&](qindex* v) { return nestedElem(lists, v, numDims); });
+        syncDiagMatr(out);
+    });
+}
+void setFullStateDiagMatrFromMultiDimLists(FullStateDiagMatr out, void* lists, int* numQubitsPerDim, int numDims) {
+    QAH_API({
+        validateVars(out.numQubits, numQubitsPerDim, numDims);
+        if (!lists) fail("NULL lists.");
+        fillDiag(out.cpuElems, out.numElems, numQubitsPerDim, numDims, false,
+                 [&](qindex* v) { return nestedElem(lists, v, numDims); });
+        syncFullStateDiagMatr(out);
     });
 }
 

@@ -10,7 +10,9 @@
     multiplying; sum_i d_i^p |a_i|^2 and sum_i d_i^p rho_ii);
   - controlled square-root-of-SWAP (operations.cpp:825-860) against the
     oracle's dense two-qubit unitary;
-  - the validation of unitarity, real exponents and divergent powers.
+  - the validation of unitarity, real exponents and divergent powers;
+  - diagonals from a C callback of integer variables and from nested lists
+    (matrices.cpp:668-730), element by element against numpy.
 Tolerances: 1e-12 relative L2 on amplitudes, 1e-10 on scalars (complex128).
 """
 
@@ -213,4 +215,106 @@ def test_diag_power_validation(api):
     finally:
         for m in (unit, nonu, zero):
             api.destroyDiagMatr(m)
+        api.destroyQureg(q)
+
+
+CALLBACK_SRC = r"""
+#include <complex.h>
+#include <stdint.h>
+double _Complex qa_test_var_func(int64_t* v) {
+    return (double)v[0] + 0.5 * (double)v[1] + I * ((double)v[0] * (double)v[1] - 1.0);
+}
+"""
+
+
+@pytest.fixture(scope="module")
+def var_func(tmp_path_factory):
+    """A C callback qcomp f(qindex*), built with gcc for the test."""
+    import ctypes as C
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if not gcc:
+        pytest.skip("gcc not available")
+    d = tmp_path_factory.mktemp("varfunc")
+    (d / "f.c").write_text(CALLBACK_SRC)
+    subprocess.run([gcc, "-shared", "-fPIC", "-O1", str(d / "f.c"), "-o", str(d / "libf.so")], check=True,
+                   capture_output=True)
+    lib = C.CDLL(str(d / "libf.so"))
+    return C.cast(lib.qa_test_var_func, C.c_void_p)
+
+
+def var_values(i, bits, signed):
+    """fastmath.hpp:100-115, as the reference computes it."""
+    out = []
+    for b in bits:
+        out.append(i & ((1 << b) - 1))
+        i >>= b
+    if signed:
+        out = [v - (1 << (b - 1)) if (v >> (b - 1)) & 1 else v for v, b in zip(out, bits)]
+    return out
+
+
+@pytest.mark.parametrize("signed", [0, 1])
+def test_diag_matrices_from_multi_var_func(api, var_func, signed):
+    """setDiagMatrFromMultiVarFunc / setFullStateDiagMatrFromMultiVarFunc
+    (matrices.cpp:668-699): host elements from the callback, in index order,
+    synced to the GPU (checked by multiplying a state)."""
+    import ctypes as C
+    say(f"[gpu] diag from multi-var func signed={signed}")
+    bits = [2, 3]
+    n = sum(bits)
+    want = np.array([complex(a + 0.5 * b, a * b - 1.0) for a, b in (var_values(i, bits, signed) for i in range(1 << n))])
+    dm = api.createDiagMatr(n)
+    fm = api.createFullStateDiagMatr(n)
+    q = api.createQureg(n)
+    try:
+        api.setDiagMatrFromMultiVarFunc(dm, var_func, api.ints(bits), 2, signed)
+        api.setFullStateDiagMatrFromMultiVarFunc(fm, var_func, api.ints(bits), 2, signed)
+        for m in (dm, fm):
+            e = np.ctypeslib.as_array(C.cast(m.cpuElems, C.POINTER(C.c_double)), shape=(2 << n,))
+            assert np.array_equal(e[0::2] + 1j * e[1::2], want)
+        init = qc.random_state(n, 2)
+        api.set_amps(q, init)
+        api.multiplyFullStateDiagMatr(q, fm)
+        assert rel_l2(api.get_amps(q), init * want) <= 1e-15
+        api.set_amps(q, init)
+        api.multiplyDiagMatr(q, api.ints(list(range(n))), n, dm)
+        assert rel_l2(api.get_amps(q), init * want) <= 1e-15
+        with pytest.raises(api.QuESTError, match="qubits"):
+            api.setDiagMatrFromMultiVarFunc(dm, var_func, api.ints([2, 2]), 2, signed)
+    finally:
+        api.destroyDiagMatr(dm)
+        api.destroyFullStateDiagMatr(fm)
+        api.destroyQureg(q)
+
+
+def test_diag_matrices_from_multi_dim_lists(api):
+    """setDiagMatrFromMultiDimLists / setFullStateDiagMatrFromMultiDimLists
+    (matrices.cpp:702-730): element i = lists[v0][v1] with v0 the low bits."""
+    import ctypes as C
+    say("[gpu] diag from nested lists")
+    bits = [3, 2]
+    n = sum(bits)
+    rng = np.random.default_rng(3)
+    table = rng.normal(size=(1 << bits[0], 1 << bits[1])) + 1j * rng.normal(size=(1 << bits[0], 1 << bits[1]))
+    rows = [(api.qcomp * (1 << bits[1]))(*[api.qcomp.of(z) for z in table[r]]) for r in range(1 << bits[0])]
+    outer = (C.c_void_p * (1 << bits[0]))(*[C.cast(r, C.c_void_p) for r in rows])
+    want = np.array([table[i & 7, i >> 3] for i in range(1 << n)])
+    dm = api.createDiagMatr(n)
+    fm = api.createFullStateDiagMatr(n)
+    q = api.createQureg(n)
+    try:
+        api.setDiagMatrFromMultiDimLists(dm, C.cast(outer, C.c_void_p), api.ints(bits), 2)
+        api.setFullStateDiagMatrFromMultiDimLists(fm, C.cast(outer, C.c_void_p), api.ints(bits), 2)
+        for m in (dm, fm):
+            e = np.ctypeslib.as_array(C.cast(m.cpuElems, C.POINTER(C.c_double)), shape=(2 << n,))
+            assert np.array_equal(e[0::2] + 1j * e[1::2], want)
+        init = qc.random_state(n, 5)
+        api.set_amps(q, init)
+        api.multiplyFullStateDiagMatr(q, fm)
+        assert rel_l2(api.get_amps(q), init * want) <= 1e-15
+    finally:
+        api.destroyDiagMatr(dm)
+        api.destroyFullStateDiagMatr(fm)
         api.destroyQureg(q)

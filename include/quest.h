@@ -583,6 +583,15 @@ void setCompMatr(CompMatr out, std::vector<std::vector<qcomp>> in);
 void setDiagMatr(DiagMatr out, std::vector<qcomp> in);
 void setKrausMap(KrausMap map, std::vector<std::vector<std::vector<qcomp>>> matrices);
 void setSuperOp(SuperOp op, std::vector<std::vector<qcomp>> matrix);
+CompMatr createInlineCompMatr(int numQb, std::vector<std::vector<qcomp>> elems);
+DiagMatr createInlineDiagMatr(int numQb, std::vector<qcomp> elems);
+KrausMap createInlineKrausMap(int numQb, int numOps, std::vector<std::vector<std::vector<qcomp>>> matrices);
+SuperOp createInlineSuperOp(int numQb, std::vector<std::vector<qcomp>> matrix);
+void setInlineCompMatr(CompMatr matr, int numQb, std::vector<std::vector<qcomp>> in);
+void setInlineDiagMatr(DiagMatr matr, int numQb, std::vector<qcomp> in);
+void setInlineFullStateDiagMatr(FullStateDiagMatr matr, qindex startInd, qindex numElems, std::vector<qcomp> in);
+void setInlineKrausMap(KrausMap map, int numQb, int numOps, std::vector<std::vector<std::vector<qcomp>>> matrices);
+void setInlineSuperOp(SuperOp op, int numQb, std::vector<std::vector<qcomp>> matrix);
 PauliStr getPauliStr(std::string paulis, int* indices, int numPaulis);
 PauliStr getPauliStr(std::string paulis, std::vector<int> indices);
 PauliStr getPauliStr(std::string paulis);

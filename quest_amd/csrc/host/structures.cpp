@@ -599,25 +599,113 @@ DiagMatr2 getDiagMatr2(std::vector<qcomp> in) {
     for (int i = 0; i < 4; i++) m.elems[i] = in.at((size_t)i);
     return m;
 }
-void setCompMatr(CompMatr out, std::vector<std::vector<qcomp>> in) {
-    std::vector<qcomp*> rows;
-    for (auto& r : in) rows.push_back(r.data());
-    setCompMatr(out, rows.data());
+// the C++ container forms check the containers' sizes against the matrix
+// (matrices.cpp / channels.cpp validate_matrixNumNewElems and friends)
+namespace {
+void requireSquare(const std::vector<std::vector<qcomp>>& in, qindex dim, const char* what) {
+    if ((qindex)in.size() != dim) fail(std::string("The ") + what + " has the wrong number of rows.");
+    for (const auto& r : in)
+        if ((qindex)r.size() != dim) fail(std::string("A row of the ") + what + " has the wrong number of elements.");
 }
-void setDiagMatr(DiagMatr out, std::vector<qcomp> in) { setDiagMatr(out, in.data()); }
+}  // namespace
+
+void setCompMatr(CompMatr out, std::vector<std::vector<qcomp>> in) {
+    QAH_API({
+        requireSquare(in, out.numRows, "CompMatr");
+        std::vector<qcomp*> rows;
+        for (auto& r : in) rows.push_back(r.data());
+        setCompMatr(out, rows.data());
+    });
+}
+void setDiagMatr(DiagMatr out, std::vector<qcomp> in) {
+    QAH_API({
+        if ((qindex)in.size() != out.numElems) fail("The DiagMatr needs exactly " + std::to_string(out.numElems) +
+                                                      " elements.");
+        setDiagMatr(out, in.data());
+    });
+}
 void setKrausMap(KrausMap map, std::vector<std::vector<std::vector<qcomp>>> ms) {
-    std::vector<std::vector<qcomp*>> rows(ms.size());
-    std::vector<qcomp**> ptrs;
-    for (size_t n = 0; n < ms.size(); n++) {
-        for (auto& r : ms[n]) rows[n].push_back(r.data());
-        ptrs.push_back(rows[n].data());
-    }
-    setKrausMap(map, ptrs.data());
+    QAH_API({
+        if ((int)ms.size() != map.numMatrices) fail("The KrausMap needs exactly " + std::to_string(map.numMatrices) +
+                                                     " matrices.");
+        for (auto& m : ms) requireSquare(m, map.numRows, "Kraus operator");
+        std::vector<std::vector<qcomp*>> rows(ms.size());
+        std::vector<qcomp**> ptrs;
+        for (size_t n = 0; n < ms.size(); n++) {
+            for (auto& r : ms[n]) rows[n].push_back(r.data());
+            ptrs.push_back(rows[n].data());
+        }
+        setKrausMap(map, ptrs.data());
+    });
 }
 void setSuperOp(SuperOp op, std::vector<std::vector<qcomp>> matrix) {
-    std::vector<qcomp*> rows;
-    for (auto& r : matrix) rows.push_back(r.data());
-    setSuperOp(op, rows.data());
+    QAH_API({
+        requireSquare(matrix, op.numRows, "SuperOp");
+        std::vector<qcomp*> rows;
+        for (auto& r : matrix) rows.push_back(r.data());
+        setSuperOp(op, rows.data());
+    });
+}
+
+// inline constructors and setters (matrices.h:1027-1301, channels.h:483-562)
+CompMatr createInlineCompMatr(int numQb, std::vector<std::vector<qcomp>> elems) {
+    QAH_API({
+        CompMatr m = createCompMatr(numQb);
+        setCompMatr(m, elems);
+        return m;
+    });
+}
+DiagMatr createInlineDiagMatr(int numQb, std::vector<qcomp> elems) {
+    QAH_API({
+        DiagMatr m = createDiagMatr(numQb);
+        setDiagMatr(m, elems);
+        return m;
+    });
+}
+KrausMap createInlineKrausMap(int numQb, int numOps, std::vector<std::vector<std::vector<qcomp>>> matrices) {
+    QAH_API({
+        KrausMap m = createKrausMap(numQb, numOps);
+        setKrausMap(m, matrices);
+        return m;
+    });
+}
+SuperOp createInlineSuperOp(int numQb, std::vector<std::vector<qcomp>> matrix) {
+    QAH_API({
+        SuperOp m = createSuperOp(numQb);
+        setSuperOp(m, matrix);
+        return m;
+    });
+}
+void setInlineCompMatr(CompMatr m, int numQb, std::vector<std::vector<qcomp>> in) {
+    QAH_API({
+        if (numQb != m.numQubits) fail("The number of qubits does not match the CompMatr's.");
+        setCompMatr(m, in);
+    });
+}
+void setInlineDiagMatr(DiagMatr m, int numQb, std::vector<qcomp> in) {
+    QAH_API({
+        if (numQb != m.numQubits) fail("The number of qubits does not match the DiagMatr's.");
+        setDiagMatr(m, in);
+    });
+}
+void setInlineFullStateDiagMatr(FullStateDiagMatr m, qindex startInd, qindex numElems, std::vector<qcomp> in) {
+    QAH_API({
+        if ((qindex)in.size() != numElems) fail("The list length does not match numElems.");
+        setFullStateDiagMatr(m, startInd, in.data(), numElems);
+    });
+}
+void setInlineKrausMap(KrausMap m, int numQb, int numOps, std::vector<std::vector<std::vector<qcomp>>> in) {
+    QAH_API({
+        if (numQb != m.numQubits || numOps != m.numMatrices)
+            fail("The number of qubits or operators does not match the KrausMap's.");
+        setKrausMap(m, in);
+    });
+}
+void setInlineSuperOp(SuperOp m, int numQb, std::vector<std::vector<qcomp>> in) {
+    QAH_API({
+        if (numQb != m.numQubits) fail("The number of qubits does not match the SuperOp's.");
+        setSuperOp(m, in);
+    });
 }
 PauliStr getPauliStr(std::string paulis, int* indices, int numPaulis) {
     return getPauliStr(paulis.c_str(), indices, numPaulis);

@@ -896,6 +896,7 @@ void setLdsOffsets(TOp* prog, size_t num, TileArgs& a, int prec) {
     int row[RB];
     for (int b = 0; b < RB; b++) row[b] = a.k - RB + b;
     offsets(row, a.rowOff);
+    if (num == 0) return;
     int cur[RB];
     std::memcpy(cur, groupSlots(prog[0]), sizeof(cur));
     for (size_t i = 0; i < num; i++) {

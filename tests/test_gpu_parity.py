@@ -209,6 +209,34 @@ def test_measurement_is_seeded(api):
     assert outs[0] == outs[1]
 
 
+def test_qureg_size_limits(api):
+    """Sizes at and past the limits are input errors, never a crash: no
+    qubits, more index bits than one GPU addresses, and a state larger than
+    the free HBM (2^36 complex128 amplitudes = 1 TiB > 288 GB); after them a
+    1-qubit and a 30-qubit state still work."""
+    say("[gpu] qureg size limits")
+    with pytest.raises(api.QuESTError, match="qubit"):
+        api.createQureg(0)
+    with pytest.raises(api.QuESTError, match="limit"):
+        api.createQureg(41)
+    with pytest.raises(api.QuESTError, match="limit"):
+        api.createDensityQureg(21)
+    with pytest.raises(api.QuESTError, match="GPU memory"):
+        api.createQureg(36)
+    q = api.createQureg(1)
+    try:
+        api.applyHadamard(q, 0)
+        assert abs(api.calcProbOfQubitOutcome(q, 0, 1) - 0.5) < 1e-15
+    finally:
+        api.destroyQureg(q)
+    q = api.createQureg(30)
+    try:
+        api.applyHadamard(q, 29)
+        assert abs(api.calcProbOfQubitOutcome(q, 29, 0) - 0.5) < 1e-12
+    finally:
+        api.destroyQureg(q)
+
+
 def test_input_errors_raise(api):
     say("[gpu] validation")
     q = api.createQureg(4)

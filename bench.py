@@ -440,7 +440,7 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--fusion", default=None,
                     help="tileQubits,lowQubits,maxQueue of the gate-fusion queue, or 'off' (default: 13,3,2048 "
-                         "for the complex128 modes, 12,4,512 for the complex64 precision mode, the shapes "
+                         "for the complex128 modes, 12,4,2048 for the complex64 precision mode, the shapes "
                          "measured fastest for each)")
     ap.add_argument("--no-unfused", action="store_true", help="skip the one-step unfused comparison")
     ap.add_argument("--prefetch", type=int, default=1, help="tile passes prefetch the next tile (1) or not (0)")
@@ -455,7 +455,7 @@ def main() -> int:
                          "as the calibration point of rocprofv3 FETCH_SIZE / WRITE_SIZE runs")
     args = ap.parse_args()
     if args.fusion is None:
-        args.fusion = "12,4,512" if args.mode == "precision" else "13,3,2048"
+        args.fusion = "12,4,2048" if args.mode == "precision" else "13,3,2048"
 
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported by this single-GPU build "

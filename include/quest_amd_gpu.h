@@ -62,6 +62,10 @@ int qa_isGpuAvailable(void);
    creates the library stream and the reduction scratch. Idempotent. */
 int qa_init(int device);
 int qa_finalize(void);
+/* a counter that changes whenever an entry point that may write amplitudes
+   runs (no counterpart in QuEST): results computed at equal values still
+   hold; a host layer keys caches of reductions on it */
+uint64_t qa_stateEpoch(void);
 /* gpu_getCurrentAvailableMemoryInBytes / gpu_getTotalMemoryInBytes */
 int qa_getMemInfo(size_t* freeBytes, size_t* totalBytes);
 /* number of compute units and device name, for reporting */
@@ -258,6 +262,10 @@ int qa_densmatr_partialTrace(const void* inAmps, int prec, int inNumQubitsN, voi
 
 int qa_statevec_calcTotalProb(const void* amps, int prec, int numQubits, double* out);
 int qa_densmatr_calcTotalProb(const void* amps, int prec, int numQubitsN, double* out);
+/* every qubit's outcome probabilities in one read of the state per 8 qubits
+   (gpu_statevec_calcProbOfMultiQubitOutcome_sub for each single qubit, as
+   calcProbOfQubitOutcome asks it): out[2q] = P(q = 0), out[2q+1] = P(q = 1) */
+int qa_statevec_calcProbsOfEveryQubit(const void* amps, int prec, int numQubits, double* out);
 int qa_statevec_calcProbOfMultiQubitOutcome(const void* amps, int prec, int numQubits,
                                             const int* qubits, const int* outcomes, int num,
                                             double* out);

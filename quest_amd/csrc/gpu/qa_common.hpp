@@ -231,18 +231,28 @@ __device__ __forceinline__ uint64_t blockId() { return (uint64_t)blockIdx.y * gr
 void setError(const char* fmt, ...);
 hipStream_t stream();
 bool isInit();
+// state epoch: bumped by every entry point that may change amplitudes
+// (QA_REQUIRE_INIT), left alone by the read-only reductions
+// (QA_REQUIRE_INIT_RO); equal epochs mean no state has changed in between
+void bumpEpoch();
 // reduction scratch: device partials and a pinned host landing area
 double* devScratch(size_t numDoubles);
 double* hostScratch(size_t numDoubles);
 
 }  // namespace qa
 
-#define QA_REQUIRE_INIT()                                                  \
+#define QA_REQUIRE_INIT_RO()                                               \
     do {                                                                   \
         if (!qa::isInit()) {                                               \
             qa::setError("%s: qa_init() has not been called", __func__);   \
             return QA_ERR_NOT_INIT;                                        \
         }                                                                  \
+    } while (0)
+
+#define QA_REQUIRE_INIT()     \
+    do {                      \
+        QA_REQUIRE_INIT_RO(); \
+        qa::bumpEpoch();      \
     } while (0)
 
 #define QA_HIP(cmd)                                                                   \

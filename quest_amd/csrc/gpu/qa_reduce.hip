@@ -121,6 +121,23 @@ __device__ __forceinline__ double norm2(cd a) { return a.re * a.re + a.im * a.im
 
 // ---- term functors --------------------------------------------------------
 
+// |a_n|^2 split by the value of each of the 8 qubits q0..q0+7:
+// o[2b + bit] for qubit q0 + b
+template <typename T>
+struct FProbsOf8 {
+    const T* a;
+    int q0;
+    __device__ void operator()(uint64_t n, double* o) const {
+        const double p = norm2(ldd(a, n));
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const bool one = (n >> (q0 + b)) & 1ull;
+            o[2 * b] = one ? 0.0 : p;
+            o[2 * b + 1] = one ? p : 0.0;
+        }
+    }
+};
+
 template <typename T>
 struct FNormAll {  // |a_n|^2
     const T* a;
@@ -470,7 +487,7 @@ using namespace qa;
 extern "C" {
 
 int qa_statevec_calcTotalProb(const void* amps, int prec, int nq, double* out) {
-    QA_REQUIRE_INIT();
+    QA_REQUIRE_INIT_RO();
     QA_FLUSH_ALL();
     if (!checkArgs(amps, prec, nq, __func__)) return QA_ERR_ARG;
     int rc;
@@ -478,8 +495,26 @@ int qa_statevec_calcTotalProb(const void* amps, int prec, int nq, double* out) {
     return rc;
 }
 
+int qa_statevec_calcProbsOfEveryQubit(const void* amps, int prec, int nq, double* out) {
+    QA_REQUIRE_INIT_RO();
+    QA_FLUSH_ALL();
+    if (!checkArgs(amps, prec, nq, __func__)) return QA_ERR_ARG;
+    QA_ARG_CHECK(out, "%s: NULL output", __func__);
+    for (int q0 = 0; q0 < nq; q0 += 8) {
+        double v[16];
+        int rc;
+        QA_DISPATCH(prec, rc = runReduceN<16>(FProbsOf8<T>{(const T*)amps, q0}, 1ull << nq, v));
+        if (rc != QA_OK) return rc;
+        for (int b = 0; b < 8 && q0 + b < nq; b++) {
+            out[2 * (q0 + b)] = v[2 * b];
+            out[2 * (q0 + b) + 1] = v[2 * b + 1];
+        }
+    }
+    return QA_OK;
+}
+
 int qa_densmatr_calcTotalProb(const void* amps, int prec, int N, double* out) {
-    QA_REQUIRE_INIT();
+    QA_REQUIRE_INIT_RO();
     QA_FLUSH_ALL();
     if (!checkArgs(amps, prec, 2 * N, __func__)) return QA_ERR_ARG;
     InsertSpec none;
@@ -492,7 +527,7 @@ int qa_densmatr_calcTotalProb(const void* amps, int prec, int N, double* out) {
 
 int qa_statevec_calcProbOfMultiQubitOutcome(const void* amps, int prec, int nq, const int* qubits,
                                             const int* outcomes, int num, double* out) {
-    QA_REQUIRE_INIT();
+    QA_REQUIRE_INIT_RO();
     QA_FLUSH_ALL();
     if (!checkArgs(amps, prec, nq, __func__)) return QA_ERR_ARG;
     InsertSpec ins;
@@ -504,7 +539,7 @@ int qa_statevec_calcProbOfMultiQubitOutcome(const void* amps, int prec, int nq, 
 
 int qa_densmatr_calcProbOfMultiQubitOutcome(const void* amps, int prec, int N, const int* qubits,
                                             const int* outcomes, int num, double* out) {
-    QA_REQUIRE_INIT();
+    QA_REQUIRE_INIT_RO();
     QA_FLUSH_ALL();
     if (!checkArgs(amps, prec, 2 * N, __func__)) return QA_ERR_ARG;
     InsertSpec ins;
@@ -516,7 +551,7 @@ int qa_densmatr_calcProbOfMultiQubitOutcome(const void* amps, int prec, int N, c
 
 int qa_statevec_calcProbsOfAllMultiQubitOutcomes(double* out, const void* amps, int prec, int nq,
                                                  const int* qubits, int num) {
-    QA_REQUIRE_INIT();
+    QA_REQUIRE_INIT_RO();
     QA_FLUSH_ALL();
     if (!checkArgs(amps, prec, nq, __func__)) return QA_ERR_ARG;
     InsertSpec chk;
@@ -529,7 +564,7 @@ int qa_statevec_calcProbsOfAllMultiQubitOutcomes(double* out, const void* amps, 
 
 int qa_densmatr_calcProbsOfAllMultiQubitOutcomes(double* out, const void* amps, int prec, int N,
                                                  const int* qubits, int num) {
-    QA_REQUIRE_INIT();
+    QA_REQUIRE_INIT_RO();
     QA_FLUSH_ALL();
     if (!checkArgs(amps, prec, 2 * N, __func__)) return QA_ERR_ARG;
     InsertSpec chk;
@@ -541,7 +576,7 @@ int qa_densmatr_calcProbsOfAllMultiQubitOutcomes(double* out, const void* amps, 
 }
 
 int qa_statevec_calcInnerProduct(const void* a, const void* b, int prec, int nq, double* out) {
-    QA_REQUIRE_INIT();
+    QA_REQUIRE_INIT_RO();
     QA_FLUSH_ALL();
     if (!checkArgs(a, prec, nq, __func__) || !b) return QA_ERR_ARG;
     int rc;
@@ -550,7 +585,7 @@ int qa_statevec_calcInnerProduct(const void* a, const void* b, int prec, int nq,
 }
 
 int qa_densmatr_calcHilbertSchmidtDistance(const void* a, const void* b, int prec, int N, double* out) {
-    QA_REQUIRE_INIT();
+    QA_REQUIRE_INIT_RO();
     QA_FLUSH_ALL();
     if (!checkArgs(a, prec, 2 * N, __func__) || !b) return QA_ERR_ARG;
     int rc;
@@ -560,7 +595,7 @@ int qa_densmatr_calcHilbertSchmidtDistance(const void* a, const void* b, int pre
 
 int qa_densmatr_calcFidelityWithPureState(const void* rho, const void* psi, int prec, int N, int conj,
                                           double* out) {
-    QA_REQUIRE_INIT();
+    QA_REQUIRE_INIT_RO();
     QA_FLUSH_ALL();
     if (!checkArgs(rho, prec, 2 * N, __func__) || !psi) return QA_ERR_ARG;
     int rc;
@@ -569,7 +604,7 @@ int qa_densmatr_calcFidelityWithPureState(const void* rho, const void* psi, int 
 }
 
 int qa_statevec_calcExpecAnyTargZ(const void* amps, int prec, int nq, uint64_t mask, double* out) {
-    QA_REQUIRE_INIT();
+    QA_REQUIRE_INIT_RO();
     QA_FLUSH_ALL();
     if (!checkArgs(amps, prec, nq, __func__)) return QA_ERR_ARG;
     int rc;
@@ -578,7 +613,7 @@ int qa_statevec_calcExpecAnyTargZ(const void* amps, int prec, int nq, uint64_t m
 }
 
 int qa_densmatr_calcExpecAnyTargZ(const void* amps, int prec, int N, uint64_t mask, double* out) {
-    QA_REQUIRE_INIT();
+    QA_REQUIRE_INIT_RO();
     QA_FLUSH_ALL();
     if (!checkArgs(amps, prec, 2 * N, __func__)) return QA_ERR_ARG;
     int rc;
@@ -596,7 +631,7 @@ static void timesPowerOfI(double* v, int k) {
 
 int qa_statevec_calcExpecPauliStr(const void* amps, int prec, int nq, uint64_t maskX, uint64_t maskY,
                                   uint64_t maskZ, double* out) {
-    QA_REQUIRE_INIT();
+    QA_REQUIRE_INIT_RO();
     QA_FLUSH_ALL();
     if (!checkArgs(amps, prec, nq, __func__)) return QA_ERR_ARG;
     int rc;
@@ -661,7 +696,7 @@ extern "C" {
 int qa_statevec_calcExpecPauliStrSum(const void* amps, int prec, int nq, int64_t numTerms, const uint64_t* maskX,
                                      const uint64_t* maskY, const uint64_t* maskZ, const double* coeffs,
                                      double* termValues, double* out) {
-    QA_REQUIRE_INIT();
+    QA_REQUIRE_INIT_RO();
     QA_FLUSH_ALL();
     if (!checkArgs(amps, prec, nq, __func__)) return QA_ERR_ARG;
     QA_ARG_CHECK(numTerms >= 0 && (numTerms == 0 || (maskX && maskY && maskZ && coeffs)),
@@ -715,7 +750,7 @@ int qa_statevec_calcExpecPauliStrSum(const void* amps, int prec, int nq, int64_t
 
 int qa_densmatr_calcExpecPauliStr(const void* amps, int prec, int N, uint64_t maskX, uint64_t maskY,
                                   uint64_t maskZ, double* out) {
-    QA_REQUIRE_INIT();
+    QA_REQUIRE_INIT_RO();
     QA_FLUSH_ALL();
     if (!checkArgs(amps, prec, 2 * N, __func__)) return QA_ERR_ARG;
     int rc;
@@ -727,7 +762,7 @@ int qa_densmatr_calcExpecPauliStr(const void* amps, int prec, int N, uint64_t ma
 
 int qa_statevec_calcExpecFullStateDiagMatr(const void* amps, int prec, int nq, const void* e, int powMode,
                                            double xr, double xi, double* out) {
-    QA_REQUIRE_INIT();
+    QA_REQUIRE_INIT_RO();
     QA_FLUSH_ALL();
     if (!checkArgs(amps, prec, nq, __func__) || !e) return QA_ERR_ARG;
     int rc;
@@ -738,7 +773,7 @@ int qa_statevec_calcExpecFullStateDiagMatr(const void* amps, int prec, int nq, c
 
 int qa_densmatr_calcExpecFullStateDiagMatr(const void* amps, int prec, int N, const void* e, int powMode,
                                            double xr, double xi, double* out) {
-    QA_REQUIRE_INIT();
+    QA_REQUIRE_INIT_RO();
     QA_FLUSH_ALL();
     if (!checkArgs(amps, prec, 2 * N, __func__) || !e) return QA_ERR_ARG;
     int rc;

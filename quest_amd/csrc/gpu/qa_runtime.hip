@@ -1,6 +1,7 @@
 // Device binding, memory, copies and timing of the kernel layer.
 // Replaces quest/src/gpu/gpu_config.cpp (device query/bind :124-353,
 // alloc :399-433, copies :449-606, cache :632-687) for one MI355X.
+#include <atomic>
 #include <cstdarg>
 #include <cstring>
 #include <vector>
@@ -59,6 +60,9 @@ void setError(const char* fmt, ...) {
 hipStream_t stream() { return g_stream; }
 bool isInit() { return g_init; }
 
+std::atomic<uint64_t> g_epoch{1};
+void bumpEpoch() { g_epoch.fetch_add(1, std::memory_order_relaxed); }
+
 double* devScratch(size_t n) {
     if (n > g_devScratchLen) {
         if (g_devScratch) (void)hipFree(g_devScratch);
@@ -99,7 +103,10 @@ int qa_isGpuAvailable(void) {
     return n > 0;
 }
 
+uint64_t qa_stateEpoch(void) { return qa::g_epoch.load(std::memory_order_relaxed); }
+
 int qa_init(int device) {
+    qa::bumpEpoch();
     if (g_init) {
         if (device == g_device || device < 0) return QA_OK;
         qa::setError("qa_init: already bound to device %d", g_device);
@@ -127,6 +134,7 @@ int qa_init(int device) {
 }
 
 int qa_finalize(void) {
+    qa::bumpEpoch();
     if (!g_init) return QA_OK;
     QA_FLUSH_ALL();
     QA_HIP(hipStreamSynchronize(g_stream));

@@ -306,8 +306,35 @@ qreal calcProbOfMultiQubitOutcome(Qureg q, int* qubits, int* outcomes, int n) {
     });
 }
 
+// calculations.cpp:241-262.  Callers ask this for qubit after qubit, so on
+// statevectors of at least 16 qubits the first call computes every qubit's
+// probabilities with one read of the state per 8 qubits and later calls
+// take them from a cache, valid while the kernel layer's state epoch is
+// unchanged (any entry point that may write amplitudes moves it)
 qreal calcProbOfQubitOutcome(Qureg q, int qubit, int outcome) {
-    return calcProbOfMultiQubitOutcome(q, &qubit, &outcome, 1);
+    if (q.isDensityMatrix || q.numQubits < 16) return calcProbOfMultiQubitOutcome(q, &qubit, &outcome, 1);
+    QAH_API({
+        validateTarget(q, qubit);
+        validateOutcome(outcome);
+        static struct {
+            const void* amps = nullptr;
+            uint64_t epoch = 0;
+            int numQubits = 0, prec = 0;
+            std::vector<double> p;
+        } cache;
+        const uint64_t epoch = qa_stateEpoch();
+        if (cache.amps != dev(q) || cache.epoch != epoch || cache.numQubits != q.numQubits ||
+            cache.prec != precOf(q)) {
+            cache.amps = nullptr;
+            cache.p.assign(2 * (size_t)q.numQubits, 0.0);
+            check(qa_statevec_calcProbsOfEveryQubit(dev(q), precOf(q), q.numQubits, cache.p.data()));
+            cache.amps = dev(q);
+            cache.epoch = epoch;
+            cache.numQubits = q.numQubits;
+            cache.prec = precOf(q);
+        }
+        return cache.p[2 * (size_t)qubit + (size_t)outcome];
+    });
 }
 
 void calcProbsOfAllMultiQubitOutcomes(qreal* out, Qureg q, int* qubits, int n) {

@@ -910,8 +910,11 @@ qreal calcProbOfMultiQubitOutcome(Qureg qureg, int* qubits, int* outcomes, int n
 int applyQubitMeasurementAndGetProb(Qureg q, int t, qreal* probability) {
     QAH_API({
         validateTarget(q, t);
-        double p0 = calcProbOfQubitOutcome(q, t, 0);
-        double p1 = calcProbOfQubitOutcome(q, t, 1);
+        // each probability from its own reduction (the reference's
+        // arithmetic), not from calcProbOfQubitOutcome's per-state cache
+        int zero = 0, one = 1;
+        double p0 = calcProbOfMultiQubitOutcome(q, &t, &zero, 1);
+        double p1 = calcProbOfMultiQubitOutcome(q, &t, &one, 1);
         if (validating() && std::abs(p0 + p1 - 1) > epsilonFor(q))
             fail("The state was unnormalised such that the probabilities of possible outcomes did not sum to one, and "
                  "ergo cannot be meaningfully sampled.");
@@ -930,7 +933,7 @@ qreal applyForcedQubitMeasurement(Qureg q, int t, int o) {
     QAH_API({
         validateTarget(q, t);
         validateOutcome(o);
-        double p = calcProbOfQubitOutcome(q, t, o);
+        double p = calcProbOfMultiQubitOutcome(q, &t, &o, 1);
         if (validating() && p <= epsilonFor(q)) fail("The forced measurement outcome has (near) zero probability.");
         projector(q, {t}, {o}, p);
         return p;

@@ -209,6 +209,48 @@ def test_measurement_is_seeded(api):
     assert outs[0] == outs[1]
 
 
+def test_qubit_probabilities_cache_follows_the_state(api):
+    """calcProbOfQubitOutcome on statevectors of >= 16 qubits computes every
+    qubit's probabilities at once and keeps them while no entry point has
+    written amplitudes: interleaved with gates, amplitude setters, a
+    measurement and a second state, every value must match the single
+    reduction (calcProbOfMultiQubitOutcome) and the oracle."""
+    from oracle import qoracle as O
+    n = 17
+    say("[gpu] qubit probability cache")
+    q = api.createQureg(n)
+    r = api.createQureg(n)
+    try:
+        init = qc.random_state(n, 41)
+        api.set_amps(q, init)
+        api.set_amps(r, qc.random_state(n, 42))
+
+        def check_all(state_amps):
+            st = O.State(n)
+            st.a[:] = state_amps
+            for t in range(n):
+                for o in (0, 1):
+                    got = api.calcProbOfQubitOutcome(q, t, o)
+                    single = api.calcProbOfMultiQubitOutcome(q, api.ints([t]), api.ints([o]), 1)
+                    assert abs(got - single) <= 1e-14, (t, o)
+                    assert abs(got - st.prob(t, o)) <= 1e-12, (t, o)
+
+        check_all(init)
+        api.applyHadamard(q, 3)
+        api.applyCompMatr1(q, 16, api.compmatr1(qc.haar_unitary(qc.SplitMix64(9), 2)))
+        check_all(api.get_amps(q))
+        api.calcProbOfQubitOutcome(r, 0, 0)  # another state in between
+        check_all(api.get_amps(q))
+        new = qc.random_state(n, 43)
+        api.set_amps(q, new)
+        check_all(new)
+        api.applyQubitMeasurement(q, 5)
+        check_all(api.get_amps(q))
+    finally:
+        api.destroyQureg(r)
+        api.destroyQureg(q)
+
+
 def test_qureg_size_limits(api):
     """Sizes at and past the limits are input errors, never a crash: no
     qubits, more index bits than one GPU addresses, and a state larger than

@@ -262,7 +262,7 @@ int qa_densmatr_partialTrace(const void* inAmps, int prec, int inNumQubitsN, voi
 
 int qa_statevec_calcTotalProb(const void* amps, int prec, int numQubits, double* out);
 int qa_densmatr_calcTotalProb(const void* amps, int prec, int numQubitsN, double* out);
-/* every qubit's outcome probabilities in one read of the state per 8 qubits
+/* every qubit's outcome probabilities in one read of the state per 15 qubits
    (gpu_statevec_calcProbOfMultiQubitOutcome_sub for each single qubit, as
    calcProbOfQubitOutcome asks it): out[2q] = P(q = 0), out[2q+1] = P(q = 1) */
 int qa_statevec_calcProbsOfEveryQubit(const void* amps, int prec, int numQubits, double* out);

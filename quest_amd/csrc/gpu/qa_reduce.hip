@@ -121,20 +121,17 @@ __device__ __forceinline__ double norm2(cd a) { return a.re * a.re + a.im * a.im
 
 // ---- term functors --------------------------------------------------------
 
-// |a_n|^2 split by the value of each of the 8 qubits q0..q0+7:
-// o[2b + bit] for qubit q0 + b
+// |a_n|^2 where each of the 15 qubits q0..q0+14 is 0 (o[b] for qubit
+// q0 + b), and the total (o[15]); P(q = 1) is then the total minus P(q = 0)
 template <typename T>
-struct FProbsOf8 {
+struct FProbsOf15 {
     const T* a;
     int q0;
     __device__ void operator()(uint64_t n, double* o) const {
         const double p = norm2(ldd(a, n));
 #pragma unroll
-        for (int b = 0; b < 8; b++) {
-            const bool one = (n >> (q0 + b)) & 1ull;
-            o[2 * b] = one ? 0.0 : p;
-            o[2 * b + 1] = one ? p : 0.0;
-        }
+        for (int b = 0; b < 15; b++) o[b] = ((n >> (q0 + b)) & 1ull) ? 0.0 : p;
+        o[15] = p;
     }
 };
 
@@ -500,14 +497,14 @@ int qa_statevec_calcProbsOfEveryQubit(const void* amps, int prec, int nq, double
     QA_FLUSH_ALL();
     if (!checkArgs(amps, prec, nq, __func__)) return QA_ERR_ARG;
     QA_ARG_CHECK(out, "%s: NULL output", __func__);
-    for (int q0 = 0; q0 < nq; q0 += 8) {
+    for (int q0 = 0; q0 < nq; q0 += 15) {
         double v[16];
         int rc;
-        QA_DISPATCH(prec, rc = runReduceN<16>(FProbsOf8<T>{(const T*)amps, q0}, 1ull << nq, v));
+        QA_DISPATCH(prec, rc = runReduceN<16>(FProbsOf15<T>{(const T*)amps, q0}, 1ull << nq, v));
         if (rc != QA_OK) return rc;
-        for (int b = 0; b < 8 && q0 + b < nq; b++) {
-            out[2 * (q0 + b)] = v[2 * b];
-            out[2 * (q0 + b) + 1] = v[2 * b + 1];
+        for (int b = 0; b < 15 && q0 + b < nq; b++) {
+            out[2 * (q0 + b)] = v[b];
+            out[2 * (q0 + b) + 1] = v[15] - v[b];
         }
     }
     return QA_OK;

@@ -427,7 +427,7 @@ def main() -> int:
     ap.add_argument("--circuits", type=int, default=4096, help="serve mode: number of requests")
     ap.add_argument("--arena-gib", type=int, default=48, help="serve mode: HBM arena of the engine")
     ap.add_argument("--max-active", type=int, default=256, help="serve mode: requests running at once")
-    ap.add_argument("--tile", type=int, default=12, help="serve mode: tile qubits")
+    ap.add_argument("--tile", type=int, default=13, help="serve mode: tile qubits")
     ap.add_argument("--baseline-sample", type=int, default=24, help="serve mode: requests run sequentially")
     ap.add_argument("--mode", choices=["circuit", "density", "serve", "precision", "sweep"], default="circuit",
                     help="circuit: the headline 30-qubit statevector workload; density: the 14-qubit noisy "

@@ -262,9 +262,10 @@ int qa_densmatr_partialTrace(const void* inAmps, int prec, int inNumQubitsN, voi
 
 int qa_statevec_calcTotalProb(const void* amps, int prec, int numQubits, double* out);
 int qa_densmatr_calcTotalProb(const void* amps, int prec, int numQubitsN, double* out);
-/* every qubit's outcome probabilities in one read of the state per 15 qubits
+/* every qubit's outcome probabilities in one read of the state per 10 qubits
    (gpu_statevec_calcProbOfMultiQubitOutcome_sub for each single qubit, as
-   calcProbOfQubitOutcome asks it): out[2q] = P(q = 0), out[2q+1] = P(q = 1) */
+   calcProbOfQubitOutcome asks it): out[2q] = P(q = 0), out[2q+1] = P(q = 1),
+   each the compensated sum of its own amplitudes' |a|^2 */
 int qa_statevec_calcProbsOfEveryQubit(const void* amps, int prec, int numQubits, double* out);
 int qa_statevec_calcProbOfMultiQubitOutcome(const void* amps, int prec, int numQubits,
                                             const int* qubits, const int* outcomes, int num,

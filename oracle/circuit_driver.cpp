@@ -75,7 +75,8 @@ int main(int argc, char** argv) {
 
     initQuESTEnv();
     Qureg q{};
-    bool have = false;
+    Qureg aux{};
+    bool have = false, haveAux = false;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     long lineNo = 0;
     string line;
@@ -191,6 +192,38 @@ int main(int argc, char** argv) {
         } else if (op == "MEAS") {
             int a = t.i(); int o = applyQubitMeasurement(q, a);
             std::printf("R meas %d\n", o);
+        } else if (op == "MEASN") {
+            int nq = t.i(); auto qs = t.ints(nq);
+            std::printf("R measn %lld\n", (long long) applyMultiQubitMeasurement(q, qs.data(), nq));
+        } else if (op == "RANDPURE" || op == "RANDMIXED") {
+            // a scratch qureg: only the RNG stream it advances is observed
+            int n = t.i(); long long k = t.ll();
+            Qureg s = (op == "RANDMIXED" || k) ? createDensityQureg(n) : createQureg(n);
+            if (op == "RANDPURE") initRandomPureState(s);
+            else initRandomMixedState(s, k);
+            destroyQureg(s);
+        } else if (op == "AUXFILE" || op == "AUXCOPY") {
+            // the auxiliary qureg of the two-state calculations: amplitudes
+            // from a file, or a clone of the main qureg
+            if (haveAux) destroyQureg(aux);
+            if (op == "AUXFILE") {
+                int n = t.i(); int dm = t.i(); string path = t.s();
+                aux = dm ? createDensityQureg(n) : createQureg(n);
+                std::ifstream bin(path, std::ios::binary);
+                vector<qcomp> amps(aux.numAmps);
+                bin.read(reinterpret_cast<char*>(amps.data()), (std::streamsize)(amps.size() * sizeof(qcomp)));
+                if (!bin) { std::fprintf(stderr, "short read %s\n", path.c_str()); return 3; }
+                if (dm) setDensityQuregFlatAmps(aux, 0, amps.data(), aux.numAmps);
+                else setQuregAmps(aux, 0, amps.data(), aux.numAmps);
+            } else {
+                aux = createCloneQureg(q);
+            }
+            haveAux = true;
+        } else if (op == "OUT_INNER") {
+            qcomp z = calcInnerProduct(q, aux);
+            std::printf("R inner %.17g %.17g\n", std::real(z), std::imag(z));
+        } else if (op == "OUT_FIDELITY") { printReal("fidelity", calcFidelity(q, aux));
+        } else if (op == "OUT_DISTANCE") { printReal("distance", calcDistance(q, aux));
         } else if (op == "OUT_PROB") {
             int a = t.i(); int o = t.i(); printReal("prob", calcProbOfQubitOutcome(q, a, o));
         } else if (op == "OUT_EXPEC") {
@@ -230,6 +263,7 @@ int main(int argc, char** argv) {
         }
         std::fflush(stdout);
     }
+    if (haveAux) destroyQureg(aux);
     if (have) destroyQureg(q);
     finalizeQuESTEnv();
     return 0;

@@ -487,6 +487,53 @@ void qo_dm_expecPauli(double* out, const qo_c* a, int N, uint64_t mx, uint64_t m
     out[1] = cimag(v);
 }
 
+/* cpu_densmatr_calcProbsOfAllMultiQubitOutcomes_sub (:1866-1903): the
+   diagonal element of every column, binned by the outcome of its row */
+void qo_dm_probsAllOutcomes(double* out, const qo_c* a, int nq, const int* qubits, int num) {
+    long long nb = 1ll << num;
+    for (long long b = 0; b < nb; b++) out[b] = 0;
+    long long dim = 1ll << nq;
+    for (long long n = 0; n < dim; n++) out[value_of_bits((uint64_t)n, qubits, num)] += creal(a[n * (dim + 1)]);
+}
+
+/* cpu_densmatr_calcHilbertSchmidtDistance_sub (:1940-1953): sum |A-B|^2 */
+double qo_dm_hsDistanceSq(const qo_c* a, const qo_c* b, int nq) {
+    double d = 0;
+    long long num = 1ll << (2 * nq);
+#pragma omp parallel for reduction(+ : d)
+    for (long long n = 0; n < num; n++) {
+        qo_c t = a[n] - b[n];
+        d += creal(t) * creal(t) + cimag(t) * cimag(t);
+    }
+    return d;
+}
+
+/* cpu_densmatr_calcFidelityWithPureState_sub (:1956-1994): <psi|rho|psi>,
+   or <psi|rho^dagger|psi> when conjugated */
+void qo_dm_fidelityPure(double* out, const qo_c* rho, const qo_c* psi, int nq, int conjugate) {
+    double re = 0, im = 0;
+    long long num = 1ll << (2 * nq);
+    long long dim = 1ll << nq;
+#pragma omp parallel for reduction(+ : re, im)
+    for (long long i = 0; i < num; i++) {
+        long long r = i & (dim - 1), c = i >> nq;
+        qo_c rhoAmp = rho[i], rowAmp = psi[r], colAmp = psi[c];
+        if (conjugate) {
+            rhoAmp = conj(rhoAmp);
+            colAmp = conj(colAmp);
+        } else {
+            rowAmp = conj(rowAmp);
+        }
+        qo_c t = rhoAmp * rowAmp * colAmp;
+        re += creal(t);
+        im += cimag(t);
+    }
+    out[0] = re;
+    out[1] = im;
+}
+
+/* cpu_statevec_calcInnerProduct_sub (:1919-1937): sum conj(a) b; density
+   matrices use it on their flat amplitudes (Tr(A^dagger B)) */
 void qo_innerProduct(double* out, const qo_c* a, const qo_c* b, int nq) {
     double re = 0, im = 0;
     long long num = 1ll << nq;

@@ -53,6 +53,9 @@ double qo_expecZ(const qo_c* a, int nq, uint64_t mask);
 void qo_expecPauli(double* out2, const qo_c* a, int nq, uint64_t mx, uint64_t my, uint64_t mz);
 void qo_dm_expecPauli(double* out2, const qo_c* a, int N, uint64_t mx, uint64_t my, uint64_t mz);
 void qo_innerProduct(double* out2, const qo_c* a, const qo_c* b, int nq);
+void qo_dm_probsAllOutcomes(double* out, const qo_c* a, int nq, const int* qubits, int num);
+double qo_dm_hsDistanceSq(const qo_c* a, const qo_c* b, int nq);
+void qo_dm_fidelityPure(double* out2, const qo_c* rho, const qo_c* psi, int nq, int conjugate);
 
 /* initialisation */
 void qo_initDebug(qo_c* a, int nq);

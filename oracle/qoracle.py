@@ -56,6 +56,7 @@ def _load():
         "qo_probsAllOutcomes": [P, P, I, IP, I],
         "qo_expecPauli": [P, P, I, U, U, U], "qo_dm_expecPauli": [P, P, I, U, U, U],
         "qo_innerProduct": [P, P, P, I], "qo_initDebug": [P, I],
+        "qo_dm_probsAllOutcomes": [P, P, I, IP, I], "qo_dm_fidelityPure": [P, P, P, I, I],
     }
     for name, args in sigs.items():
         f = getattr(lib, name)
@@ -63,7 +64,7 @@ def _load():
         f.restype = None
     for name, args in {"qo_totalProb": [P, I], "qo_dm_totalProb": [P, I],
                        "qo_probOutcome": [P, I, IP, IP, I], "qo_dm_probOutcome": [P, I, IP, IP, I],
-                       "qo_expecZ": [P, I, U]}.items():
+                       "qo_expecZ": [P, I, U], "qo_dm_hsDistanceSq": [P, P, I]}.items():
         f = getattr(lib, name)
         f.argtypes = args
         f.restype = C.c_double
@@ -339,6 +340,43 @@ class State:
         p = lib().qo_totalProb(_p(self.a), self.bits)
         return p if self.dens else p * p
 
+    # calculations.cpp:49-61: <A|B>, or Tr(A^dagger B) of two density
+    # matrices (their flat inner product); with one density matrix rho and
+    # one statevector psi, <psi|rho^dagger|psi> (rho first) or <psi|rho|psi>
+    def _pure_fid(self, rho, psi, conjugate):
+        out = np.zeros(2)
+        lib().qo_dm_fidelityPure(_p(out), _p(rho), _p(psi), self.n, int(conjugate))
+        return complex(out[0], out[1])
+
+    def inner(self, other, other_dens):
+        if self.dens == other_dens:
+            out = np.zeros(2)
+            lib().qo_innerProduct(_p(out), _p(self.a), _p(other), self.bits)
+            return complex(out[0], out[1])
+        if self.dens:
+            return self._pure_fid(self.a, other, True)
+        return self._pure_fid(other, self.a, False)
+
+    # calculations.cpp:303-328: |<A|B>|^2, or Re<psi|rho|psi>
+    def fidelity(self, other, other_dens):
+        if self.dens and other_dens:
+            raise ValueError("fidelity of two density matrices is not defined by the reference")
+        if not self.dens and not other_dens:
+            return abs(self.inner(other, other_dens)) ** 2
+        rho, psi = (self.a, other) if self.dens else (other, self.a)
+        return self._pure_fid(rho, psi, False).real
+
+    # calculations.cpp:331-364: Hilbert-Schmidt, Bures or purified distance
+    def distance(self, other, other_dens):
+        if self.dens and other_dens:
+            return math.sqrt(lib().qo_dm_hsDistanceSq(_p(self.a), _p(other), self.n))
+        if not self.dens and not other_dens:
+            mag = min(abs(self.inner(other, other_dens)), 1.0)
+            return math.sqrt(2 - 2 * mag)
+        rho, psi = (self.a, other) if self.dens else (other, self.a)
+        re = min(self._pure_fid(rho, psi, False).real, 1.0)
+        return math.sqrt(1 - re)
+
     def total_prob(self):
         if self.dens:
             return lib().qo_dm_totalProb(_p(self.a), self.n)
@@ -456,10 +494,24 @@ def apply_op(st: State, op, results: list) -> None:
     elif k == "OUT_PROBS":
         out = np.zeros(1 << len(op[1]))
         if st.dens:
-            raise NotImplementedError("density-matrix outcome tables are checked through OUT_PROB")
-        lib().qo_probsAllOutcomes(_p(out), _p(st.a), st.n, _ip(op[1]), len(op[1]))
+            lib().qo_dm_probsAllOutcomes(_p(out), _p(st.a), st.n, _ip(op[1]), len(op[1]))
+        else:
+            lib().qo_probsAllOutcomes(_p(out), _p(st.a), st.n, _ip(op[1]), len(op[1]))
         results.append(("probs", list(out)))
-    elif k in ("SEEDS", "MEAS"):
+    elif k in ("AUX", "AUXCOPY"):
+        if k == "AUX":
+            from quest_amd.circuits import random_state
+            st.aux = (np.ascontiguousarray(random_state(op[1], op[3], op[2]), dtype=np.complex128), op[2])
+        else:
+            st.aux = (st.a.copy(), st.dens)
+    elif k == "OUT_INNER":
+        z = st.inner(*st.aux)
+        results.append(("inner", [z.real, z.imag]))
+    elif k == "OUT_FIDELITY":
+        results.append(("fidelity", st.fidelity(*st.aux)))
+    elif k == "OUT_DISTANCE":
+        results.append(("distance", st.distance(*st.aux)))
+    elif k in ("SEEDS", "MEAS", "MEASN", "RANDPURE", "RANDMIXED"):
         raise NotImplementedError("measurement sampling is checked against reference fixtures directly")
     else:
         raise ValueError(f"unknown op {k}")
@@ -477,7 +529,20 @@ def run_reference(circuit, workdir: Path, init_amps: np.ndarray | None = None, w
         np.ascontiguousarray(amps, dtype=np.complex128).tofile(init_path)
     out_path = workdir / f"{circuit.name}_out.bin" if want_amps else None
     txt = workdir / f"{circuit.name}.txt"
-    txt.write_text(to_text(circuit, str(init_path) if init_path else None, str(out_path) if out_path else None))
+    text = to_text(circuit, str(init_path) if init_path else None, str(out_path) if out_path else None)
+    # AUX n dm seed -> AUXFILE n dm path: the driver reads the auxiliary
+    # qureg's amplitudes (the project's seeded random state) from a file
+    lines = []
+    for i, line in enumerate(text.splitlines()):
+        t = line.split()
+        if t and t[0] == "AUX":
+            from quest_amd.circuits import random_state
+            n, dm, seed = int(t[1]), bool(int(t[2])), int(t[3])
+            path = workdir / f"{circuit.name}_aux{i}.bin"
+            np.ascontiguousarray(random_state(n, seed, dm), dtype=np.complex128).tofile(path)
+            line = f"AUXFILE {n} {int(dm)} {path}"
+        lines.append(line)
+    txt.write_text("\n".join(lines) + "\n")
     proc = subprocess.run([str(REF_DRIVER), str(txt)], capture_output=True, text=True, check=True)
     amps_out = np.fromfile(out_path, dtype=np.complex128) if out_path else None
     return amps_out, parse_results(proc.stdout), proc.stdout

@@ -256,6 +256,8 @@ _SIGS = {
     "applyQubitProjector": (None, [Qureg, cint, cint]),
     "applyQubitMeasurement": (cint, [Qureg, cint]),
     "applyForcedQubitMeasurement": (qreal, [Qureg, cint, cint]),
+    "applyMultiQubitMeasurement": (qindex, [Qureg, IntP, cint]),
+    "applyForcedMultiQubitMeasurement": (qreal, [Qureg, IntP, IntP, cint]),
     # decoherence
     "mixDephasing": (None, [Qureg, cint, qreal]), "mixTwoQubitDephasing": (None, [Qureg, cint, cint, qreal]),
     "mixDepolarising": (None, [Qureg, cint, qreal]),

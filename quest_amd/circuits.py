@@ -228,6 +228,43 @@ def variational(n: int, depth: int, seed: int, *, num_expec: int = 8) -> Circuit
     return c
 
 
+def _dagger(m):
+    return [[complex(m[c][r]).conjugate() for c in range(len(m))] for r in range(len(m))]
+
+
+def inverse_ops(c: Circuit) -> list:
+    """The gates of c undone, in reverse order: matrices by their adjoints,
+    phase shifts and Pauli gadgets by their negated angles, a full QFT by its
+    gate sequence (operations.cpp:1934-1953) reversed and conjugated.
+    Outputs are dropped; only unitary ops are supported."""
+    out = []
+    for op in reversed(c.ops):
+        k = op[0]
+        if k.startswith("OUT_"):
+            continue
+        if k == "U1":
+            out.append(("U1", op[1], _dagger(op[2])))
+        elif k == "CU1":
+            out.append(("CU1", op[1], op[2], _dagger(op[3])))
+        elif k == "PS2":
+            out.append(("PS2", op[1], op[2], -op[3]))
+        elif k == "PG":
+            out.append(("PG", op[1], op[2], -op[3]))
+        elif k == "FQFT":
+            n = c.num_qubits
+            fwd = []
+            for t in range(n - 1, -1, -1):
+                fwd.append(("H", t))
+                for m in range(t):
+                    fwd.append(("PS2", t, t - m - 1, math.pi / (1 << (m + 1))))
+            fwd += [("SWAP", t, n - 1 - t) for t in range(n // 2)]
+            for g in reversed(fwd):
+                out.append(("PS2", g[1], g[2], -g[3]) if g[0] == "PS2" else g)
+        else:
+            raise ValueError(f"inverse_ops: unsupported op {k}")
+    return out
+
+
 # ------------------------------------------------------------ text format
 
 def _fmt(x: float) -> str:
@@ -302,6 +339,16 @@ def op_to_line(op) -> str:
         return f"OUT_EXPEC {op[1]} {len(op[2])} {' '.join(map(str, op[2]))}"
     if k == "OUT_PROBS":
         return f"OUT_PROBS {len(op[1])} {' '.join(map(str, op[1]))}"
+    if k == "MEASN":
+        return f"MEASN {len(op[1])} {' '.join(map(str, op[1]))}"
+    if k == "RANDPURE":
+        return f"RANDPURE {op[1]} {int(op[2])}"
+    if k == "RANDMIXED":
+        return f"RANDMIXED {op[1]} {op[2]}"
+    if k == "AUX":
+        return f"AUX {op[1]} {int(op[2])} {op[3]}"
+    if k in ("AUXCOPY", "OUT_INNER", "OUT_FIDELITY", "OUT_DISTANCE"):
+        return k
     raise ValueError(f"unknown op {k}")
 
 
@@ -433,9 +480,17 @@ def from_text(text: str) -> Circuit:
         elif k == "SEEDS":
             ns = int(take()[0])
             c.ops.append(("SEEDS", ints(ns)))
-        elif k == "OUT_PROBS":
+        elif k in ("OUT_PROBS", "MEASN"):
             nq = int(take()[0])
-            c.ops.append(("OUT_PROBS", ints(nq)))
+            c.ops.append((k, ints(nq)))
+        elif k == "RANDPURE":
+            c.ops.append((k, int(t[1]), bool(int(t[2]))))
+        elif k == "RANDMIXED":
+            c.ops.append((k, int(t[1]), int(t[2])))
+        elif k == "AUX":
+            c.ops.append((k, int(t[1]), bool(int(t[2])), int(t[3])))
+        elif k in ("AUXCOPY", "OUT_INNER", "OUT_FIDELITY", "OUT_DISTANCE"):
+            c.ops.append((k,))
         else:
             raise ValueError(f"unknown op line: {line}")
     return c
@@ -468,14 +523,21 @@ def run(c: Circuit, q=None, record: list | None = None, api=None):
     elif init == "random":
         api.set_amps(q, initial_amps(c))
     results = [] if record is None else record
-    for op in c.ops:
-        apply_op(api, q, op, results)
+    ctx = {}
+    try:
+        for op in c.ops:
+            apply_op(api, q, op, results, ctx)
+    finally:
+        if "aux" in ctx:
+            api.destroyQureg(ctx.pop("aux"))
     return q, results
 
 
-def apply_op(api, q, op, results: list) -> None:
+def apply_op(api, q, op, results: list, ctx: dict | None = None) -> None:
+    """One op on qureg q; ctx holds the auxiliary qureg of AUX/AUXCOPY."""
     k = op[0]
     A = api
+    ctx = {} if ctx is None else ctx
     if k == "U1":
         A.applyCompMatr1(q, op[1], A.compmatr1(op[2]))
     elif k == "CU1":
@@ -568,6 +630,33 @@ def apply_op(api, q, op, results: list) -> None:
         results.append(("totalprob", A.calcTotalProb(q)))
     elif k == "OUT_PROBS":
         results.append(("probs", list(A.probs_of_all_outcomes(q, op[1]))))
+    elif k == "MEASN":
+        results.append(("measn", A.applyMultiQubitMeasurement(q, A.ints(op[1]), len(op[1]))))
+    elif k in ("RANDPURE", "RANDMIXED"):
+        # a scratch qureg: only the measurement RNG stream it advances matters
+        s = A.createDensityQureg(op[1]) if (k == "RANDMIXED" or op[2]) else A.createQureg(op[1])
+        try:
+            if k == "RANDPURE":
+                A.initRandomPureState(s)
+            else:
+                A.initRandomMixedState(s, op[2])
+        finally:
+            A.destroyQureg(s)
+    elif k in ("AUX", "AUXCOPY"):
+        if "aux" in ctx:
+            A.destroyQureg(ctx.pop("aux"))
+        if k == "AUX":
+            ctx["aux"] = A.createDensityQureg(op[1]) if op[2] else A.createQureg(op[1])
+            A.set_amps(ctx["aux"], random_state(op[1], op[3], op[2]))
+        else:
+            ctx["aux"] = A.createCloneQureg(q)
+    elif k == "OUT_INNER":
+        z = A.inner_product(q, ctx["aux"])
+        results.append(("inner", [z.real, z.imag]))
+    elif k == "OUT_FIDELITY":
+        results.append(("fidelity", A.calcFidelity(q, ctx["aux"])))
+    elif k == "OUT_DISTANCE":
+        results.append(("distance", A.calcDistance(q, ctx["aux"])))
     else:
         raise ValueError(f"unknown op {k}")
 
@@ -580,10 +669,10 @@ def parse_results(text: str) -> list:
         if not parts or parts[0] != "R":
             continue
         label = parts[1]
-        if label == "meas":
-            out.append(("meas", int(parts[2])))
-        elif label == "probs":
-            out.append(("probs", [float(v) for v in parts[2:]]))
+        if label in ("meas", "measn"):
+            out.append((label, int(parts[2])))
+        elif label in ("probs", "inner"):
+            out.append((label, [float(v) for v in parts[2:]]))
         else:
             out.append((label, float(parts[2])))
     return out
@@ -632,7 +721,7 @@ def count_gates(c: Circuit) -> int:
     total = 0
     for op in c.ops:
         k = op[0]
-        if k.startswith("OUT_") or k in ("SEEDS", "MEAS", "DEPOL", "DEPOL2", "DEPH", "DEPH2", "DAMP", "PAULIS",
+        if k.startswith("OUT_") or k in ("SEEDS", "MEAS", "MEASN", "RANDPURE", "RANDMIXED", "AUX", "AUXCOPY", "DEPOL", "DEPOL2", "DEPH", "DEPH2", "DAMP", "PAULIS",
                                          "KRAUS"):
             continue
         if k == "FQFT":

@@ -237,6 +237,7 @@ bool isInit();
 void bumpEpoch();
 // reduction scratch: device partials and a pinned host landing area
 double* devScratch(size_t numDoubles);
+void trimDevScratch();  // frees the device partials if they grew past 32 MiB
 double* hostScratch(size_t numDoubles);
 
 }  // namespace qa

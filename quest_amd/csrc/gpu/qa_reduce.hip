@@ -121,17 +121,23 @@ __device__ __forceinline__ double norm2(cd a) { return a.re * a.re + a.im * a.im
 
 // ---- term functors --------------------------------------------------------
 
-// |a_n|^2 where each of the 15 qubits q0..q0+14 is 0 (o[b] for qubit
-// q0 + b), and the total (o[15]); P(q = 1) is then the total minus P(q = 0)
+// both outcome probabilities of PQ qubits q0..q0+PQ-1 in one read of the
+// state: |a_n|^2 goes to o[b] when qubit q0 + b is 0 in n, to o[PQ + b]
+// when it is 1, so each outcome is the compensated sum of its own terms
+// (never a difference of two sums, which would lose a small probability)
+constexpr int PQ = 10;
 template <typename T>
-struct FProbsOf15 {
+struct FProbsOfQubits {
     const T* a;
     int q0;
     __device__ void operator()(uint64_t n, double* o) const {
         const double p = norm2(ldd(a, n));
 #pragma unroll
-        for (int b = 0; b < 15; b++) o[b] = ((n >> (q0 + b)) & 1ull) ? 0.0 : p;
-        o[15] = p;
+        for (int b = 0; b < PQ; b++) {
+            const bool one = (n >> (q0 + b)) & 1ull;
+            o[b] = one ? 0.0 : p;
+            o[PQ + b] = one ? p : 0.0;
+        }
     }
 };
 
@@ -372,20 +378,44 @@ __global__ void __launch_bounds__(BLOCK) k_finalizeBins(const double* __restrict
     out[b] = acc.s + acc.c;
 }
 
+// outcome tables of more than 4 qubits: thread (b, c) sums, compensated and
+// in index order, the terms of outcome b whose other bits r satisfy
+// r = c (mod R), so consecutive threads read consecutive amplitudes; the R
+// partial sums of each outcome are then merged in order of c.  The order of
+// every sum depends on the sizes only, so tables are deterministic.  With
+// R = 1 the thread writes its outcome's probability directly.
 template <typename T>
-__global__ void __launch_bounds__(BLOCK) k_allOutcomesAtomic(const T* __restrict__ a, uint64_t numTerms,
-                                                            TargSpec qs, uint64_t diagStride,
-                                                            double* __restrict__ bins) {
-    const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
-    for (uint64_t n = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; n < numTerms; n += stride) {
-        double p = diagStride ? ldd(a, n * diagStride).re : norm2(ldd(a, n));
-        atomicAdd(&bins[valueOfBits(n, qs)], p);
+__global__ void __launch_bounds__(BLOCK) k_allOutcomesSplit(const T* __restrict__ a, uint64_t restTerms,
+                                                            InsertSpec rest, TargSpec qs, uint64_t diagStride,
+                                                            int logR, uint64_t numThreads,
+                                                            double* __restrict__ dst) {
+    const uint64_t t = blockId() * BLOCK + threadIdx.x;
+    if (t >= numThreads) return;
+    const uint64_t R = 1ull << logR;
+    const uint64_t b = t >> logR, c = t & (R - 1);
+    uint64_t bits = 0;
+    for (int i = 0; i < qs.num; i++) bits |= ((b >> i) & 1ull) << qs.targ[i];
+    KB acc = {0, 0};
+    for (uint64_t r = c; r < restTerms; r += R) {
+        const uint64_t n = insertBits(r, rest) | bits;
+        kadd(acc, diagStride ? ldd(a, n * diagStride).re : norm2(ldd(a, n)));
+    }
+    if (logR == 0) {
+        dst[t] = acc.s + acc.c;
+    } else {
+        dst[2 * t] = acc.s;
+        dst[2 * t + 1] = acc.c;
     }
 }
 
-__global__ void k_zero(double* __restrict__ p, uint64_t n) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0.0;
+__global__ void __launch_bounds__(BLOCK) k_mergeSplitBins(const double* __restrict__ partials, int logR, uint64_t nb,
+                                                          double* __restrict__ out) {
+    const uint64_t b = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (b >= nb) return;
+    const uint64_t R = 1ull << logR;
+    KB acc = {0, 0};
+    for (uint64_t c = 0; c < R; c++) kmerge(acc, KB{partials[2 * (b * R + c)], partials[2 * (b * R + c) + 1]});
+    out[b] = acc.s + acc.c;
 }
 
 template <typename T>
@@ -419,13 +449,33 @@ int allOutcomes(double* out, const T* a, uint64_t numTerms, const int* qubits, i
         for (uint64_t b = 0; b < nb; b++) out[b] = host[b];
         return QA_OK;
     }
-    double* bins = devScratch((size_t)nb);
-    if (!bins) return QA_ERR_OOM;
-    k_zero<<<(unsigned)((nb + 255) / 256), 256, 0, stream()>>>(bins, nb);
-    k_allOutcomesAtomic<T><<<blocks * 4, BLOCK, 0, stream()>>>(a, numTerms, ts, diagStride, bins);
+    // about 2^20 threads: R partial sums per outcome
+    const int logNb = num;
+    int logTerms = 0;
+    while ((1ull << logTerms) < numTerms) logTerms++;
+    const int logRest = logTerms - num;
+    const int logR = std::max(0, std::min(logRest, 20 - logNb));
+    const uint64_t threads = nb << logR;
+    // the other bits of a term's index: zeros inserted at the table's qubits
+    InsertSpec rest;
+    rest.num = num;
+    rest.valueMask = 0;
+    for (int i = 0; i < num; i++) rest.pos[i] = qubits[i];
+    std::sort(rest.pos, rest.pos + num);
+    const size_t partialDoubles = logR ? 2 * (size_t)threads : 0;
+    double* scratch = devScratch(partialDoubles + (size_t)nb);
+    if (!scratch) return QA_ERR_OOM;
+    double* res = scratch + partialDoubles;
+    k_allOutcomesSplit<T><<<gridFor(threads, 1), BLOCK, 0, stream()>>>(a, 1ull << logRest, rest, ts, diagStride, logR,
+                                                                      threads, logR ? scratch : res);
     QA_LAUNCH_CHECK();
-    QA_HIP(hipMemcpyAsync(out, bins, nb * sizeof(double), hipMemcpyDeviceToHost, stream()));
+    if (logR) {
+        k_mergeSplitBins<<<(unsigned)((nb + BLOCK - 1) / BLOCK), BLOCK, 0, stream()>>>(scratch, logR, nb, res);
+        QA_LAUNCH_CHECK();
+    }
+    QA_HIP(hipMemcpyAsync(out, res, nb * sizeof(double), hipMemcpyDeviceToHost, stream()));
     QA_HIP(hipStreamSynchronize(stream()));
+    trimDevScratch();
     return QA_OK;
 }
 
@@ -497,14 +547,14 @@ int qa_statevec_calcProbsOfEveryQubit(const void* amps, int prec, int nq, double
     QA_FLUSH_ALL();
     if (!checkArgs(amps, prec, nq, __func__)) return QA_ERR_ARG;
     QA_ARG_CHECK(out, "%s: NULL output", __func__);
-    for (int q0 = 0; q0 < nq; q0 += 15) {
-        double v[16];
+    for (int q0 = 0; q0 < nq; q0 += PQ) {
+        double v[2 * PQ];
         int rc;
-        QA_DISPATCH(prec, rc = runReduceN<16>(FProbsOf15<T>{(const T*)amps, q0}, 1ull << nq, v));
+        QA_DISPATCH(prec, rc = runReduceN<2 * PQ>(FProbsOfQubits<T>{(const T*)amps, q0}, 1ull << nq, v));
         if (rc != QA_OK) return rc;
-        for (int b = 0; b < 15 && q0 + b < nq; b++) {
+        for (int b = 0; b < PQ && q0 + b < nq; b++) {
             out[2 * (q0 + b)] = v[b];
-            out[2 * (q0 + b) + 1] = v[15] - v[b];
+            out[2 * (q0 + b) + 1] = v[PQ + b];
         }
     }
     return QA_OK;

@@ -77,6 +77,18 @@ double* devScratch(size_t n) {
     return g_devScratch;
 }
 
+// a reduction that needed an unusually large scratch (an outcome table of
+// many qubits) gives it back rather than hold it until the library ends
+void trimDevScratch() {
+    constexpr size_t KEEP = 1u << 22;  // 32 MiB
+    if (g_devScratchLen > KEEP) {
+        (void)hipStreamSynchronize(stream());
+        (void)hipFree(g_devScratch);
+        g_devScratch = nullptr;
+        g_devScratchLen = 0;
+    }
+}
+
 double* hostScratch(size_t n) {
     if (n > g_hostScratchLen) {
         if (g_hostScratch) (void)hipHostFree(g_hostScratch);

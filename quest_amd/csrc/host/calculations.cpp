@@ -308,7 +308,7 @@ qreal calcProbOfMultiQubitOutcome(Qureg q, int* qubits, int* outcomes, int n) {
 
 // calculations.cpp:241-262.  Callers ask this for qubit after qubit, so on
 // statevectors of at least 16 qubits the first call computes every qubit's
-// probabilities with one read of the state per 15 qubits and later calls
+// probabilities with one read of the state per 10 qubits and later calls
 // take them from a cache, valid while the kernel layer's state epoch is
 // unchanged (any entry point that may write amplitudes moves it)
 qreal calcProbOfQubitOutcome(Qureg q, int qubit, int outcome) {

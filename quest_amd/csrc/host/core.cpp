@@ -324,6 +324,7 @@ void initQuESTEnv(void) { initCustomQuESTEnv(0, -1, -1); }
 void finalizeQuESTEnv(void) {
     QAH_API({
         requireEnv();
+        releaseInitScratch();
         check(qa_finalize());
         g_envInit = false;
         g_envFinalized = true;

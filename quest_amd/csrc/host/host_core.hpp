@@ -102,6 +102,8 @@ int pauliMaxIndex(PauliStr s);
 int randomQubitOutcome(double probOfZero);
 qindex randomMultiQubitOutcome(const vector<double>& probs);
 unsigned randomSharedSeed();
+// frees the density-matrix initialisers' statevector scratch (qureg.cpp)
+void releaseInitScratch();
 
 }  // namespace qah
 

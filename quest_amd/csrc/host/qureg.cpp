@@ -11,6 +11,24 @@ using namespace qah;
 
 namespace {
 
+// statevector scratch of the density-matrix initialisers (a random or
+// given pure state before rho = |psi><psi|): allocated on first need, grown
+// only when a larger density matrix asks, freed by finalizeQuESTEnv, so
+// repeated initialisations allocate nothing
+void* g_initScratch = nullptr;
+size_t g_initScratchBytes = 0;
+
+void* initScratch(size_t bytes) {
+    if (bytes > g_initScratchBytes) {
+        if (g_initScratch) check(qa_free(g_initScratch));
+        g_initScratch = nullptr;
+        g_initScratchBytes = 0;
+        check(qa_malloc(&g_initScratch, bytes));
+        g_initScratchBytes = bytes;
+    }
+    return g_initScratch;
+}
+
 void fillZero(const Qureg& q) { check(qa_statevec_initUniformState(dev(q), precOf(q), idxBits(q), 0, 0)); }
 
 void setOne(const Qureg& q, qindex ind, double re, double im) {
@@ -25,6 +43,14 @@ void validateAmpRange(const Qureg& q, qindex start, qindex num) {
 }
 
 }  // namespace
+
+namespace qah {
+void releaseInitScratch() {
+    if (g_initScratch) qa_free(g_initScratch);
+    g_initScratch = nullptr;
+    g_initScratchBytes = 0;
+}
+}  // namespace qah
 
 extern "C" {
 
@@ -203,15 +229,11 @@ void initArbitraryPureState(Qureg qureg, qcomp* amps) {
             check(qa_setAmps(dev(qureg), precOf(qureg), 0, dbl(amps), dim));
             return;
         }
-        // rho = |psi><psi| through a temporary statevector
-        void* psi = nullptr;
+        // rho = |psi><psi| through the library's statevector scratch
         int p = precOf(qureg);
-        check(qa_malloc(&psi, (size_t)dim * (p == QA_FP64 ? 16 : 8)));
-        int rc = qa_setAmps(psi, p, 0, dbl(amps), dim);
-        if (rc == QA_OK) rc = qa_densmatr_initPureState(dev(qureg), psi, p, qureg.numQubits);
-        int rc2 = qa_free(psi);
-        check(rc);
-        check(rc2);
+        void* psi = initScratch((size_t)dim * (p == QA_FP64 ? 16 : 8));
+        check(qa_setAmps(psi, p, 0, dbl(amps), dim));
+        check(qa_densmatr_initPureState(dev(qureg), psi, p, qureg.numQubits));
     });
 }
 
@@ -230,32 +252,31 @@ void initPureState(Qureg qureg, Qureg pure) {
 
 qreal setQuregToRenormalized(Qureg qureg);
 
+// one normalised random statevector of n qubits at psi, from one seed of
+// the measurement RNG (initialisations.cpp:121-133: the unnormalised
+// random amplitudes, then setQuregToRenormalized's 1/sqrt(total))
+static void randomPureInto(void* psi, int p, int n) {
+    unsigned seed = randomSharedSeed();
+    check(qa_statevec_initUnnormalisedUniformlyRandomPureStateAmps(psi, p, n, seed));
+    double norm = 0;
+    check(qa_statevec_calcTotalProb(psi, p, n, &norm));
+    if (norm <= 0) fail("The random state has zero norm.");
+    check(qa_statevec_setQuregToSuperposition(p, n, 1 / std::sqrt(norm), 0, psi, 0, 0, psi, 0, 0, psi));
+}
+
 void initRandomPureState(Qureg qureg) {
     QAH_API({
         validateQureg(qureg);
-        unsigned seed = randomSharedSeed();
-        qindex dim = 1LL << qureg.numQubits;
         int p = precOf(qureg);
         if (!qureg.isDensityMatrix) {
-            check(qa_statevec_initUnnormalisedUniformlyRandomPureStateAmps(dev(qureg), p, qureg.numQubits, seed));
-            double norm = 0;
-            check(qa_statevec_calcTotalProb(dev(qureg), p, qureg.numQubits, &norm));
-            double f = 1 / std::sqrt(norm);
-            check(qa_statevec_setQuregToSuperposition(p, qureg.numQubits, f, 0, dev(qureg), 0, 0, dev(qureg), 0, 0,
-                                                      dev(qureg)));
+            randomPureInto(dev(qureg), p, qureg.numQubits);
             return;
         }
-        void* psi = nullptr;
-        check(qa_malloc(&psi, (size_t)dim * (p == QA_FP64 ? 16 : 8)));
-        double norm = 0;
-        int rc = qa_statevec_initUnnormalisedUniformlyRandomPureStateAmps(psi, p, qureg.numQubits, seed);
-        if (rc == QA_OK) rc = qa_statevec_calcTotalProb(psi, p, qureg.numQubits, &norm);
-        if (rc == QA_OK)
-            rc = qa_statevec_setQuregToSuperposition(p, qureg.numQubits, 1 / std::sqrt(norm), 0, psi, 0, 0, psi, 0, 0, psi);
-        if (rc == QA_OK) rc = qa_densmatr_initPureState(dev(qureg), psi, p, qureg.numQubits);
-        int rc2 = qa_free(psi);
-        check(rc);
-        check(rc2);
+        // localiser.cpp:774-798: a random statevector, then rho = |psi><psi|
+        qindex dim = 1LL << qureg.numQubits;
+        void* psi = initScratch((size_t)dim * (p == QA_FP64 ? 16 : 8));
+        randomPureInto(psi, p, qureg.numQubits);
+        check(qa_densmatr_initPureState(dev(qureg), psi, p, qureg.numQubits));
     });
 }
 
@@ -263,33 +284,23 @@ void initRandomMixedState(Qureg qureg, qindex numPureStates) {
     QAH_API({
         validateDensMatr(qureg);
         if (numPureStates < 1) fail("Invalid number of pure states; must be >= 1.");
+        // localiser.cpp:801-827: from a blank matrix, mix in numPureStates
+        // random pure states with weight 1/numPureStates each (one seed of
+        // the measurement RNG per state), then renormalise by the trace
         qindex dim = 1LL << qureg.numQubits;
         int p = precOf(qureg);
-        size_t bytes = (size_t)dim * (p == QA_FP64 ? 16 : 8);
-        void* psi = nullptr;
-        check(qa_malloc(&psi, bytes));
-        int rc = QA_OK;
-        // uniform random probabilities (as a normalised vector of uniforms)
-        std::vector<double> probs((size_t)numPureStates);
-        double tot = 0;
-        for (auto& x : probs) {
-            x = (double)randomSharedSeed() / 4294967295.0;
-            tot += x;
+        void* psi = initScratch((size_t)dim * (p == QA_FP64 ? 16 : 8));
+        check(qa_statevec_initUniformState(dev(qureg), p, 2 * qureg.numQubits, 0, 0));
+        for (qindex k = 0; k < numPureStates; k++) {
+            randomPureInto(psi, p, qureg.numQubits);
+            check(qa_densmatr_mixQuregWithStatevec(1.0, dev(qureg), 1.0 / (double)numPureStates, psi, p,
+                                                   qureg.numQubits));
         }
-        if (rc == QA_OK) rc = qa_statevec_initUniformState(dev(qureg), p, 2 * qureg.numQubits, 0, 0);
-        for (qindex k = 0; k < numPureStates && rc == QA_OK; k++) {
-            double norm = 0;
-            rc = qa_statevec_initUnnormalisedUniformlyRandomPureStateAmps(psi, p, qureg.numQubits, randomSharedSeed());
-            if (rc == QA_OK) rc = qa_statevec_calcTotalProb(psi, p, qureg.numQubits, &norm);
-            if (rc == QA_OK)
-                rc = qa_statevec_setQuregToSuperposition(p, qureg.numQubits, 1 / std::sqrt(norm), 0, psi, 0, 0, psi, 0,
-                                                         0, psi);
-            if (rc == QA_OK) rc = qa_densmatr_mixQuregWithStatevec(1.0, dev(qureg), probs[(size_t)k] / tot, psi, p,
-                                                                  qureg.numQubits);
-        }
-        int r2 = qa_free(psi);
-        check(rc);
-        check(r2);
+        double tr = 0;
+        check(qa_densmatr_calcTotalProb(dev(qureg), p, qureg.numQubits, &tr));
+        if (tr <= 0) fail("The random mixed state has zero trace.");
+        check(qa_statevec_setQuregToSuperposition(p, 2 * qureg.numQubits, 1 / tr, 0, dev(qureg), 0, 0, dev(qureg), 0,
+                                                  0, dev(qureg)));
     });
 }
 

@@ -42,9 +42,9 @@ def load_golden(name: str):
     results = []
     for i, lab in enumerate(labels):
         v = vals[offs[i]:offs[i + 1]]
-        if lab == "meas":
+        if lab in ("meas", "measn"):
             results.append((lab, int(v[0])))
-        elif lab == "probs":
+        elif lab in ("probs", "inner"):
             results.append((lab, list(v)))
         else:
             results.append((lab, float(v[0])))
@@ -53,7 +53,11 @@ def load_golden(name: str):
 
 
 GOLDEN_CASES = ["layered_n10", "layered_n12", "density_n5", "variational_n8", "allops_sv_n7", "allops_dm_n4",
-                "measure_sv_n6", "measure_dm_n3", "layered_n20_c1"]
+                "measure_sv_n6", "measure_dm_n3", "layered_n20_c1", "random_seq_sv_n5", "measn_sv_n6",
+                "measn_dm_n3", "overlap_sv_n6", "overlap_dm_n3", "probs_sv_n12", "probs_dm_n5"]
+# fixtures whose outcomes come from the measurement RNG (pinned on the GPU
+# path directly; the oracle has no sampling)
+SAMPLED_CASES = {"measure_sv_n6", "measure_dm_n3", "random_seq_sv_n5", "measn_sv_n6", "measn_dm_n3"}
 
 
 def rel_l2(a, b) -> float:
@@ -66,7 +70,7 @@ def compare_results(got, want, tol=1e-10):
     assert len(got) == len(want), (len(got), len(want))
     for (lg, vg), (lw, vw) in zip(got, want):
         assert lg == lw, (lg, lw)
-        if lg == "meas":
+        if lg in ("meas", "measn"):
             assert vg == vw, f"measurement outcome {vg} != reference {vw}"
         else:
             np.testing.assert_allclose(np.atleast_1d(vg), np.atleast_1d(vw), rtol=0, atol=tol, err_msg=lg)

@@ -11,7 +11,7 @@ and stores, per case, one .npz holding
   sample    indices of the sampled amplitudes (large cases)
   labels / values   every OUT_* / MEAS result, in order
 
-Usage:  python tests/golden/make_golden.py
+Usage:  python tests/golden/make_golden.py [case names...]   (all by default)
 """
 
 from __future__ import annotations
@@ -47,6 +47,13 @@ def cases():
     yield "measure_dm_n3", measure_case(3, True, 808), True
     # the 20-qubit plain-CPU configuration at full size: results + a sample
     yield "layered_n20_c1", qc.layered(20, 20, 909, num_expec=1), False
+    yield "random_seq_sv_n5", random_seq_case(5, 1001), True
+    yield "measn_sv_n6", measn_case(6, False, 1102), True
+    yield "measn_dm_n3", measn_case(3, True, 1203), True
+    yield "overlap_sv_n6", overlap_case(6, False, 1304), True
+    yield "overlap_dm_n3", overlap_case(3, True, 1405), True
+    yield "probs_sv_n12", probs_case(12, False, 1506), True
+    yield "probs_dm_n5", probs_case(5, True, 1607), True
 
 
 def allops(n: int, dens: bool, seed: int) -> qc.Circuit:
@@ -96,10 +103,93 @@ def measure_case(n: int, dens: bool, seed: int) -> qc.Circuit:
     return c
 
 
+def random_seq_case(n: int, seed: int) -> qc.Circuit:
+    """Seeded measurements interleaved with random-state initialisations of
+    scratch quregs: each initRandomPureState draws one seed from the
+    measurement RNG (randomiser.cpp:175-193, initialisations.cpp:121-133),
+    initRandomMixedState one per pure state it mixes (localiser.cpp:801-827),
+    so the outcomes pin how far each advances the stream."""
+    rng = qc.SplitMix64(seed)
+    c = qc.Circuit(n, False, [], ("random", seed), f"random_seq_sv_n{n}")
+    c.ops.append(("SEEDS", [seed, 11, 12345]))
+    c.ops.append(("MEAS", 0))
+    c.ops.append(("RANDPURE", 4, False))
+    for q in range(n):
+        c.ops.append(("U1", q, qc.haar_unitary(rng, 2)))
+        c.ops.append(("MEAS", q))
+    c.ops.append(("RANDPURE", 3, True))
+    c.ops.append(("MEAS", 1))
+    c.ops.append(("RANDMIXED", 3, 3))
+    for q in range(n):
+        c.ops.append(("U1", q, qc.haar_unitary(rng, 2)))
+    c.ops.append(("MEAS", 2))
+    c.ops.append(("RANDPURE", 2, False))
+    c.ops.append(("MEASN", [4, 0, 3]))
+    c.ops.append(("OUT_TOTALPROB",))
+    return c
+
+
+def measn_case(n: int, dens: bool, seed: int) -> qc.Circuit:
+    """Seeded multi-qubit measurements (operations.cpp:1848-1925, outcome
+    sampled by randomiser.cpp:140-166): outcomes bit-exact, then the
+    collapsed state."""
+    rng = qc.SplitMix64(seed)
+    c = qc.Circuit(n, dens, [], ("random", seed), f"measn_{'dm' if dens else 'sv'}_n{n}")
+    c.ops.append(("SEEDS", [seed, 3]))
+    groups = [[1, 0], [n - 1, 1, 0], [2]] if n >= 3 else [[1, 0], [0]]
+    for g in groups:
+        for q in range(n):
+            c.ops.append(("U1", q, qc.haar_unitary(rng, 2)))
+        c.ops.append(("MEASN", g))
+        c.ops.append(("OUT_PROBS", sorted(g)))
+    c.ops.append(("MEAS", 0))
+    c.ops.append(("OUT_TOTALPROB",))
+    return c
+
+
+def overlap_case(n: int, dens: bool, seed: int) -> qc.Circuit:
+    """Inner products, fidelities and distances (calculations.cpp:49-61,
+    303-364) between the main qureg and an auxiliary one of every kind the
+    reference accepts: statevector pairs (Bures distance), density-matrix
+    pairs (Hilbert-Schmidt distance), and mixed pairs (<psi|rho|psi>,
+    purified distance)."""
+    rng = qc.SplitMix64(seed)
+    c = qc.Circuit(n, dens, [], ("random", seed), f"overlap_{'dm' if dens else 'sv'}_n{n}")
+    ops = c.ops
+    if dens:
+        for q in range(n):
+            ops.append(("DEPOL", q, 0.05 + 0.1 * rng.uniform()))
+    outs = [("OUT_INNER",), ("OUT_DISTANCE",)]
+    ops.append(("AUX", n, dens, seed + 1))
+    ops += outs + ([] if dens else [("OUT_FIDELITY",)])
+    ops.append(("AUX", n, not dens, seed + 2))
+    ops += outs + [("OUT_FIDELITY",)]
+    ops.append(("AUXCOPY",))
+    for q in range(n):
+        ops.append(("U1", q, qc.haar_unitary(rng, 2)))
+    ops += outs + ([] if dens else [("OUT_FIDELITY",)])
+    return c
+
+
+def probs_case(n: int, dens: bool, seed: int) -> qc.Circuit:
+    """All-outcome probability tables of 1 to 8 qubits, unsorted qubit lists
+    included (cpu_subroutines.cpp:1828-1903)."""
+    rng = qc.SplitMix64(seed)
+    c = qc.noisy_density(n, 3, seed) if dens else qc.layered(n, 3, seed, num_expec=0, probs=False)
+    c.name = f"probs_{'dm' if dens else 'sv'}_n{n}"
+    c.ops = [op for op in c.ops if not op[0].startswith("OUT_")]
+    for k in range(1, min(n, 8) + 1):
+        c.ops.append(("OUT_PROBS", rng.sample(n, k)))
+    return c
+
+
 def main() -> None:
     subprocess.run(["make", "-s", "-C", str(ROOT / "oracle"), "ref"], check=True)
     with tempfile.TemporaryDirectory() as tmp:
+        only = set(sys.argv[1:])
         for name, circ, full in cases():
+            if only and name not in only:
+                continue
             circ.name = name
             init = qc.initial_amps(circ)
             amps, results, _ = qoracle.run_reference(circ, Path(tmp), init_amps=init, want_amps=True)

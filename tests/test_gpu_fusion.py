@@ -23,8 +23,9 @@ pytestmark = pytest.mark.gpu
 AMP_TOL = 1e-12
 SCALAR_TOL = 1e-10
 
-# (tileQubits, lowQubits): default, smallest, no always-local rows, widest
-TILE_SHAPES = [(12, 4), (4, 0), (6, 2), (8, 5), (13, 8), (11, 3)]
+# (tileQubits, lowQubits): default, smallest, no always-local rows, widest, and
+# the benchmark's (13, 3)
+TILE_SHAPES = [(12, 4), (4, 0), (6, 2), (8, 5), (13, 8), (11, 3), (13, 3)]
 
 
 def say(*a):

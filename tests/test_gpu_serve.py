@@ -118,3 +118,19 @@ def test_submit_errors(api):
         # still serving after the refusals
         got = run_alone(eng, qc.variational(10, 2, 5, num_expec=2))
         assert got.shape == (2,)
+
+
+def test_empty_circuit_reads_the_zero_state(api):
+    """A request with no gates still gets one pass that creates |0...0>, so
+    it never reads what an earlier request left in its arena block: after a
+    request has used the block, an empty circuit gives <Z...Z> = 1 and
+    <X> = 0 exactly."""
+    say("[gpu] serve empty circuit")
+    n = 12
+    busy = qc.variational(n, 6, 41, num_expec=2)
+    paulis = [("Z" * n, list(range(n))), ("X", [3]), ("ZZ", [0, n - 1])]
+    with S.Engine(16 << n, max_active=1, precision=2) as eng:  # room for exactly one state
+        run_alone(eng, busy)
+        rid = eng.submit(n, [], paulis)
+        got, _ = eng.wait(rid)
+    assert list(got) == [1.0, 0.0, 1.0], got

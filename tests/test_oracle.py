@@ -18,7 +18,7 @@ import math
 import numpy as np
 import pytest
 
-from conftest import GOLDEN_CASES, compare_results, load_golden, rel_l2
+from conftest import GOLDEN_CASES, SAMPLED_CASES, compare_results, load_golden, rel_l2
 from quest_amd import circuits as qc
 from oracle import qoracle as O
 
@@ -28,7 +28,7 @@ PAULI = {"I": np.eye(2), "X": np.array([[0, 1], [1, 0]]), "Y": np.array([[0, -1j
 
 # ------------------------------------------------------------- 1. golden
 
-@pytest.mark.parametrize("name", [c for c in GOLDEN_CASES if not c.startswith("measure")])
+@pytest.mark.parametrize("name", [c for c in GOLDEN_CASES if c not in SAMPLED_CASES])
 def test_oracle_matches_reference_fixture(name):
     circ, init, amps, sample, results = load_golden(name)
     got, res = O.run(circ, init_amps=init) if init is not None else O.run(circ)
@@ -225,3 +225,49 @@ def test_pure_state_purity_and_normalisation():
     st.a[:] = qc.random_state(4, 9, True)
     assert abs(st.purity() - 1) < 1e-13
     assert abs(st.total_prob() - 1) < 1e-13
+
+
+# ------------------------------------------------------------- 4. two-state calculations
+
+def test_overlaps_reduce_to_closed_forms():
+    """calculations.cpp:49-61, 303-364 on states whose answers are known:
+    rho = |psi><psi| makes the mixed forms equal the statevector ones
+    (<phi|rho|phi> = |<phi|psi>|^2, Tr(rho sigma) = |<psi|phi>|^2), a state
+    has fidelity 1 and distance 0 with itself, and the density-matrix
+    outcome table of a pure state is the statevector's."""
+    n = 4
+    psi = qc.random_state(n, 11)
+    phi = qc.random_state(n, 12)
+    rho = qc.random_state(n, 11, dens=True)
+    sigma = qc.random_state(n, 12, dens=True)
+    sv = O.State(n, False)
+    sv.a[:] = psi
+    dm = O.State(n, True)
+    dm.a[:] = rho
+    ov = np.vdot(psi, phi)
+    assert abs(sv.inner(phi, False) - ov) < 1e-14
+    assert abs(sv.fidelity(phi, False) - abs(ov) ** 2) < 1e-14
+    assert abs(sv.distance(phi, False) - math.sqrt(2 - 2 * abs(ov))) < 1e-14
+    assert abs(sv.fidelity(psi, False) - 1) < 1e-14 and sv.distance(psi, False) < 1e-7
+    # density matrices: Tr(rho^dagger sigma) of pure states, HS distance
+    assert abs(dm.inner(sigma, True) - abs(ov) ** 2) < 1e-14
+    hs = math.sqrt(2 - 2 * abs(ov) ** 2)  # ||rho - sigma||_F of two pure states
+    assert abs(dm.distance(sigma, True) - hs) < 1e-13
+    # mixed forms: <phi|rho|phi>, purified distance, both argument orders
+    assert abs(dm.fidelity(phi, False) - abs(ov) ** 2) < 1e-14
+    assert abs(dm.inner(phi, False) - abs(ov) ** 2) < 1e-14
+    assert abs(dm.distance(phi, False) - math.sqrt(1 - abs(ov) ** 2)) < 1e-14
+    sv2 = O.State(n, False)
+    sv2.a[:] = phi
+    assert abs(sv2.fidelity(rho, True) - abs(ov) ** 2) < 1e-14
+    # outcome tables: pure density matrix = statevector, unsorted qubits
+    for qs in ([2], [3, 0], [1, 3, 0, 2]):
+        _, r1 = O.run(qc.Circuit(n, False, [("OUT_PROBS", qs)], ("random", 11)))
+        _, r2 = O.run(qc.Circuit(n, True, [("OUT_PROBS", qs)], ("random", 11)))
+        np.testing.assert_allclose(r1[0][1], r2[0][1], atol=1e-15)
+        assert abs(sum(r2[0][1]) - 1) < 1e-14
+        # against an explicit sum over the basis states
+        want = np.zeros(1 << len(qs))
+        for i, p in enumerate(np.abs(psi) ** 2):
+            want[sum(((i >> q) & 1) << b for b, q in enumerate(qs))] += p
+        np.testing.assert_allclose(r1[0][1], want, atol=1e-15)

@@ -66,6 +66,25 @@ CompMatr2 readM2(Tok& t) {
 
 void printReal(const string& label, double v) { std::printf("R %s %.17g\n", label.c_str(), v); }
 
+// QD_DEPLOY=gpu: every Qureg on the GPU (createCustomQureg with
+// useGpuAccel = 1), whatever its size, so that a build whose accelerator
+// seam is another GPU backend runs every operation there; otherwise QuEST's
+// own automatic deployment
+bool forceGpu() {
+    const char* d = std::getenv("QD_DEPLOY");
+    return d && std::string(d) == "gpu";
+}
+
+Qureg makeQureg(int n, int dm) {
+    if (!forceGpu()) return dm ? createDensityQureg(n) : createQureg(n);
+    Qureg q = createCustomQureg(n, dm, 0, 1, 0);
+    if (!q.isGpuAccelerated) {
+        std::fprintf(stderr, "QD_DEPLOY=gpu but the Qureg is not GPU-accelerated\n");
+        std::exit(5);
+    }
+    return q;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -90,8 +109,9 @@ int main(int argc, char** argv) {
 
         if (op == "QUREG") {
             int n = t.i(); int dm = t.i();
-            q = dm ? createDensityQureg(n) : createQureg(n);
+            q = makeQureg(n, dm);
             have = true;
+            if (forceGpu()) std::fprintf(stderr, "qureg of %d qubits on the GPU\n", n);
         } else if (op == "INITAMPS") {
             string path = t.s();
             std::ifstream bin(path, std::ios::binary);
@@ -198,7 +218,7 @@ int main(int argc, char** argv) {
         } else if (op == "RANDPURE" || op == "RANDMIXED") {
             // a scratch qureg: only the RNG stream it advances is observed
             int n = t.i(); long long k = t.ll();
-            Qureg s = (op == "RANDMIXED" || k) ? createDensityQureg(n) : createQureg(n);
+            Qureg s = makeQureg(n, (op == "RANDMIXED" || k) ? 1 : 0);
             if (op == "RANDPURE") initRandomPureState(s);
             else initRandomMixedState(s, k);
             destroyQureg(s);
@@ -208,7 +228,7 @@ int main(int argc, char** argv) {
             if (haveAux) destroyQureg(aux);
             if (op == "AUXFILE") {
                 int n = t.i(); int dm = t.i(); string path = t.s();
-                aux = dm ? createDensityQureg(n) : createQureg(n);
+                aux = makeQureg(n, dm);
                 std::ifstream bin(path, std::ios::binary);
                 vector<qcomp> amps(aux.numAmps);
                 bin.read(reinterpret_cast<char*>(amps.data()), (std::streamsize)(amps.size() * sizeof(qcomp)));

@@ -23,6 +23,8 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 LIB = HERE / "build" / "liboracle.so"
 REF_DRIVER = HERE / "_ref" / "circuit_driver"
+# the reference's front end on this backend's kernels (oracle/Makefile ref-amd)
+REF_AMD_DRIVER = HERE / "_ref" / "circuit_driver_amd"
 
 
 def build() -> None:
@@ -517,9 +519,11 @@ def apply_op(st: State, op, results: list) -> None:
         raise ValueError(f"unknown op {k}")
 
 
-def run_reference(circuit, workdir: Path, init_amps: np.ndarray | None = None, want_amps: bool = True):
-    """Run the circuit through the reference's own CPU build (oracle/_ref).
-    Returns (amps or None, results, stdout)."""
+def run_reference(circuit, workdir: Path, init_amps: np.ndarray | None = None, want_amps: bool = True,
+                  driver: Path | None = None, env: dict | None = None):
+    """Run the circuit through the reference's own CPU build (oracle/_ref),
+    or through another build of the reference's driver (`driver`, with extra
+    environment `env`).  Returns (amps or None, results, stdout)."""
     from quest_amd.circuits import to_text, parse_results, initial_amps
     workdir.mkdir(parents=True, exist_ok=True)
     init_path = None
@@ -543,6 +547,10 @@ def run_reference(circuit, workdir: Path, init_amps: np.ndarray | None = None, w
             line = f"AUXFILE {n} {int(dm)} {path}"
         lines.append(line)
     txt.write_text("\n".join(lines) + "\n")
-    proc = subprocess.run([str(REF_DRIVER), str(txt)], capture_output=True, text=True, check=True)
+    import os
+    full_env = dict(os.environ, **(env or {}))
+    proc = subprocess.run([str(driver or REF_DRIVER), str(txt)], capture_output=True, text=True, env=full_env)
+    if proc.returncode != 0:
+        raise RuntimeError(f"{driver or REF_DRIVER} failed ({proc.returncode}):\n{proc.stdout[-2000:]}\n{proc.stderr[-2000:]}")
     amps_out = np.fromfile(out_path, dtype=np.complex128) if out_path else None
-    return amps_out, parse_results(proc.stdout), proc.stdout
+    return amps_out, parse_results(proc.stdout), proc.stdout + proc.stderr

@@ -4,6 +4,7 @@ deterministic and self-consistent, and the product refuses to run without a
 GPU instead of falling back to the CPU."""
 
 import ctypes
+import subprocess
 import re
 from pathlib import Path
 
@@ -61,13 +62,46 @@ def test_gpu_layer_exports_cover_reference_boundary():
         assert need in names
 
 
-def test_struct_layouts_match_reference():
-    """Sizes of the by-value structs of quest.h (qureg.h:49-80, matrices.h:68-98)."""
+REF_DEFS = ["-DFLOAT_PRECISION=2", "-DCOMPILE_MPI=0", "-DCOMPILE_OPENMP=1", "-DCOMPILE_CUDA=0",
+            "-DCOMPILE_CUQUANTUM=0", "-DINCLUDE_DEPRECATED_FUNCTIONS=0"]
+
+
+def layout(tmp_path, reference: bool) -> dict:
+    """sizeof / offsetof of every by-value quest.h struct, from
+    tests/layout_probe.cpp compiled against include/quest.h or QuEST's own
+    headers."""
+    exe = tmp_path / ("probe_ref" if reference else "probe")
+    flags = ["-DQA_LAYOUT_REFERENCE", *REF_DEFS, "-I/root/reference"] if reference else [f"-I{INCLUDE}"]
+    subprocess.run(["g++", "-std=c++17", *flags, str(Path(__file__).parent / "layout_probe.cpp"), "-o", str(exe)],
+                   check=True, capture_output=True, text=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    return {" ".join(l.split()[:2]): int(l.split()[2]) for l in out.splitlines()}
+
+
+@pytest.mark.skipif(not Path("/root/reference/quest/include").is_dir(),
+                    reason="QuEST's headers are only present in the development container")
+def test_struct_layouts_match_reference(tmp_path):
+    """Every struct size and field offset of include/quest.h equals QuEST's
+    (quest/include/qureg.h:49-80, matrices.h:68-230, paulis.h:53-80,
+    channels.h:74-118, environment.h), compiled from both headers."""
+    ours, ref = layout(tmp_path, False), layout(tmp_path, True)
+    assert len(ref) >= 90
+    assert ours == ref
+
+
+def test_ctypes_layouts_match_header(tmp_path):
+    """The ctypes mirrors in quest_amd/api.py have the header's sizes and
+    field offsets."""
     from quest_amd import api as Q
-    assert ctypes.sizeof(Q.Qureg) == 8 * 4 + 5 * 8 + 4 * 8
-    assert ctypes.sizeof(Q.CompMatr1) == 16 + 4 * 16
-    assert ctypes.sizeof(Q.CompMatr2) == 16 + 16 * 16
-    assert ctypes.sizeof(Q.PauliStr) == 16
+    lay = layout(tmp_path, False)
+    for name in ("Qureg", "CompMatr1", "CompMatr2", "CompMatr", "DiagMatr1", "DiagMatr2", "DiagMatr",
+                 "FullStateDiagMatr", "PauliStr", "PauliStrSum", "SuperOp", "KrausMap", "QuESTEnv"):
+        cls = getattr(Q, name)
+        assert ctypes.sizeof(cls) == lay[f"sizeof {name}"], name
+        for fname, _ in cls._fields_:
+            key = f"offsetof {name}.{fname}"
+            if key in lay:
+                assert getattr(cls, fname).offset == lay[key], key
 
 
 def test_splitmix64_known_answer():

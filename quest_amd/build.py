@@ -69,8 +69,18 @@ def _run(cmd: list[str]) -> None:
 # circuit passes 1.3% faster and 14-qubit density-matrix passes 2% faster
 # (the complex64 instances, in qa_fused.hip and qa_serve.hip, measured
 # slower so, and keep the default).
+#
+# The tile-pass kernels (qa_fused*.hip, qa_serve.hip) keep a tile's
+# amplitudes in an array of registers that every op indexes with constants.
+# LLVM's CFG simplification would merge the per-register-bit arms of an op
+# switch (identical code up to which registers) into one block addressing
+# the array through pointer phis, which forces the array into scratch
+# memory; common-code sinking is therefore off for those files.
+NO_SINK = ["-mllvm", "-simplifycfg-sink-common=false"]
 FILE_FLAGS = {
-    "qa_fused_f64.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"],
+    "qa_fused_f64.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp", *NO_SINK],
+    "qa_fused.hip": NO_SINK,
+    "qa_serve.hip": NO_SINK,
 }
 
 

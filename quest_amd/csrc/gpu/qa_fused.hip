@@ -192,7 +192,8 @@ TOp toTile(const GOp& g, const Layout& L) {
         t.b[4] = jm2;
         t.b[5] = gm2;
     }
-    if (g.kind == F_DENSE2) std::memcpy(t.fx, g.f + 8, sizeof(t.fx));
+    // DENSE2: matrix elements 4..15 (the first 4 are in b)
+    if (g.kind == F_DENSE2) std::memcpy(t.fx, g.f + 8, 24 * sizeof(double));
     if (g.kind == F_DENSE1V) {
         // variant qubits (non-local): the lowest in rb2, the other (+1) in pad
         int nv = 0;
@@ -203,7 +204,9 @@ TOp toTile(const GOp& g, const Layout& L) {
                 else h.pad = (uint32_t)(x + 1);
                 nv++;
             }
-        std::memcpy(t.fx, g.f + 8, sizeof(t.fx));
+        // all four variant matrices in fx, so a tile reads its variant's
+        // matrix at one computed offset (variant 0 also stays in b)
+        std::memcpy(t.fx, g.f, 32 * sizeof(double));
     }
     if (g.kind == F_DIAG1 && h.rb1 >= 0 && ((h.rcMask >> h.rb1) & 1u)) {
         // phase-only diagonal (its target is one of its controls, set to 1):
@@ -279,6 +282,7 @@ void passShape(int numQubits, uint64_t localMask, TileArgs& a, Layout& L) {
             a.localPos[k++] = q;
         } else {
             a.nonLocalPos[nl++] = q;
+            a.nonLocalMask |= 1ull << q;
         }
     }
     a.k = k;
@@ -974,6 +978,8 @@ int launchPass(void* amps, const Queue& qu, const std::vector<GOp>& ops, size_t 
         return QA_ERR_HIP;
     }
     const unsigned grid = (unsigned)std::min<uint64_t>(a.numTiles, residentBlocks(fn, block, lds));
+    a.tileStep = 0;
+    for (int i = 0; i < a.numNonLocal && i < 32; i++) a.tileStep |= (uint64_t)((grid >> i) & 1u) << a.nonLocalPos[i];
     std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
     if (g_stats.timing) {
         if (g_stats.used == g_stats.pool.size()) {

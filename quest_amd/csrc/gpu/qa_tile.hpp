@@ -53,7 +53,7 @@ static_assert(sizeof(TOpHead) == 64, "op header is one 64-byte scalar load");
 // scalar load each, and (for two-qubit dense matrices only) the rest of
 // the matrix.  Body by kind:
 //   DENSE1, DENSE2: b = matrix elements 0..3 (re, im interleaved)
-//   DENSE1V: b = the 2x2 matrix for variant 0, fx = variants 1..3; the
+//   DENSE1V: b = the 2x2 matrix for variant 0, fx = variants 0..3; the
 //     variant is the value of the non-local qubits rb2 (bit 0) and pad - 1
 //     (bit 1, when pad > 0) at the tile
 //   DIAG1, PARITY, PAULI: b[0..3] = two complex factors (d0, d1 / f0, f1 /
@@ -145,6 +145,9 @@ struct TileArgs {
     int lastDirect;   // the last group stores its layout to HBM
     int usesLds;      // some layout change goes through LDS
     uint64_t numTiles;
+    uint64_t nonLocalMask;  // the non-local qubits as a mask
+    uint64_t tileStep;      // the grid size deposited over them (tile-pass kernels: the
+                            // base of tile t + gridDim.x is a masked add of it)
     const float* blocks;    // packed block matrices of the pass's BLOCK ops
     int localPos[MAXQ];     // global positions of the tile slots (ascending)
     int nonLocalPos[MAXQ];  // global positions of the other qubits (ascending)
@@ -301,73 +304,123 @@ struct RegIO<T, NT, RA> {
     __device__ __forceinline__ static void hbmStore(const ct<T>*, T*, uint64_t, const RegLayout&, const int*) {}
 };
 
-// register part of an op's control condition; C = false when the op has no
-// control among the register bits (no per-amplitude test is compiled in,
-// so updates happen in place)
-template <bool C>
-__device__ __forceinline__ bool regCond(int v, const TOpHead& h) {
-    return !C || (((uint32_t)v & h.rcMask) == h.rcVal);
-}
+// ---- ops on the registers ------------------------------------------------------
+//
+// Every op is straight-line arithmetic on all RA registers, with no branch
+// around any amplitude.  A control condition removes the op's effect where
+// it fails by substituting the identity for the op's matrix (1 for its
+// factors, "no exchange" for a swap): per register group when the condition
+// is on register bits (uniform over the workgroup, scalar selects), and per
+// lane when it is on the tile bits outside the registers (J: vector selects
+// on a lane predicate).  The identity leaves every finite amplitude value
+// unchanged (x * 1 + 0 * y = x; at most the sign of a zero changes), and
+// with no branch each amplitude keeps its registers across ops: a branch
+// around an update would make the compiler keep a second copy of all RA
+// registers for the path that skips it.
 
 template <typename T>
 __device__ __forceinline__ ct<T> cf(const double* f, int i) {
     return ct<T>{(T)f[2 * i], (T)f[2 * i + 1]};
+}
+template <typename T>
+__device__ __forceinline__ ct<T> pickC(bool p, ct<T> x, T re, T im) {
+    return ct<T>{p ? x.re : re, p ? x.im : im};
+}
+
+// a 2x2 matrix [[a, b], [c, d]]
+template <typename T>
+struct M2 {
+    ct<T> a, b, c, d;
+};
+template <typename T>
+__device__ __forceinline__ M2<T> m2At(const double* f) {
+    return M2<T>{cf<T>(f, 0), cf<T>(f, 1), cf<T>(f, 2), cf<T>(f, 3)};
+}
+// m where p holds, else the identity
+template <typename T>
+__device__ __forceinline__ M2<T> m2Pick(bool p, const M2<T>& m) {
+    return M2<T>{pickC(p, m.a, (T)1, (T)0), pickC(p, m.b, (T)0, (T)0), pickC(p, m.c, (T)0, (T)0),
+                 pickC(p, m.d, (T)1, (T)0)};
+}
+
+// a 4x4 matrix, row-major; element i is a constant once loops are unrolled
+template <typename T>
+struct M4 {
+    ct<T> e[16];
+    __device__ __forceinline__ ct<T> operator()(int i) const { return e[i]; }
+};
+// row-major elements 0..3 in f (the op body), 4..15 in fx
+template <typename T>
+__device__ __forceinline__ M4<T> m4At(const double* f, const double* __restrict__ fx) {
+    M4<T> m;
+#pragma unroll
+    for (int i = 0; i < 16; i++) m.e[i] = i < 4 ? cf<T>(f, i) : cf<T>(fx, i - 4);
+    return m;
+}
+// register part of an op's control condition for register group v; C =
+// false when the op has no control among the register bits
+template <bool C>
+__device__ __forceinline__ bool regCond(int v, const TOpHead& h) {
+    return !C || (((uint32_t)v & h.rcMask) == h.rcVal);
 }
 
 // the arithmetic of each gate below is the expression of its single-gate
 // kernel (qa_gates.hip / qa_channels.hip), term for term
 
 template <typename T, int B, bool C>
-__device__ __forceinline__ void regDense1(ct<T>* r, const TOpHead& h, const double* f) {
-    const ct<T> m00 = cf<T>(f, 0), m01 = cf<T>(f, 1), m10 = cf<T>(f, 2), m11 = cf<T>(f, 3);
+__device__ __forceinline__ void regDense1(ct<T>* r, const TOpHead& h, const M2<T>& m) {
 #pragma unroll
     for (int v = 0; v < RA; v++) {
         if (v & (1 << B)) continue;
-        if (!regCond<C>(v, h)) continue;
-        cmat2T(m00, m01, m10, m11, r[v], r[v | (1 << B)]);
+        const M2<T> mv = C ? m2Pick(regCond<C>(v, h), m) : m;
+        cmat2T(mv.a, mv.b, mv.c, mv.d, r[v], r[v | (1 << B)]);
     }
 }
 
-// element i (0..15) of a 4x4 matrix whose elements 0..3 are in f and 4..15
-// in fx (memory); i is a constant once the callers' loops are unrolled
-template <typename T>
-struct M4 {
-    const double* f;
-    const double* __restrict__ fx;
-    __device__ __forceinline__ ct<T> operator()(int i) const {
-        return i < 4 ? ct<T>{(T)f[2 * i], (T)f[2 * i + 1]} : ct<T>{(T)fx[2 * (i - 4)], (T)fx[2 * (i - 4) + 1]};
-    }
-};
-
-template <typename T, int B1, int B2, bool C>
-__device__ __forceinline__ void regDense2(ct<T>* r, const TOpHead& h, const double* f, const double* __restrict__ fx) {
+// where a condition fails (register group v for C, this lane for J) the
+// quad keeps its values: a 4x4 matrix is too large to substitute
+template <typename T, int B1, int B2, bool C, bool J>
+__device__ __forceinline__ void regDense2(ct<T>* r, const TOpHead& h, const M4<T>& m, bool act) {
     constexpr int off[4] = {0, 1 << B1, 1 << B2, (1 << B1) | (1 << B2)};
 #pragma unroll
     for (int v = 0; v < RA; v++) {
         if (v & ((1 << B1) | (1 << B2))) continue;
-        if (!regCond<C>(v, h)) continue;
-        cmat4T<T>(M4<T>{f, fx}, r[v | off[0]], r[v | off[1]], r[v | off[2]], r[v | off[3]]);
+        if constexpr (C || J) {
+            ct<T> y[4] = {r[v | off[0]], r[v | off[1]], r[v | off[2]], r[v | off[3]]};
+            cmat4T<T>(m, y[0], y[1], y[2], y[3]);
+            const bool p = regCond<C>(v, h) && (!J || act);
+#pragma unroll
+            for (int i = 0; i < 4; i++) r[v | off[i]] = pickC(p, y[i], r[v | off[i]].re, r[v | off[i]].im);
+        } else {
+            cmat4T<T>(m, r[v | off[0]], r[v | off[1]], r[v | off[2]], r[v | off[3]]);
+        }
     }
 }
 
-template <typename T, int B1, int B2, bool C>
-__device__ __forceinline__ void regSwap(ct<T>* r, const TOpHead& h, const double*) {
+// act: the lane predicate (true for every lane unless J)
+template <typename T, int B1, int B2, bool C, bool J>
+__device__ __forceinline__ void regSwap(ct<T>* r, const TOpHead& h, bool act) {
 #pragma unroll
     for (int v = 0; v < RA; v++) {
         if (!(v & (1 << B1)) || (v & (1 << B2))) continue;
-        if (!regCond<C>(v, h)) continue;
         constexpr int flip = (1 << B1) | (1 << B2);
-        const ct<T> x = r[v];
-        r[v] = r[v ^ flip];
-        r[v ^ flip] = x;
+        const ct<T> x = r[v], y = r[v ^ flip];
+        if constexpr (C || J) {
+            const bool p = regCond<C>(v, h) && (!J || act);
+            r[v] = ct<T>{p ? y.re : x.re, p ? y.im : x.im};
+            r[v ^ flip] = ct<T>{p ? x.re : y.re, p ? x.im : y.im};
+        } else {
+            r[v] = y;
+            r[v ^ flip] = x;
+        }
     }
 }
 
 // one-qubit depolarising / dephasing on (ket, bra): both formulas are
 // symmetric in the two bits, so B1 < B2 covers either order (never
 // controlled)
-template <typename T, int B1, int B2, bool C>
-__device__ __forceinline__ void regDepol1(ct<T>* r, const TOpHead&, const double* f) {
+template <typename T, int B1, int B2>
+__device__ __forceinline__ void regDepol1(ct<T>* r, const double* f) {
     const T c1 = (T)f[0], c2 = (T)f[1], c3 = (T)f[2];
 #pragma unroll
     for (int v = 0; v < RA; v++) {
@@ -381,8 +434,8 @@ __device__ __forceinline__ void regDepol1(ct<T>* r, const TOpHead&, const double
     }
 }
 
-template <typename T, int B1, int B2, bool C>
-__device__ __forceinline__ void regDeph1(ct<T>* r, const TOpHead&, const double* f) {
+template <typename T, int B1, int B2>
+__device__ __forceinline__ void regDeph1(ct<T>* r, const double* f) {
     const T fac = (T)f[0];
 #pragma unroll
     for (int v = 0; v < RA; v++) {
@@ -395,21 +448,24 @@ __device__ __forceinline__ void regDeph1(ct<T>* r, const TOpHead&, const double*
 }
 
 // Pauli tensor / gadget with X/Y bits MXY (all in registers); thrPar is the
-// parity of the maskYZ bits outside the registers for this thread
-template <typename T, int MXY, bool C>
-__device__ __forceinline__ void regPauli(ct<T>* r, const TOpHead& h, const double* f, int thrPar) {
+// parity of the maskYZ bits outside the registers for this thread.  Where
+// the condition fails the pair gets ampFac = 1, pairFac = 0.
+template <typename T, int MXY, bool C, bool J>
+__device__ __forceinline__ void regPauli(ct<T>* r, const TOpHead& h, const double* f, int thrPar, bool act) {
     constexpr int top = 31 - __builtin_clz((unsigned)MXY);
     const ct<T> ampFac = cf<T>(f, 0), pairFac = cf<T>(f, 1);
 #pragma unroll
     for (int v = 0; v < RA; v++) {
         if (v & (1 << top)) continue;
-        if (!regCond<C>(v, h)) continue;
         const int w = v ^ MXY;
+        const bool p = regCond<C>(v, h) && (!J || act);
+        const ct<T> af = (C || J) ? pickC(p, ampFac, (T)1, (T)0) : ampFac;
+        const ct<T> pf = (C || J) ? pickC(p, pairFac, (T)0, (T)0) : pairFac;
         const T sA = ((__popc((uint32_t)v & h.rm2) & 1) ^ thrPar) ? (T)-1 : (T)1;
         const T sB = ((__popc((uint32_t)w & h.rm2) & 1) ^ thrPar) ? (T)-1 : (T)1;
-        const ct<T> pB = {pairFac.re * sB, pairFac.im * sB};
-        const ct<T> pA = {pairFac.re * sA, pairFac.im * sA};
-        cmat2T(ampFac, pB, pA, ampFac, r[v], r[w]);
+        const ct<T> pB = {pf.re * sB, pf.im * sB};
+        const ct<T> pA = {pf.re * sA, pf.im * sA};
+        cmat2T(af, pB, pA, af, r[v], r[w]);
     }
 }
 
@@ -417,80 +473,86 @@ __device__ __forceinline__ void regPauli(ct<T>* r, const TOpHead& h, const doubl
 // amplitudes with bit B clear are left as they are, as in the single-gate
 // kernel, which never touches them)
 template <typename T, int B, bool C, bool PH>
-__device__ __forceinline__ void regDiag1R(ct<T>* r, const TOpHead& h, const double* f) {
-    const ct<T> d0 = cf<T>(f, 0), d1 = cf<T>(f, 1);
+__device__ __forceinline__ void regDiag1R(ct<T>* r, const TOpHead& h, ct<T> d0, ct<T> d1) {
 #pragma unroll
     for (int v = 0; v < RA; v++) {
         if (PH && !(v & (1 << B))) continue;
-        if (!regCond<C>(v, h)) continue;
-        r[v] = cmulT((v & (1 << B)) ? d1 : d0, r[v]);
+        const ct<T> d = (v & (1 << B)) ? d1 : d0;
+        r[v] = cmulT(C ? pickC(regCond<C>(v, h), d, (T)1, (T)0) : d, r[v]);
     }
 }
 
-// one-target diagonal whose target is outside the registers: one factor
-// for all of this thread's amplitudes
+// a diagonal factor per lane for all of this thread's amplitudes (target
+// outside the registers, or a parity over them)
 template <typename T, bool C>
-__device__ __forceinline__ void regDiag1T(ct<T>* r, const TOpHead& h, const double* f, bool thrBit) {
-    const ct<T> d = thrBit ? cf<T>(f, 1) : cf<T>(f, 0);
+__device__ __forceinline__ void regScale(ct<T>* r, const TOpHead& h, ct<T> d) {
+#pragma unroll
+    for (int v = 0; v < RA; v++) r[v] = cmulT(C ? pickC(regCond<C>(v, h), d, (T)1, (T)0) : d, r[v]);
+}
+
+template <typename T, bool C>
+__device__ __forceinline__ void regParity(ct<T>* r, const TOpHead& h, ct<T> f0, ct<T> f1, int thrPar) {
 #pragma unroll
     for (int v = 0; v < RA; v++) {
-        if (!regCond<C>(v, h)) continue;
-        r[v] = cmulT(d, r[v]);
+        const ct<T> f = ((__popc((uint32_t)v & h.rm1) & 1) ^ thrPar) ? f1 : f0;
+        r[v] = cmulT(C ? pickC(regCond<C>(v, h), f, (T)1, (T)0) : f, r[v]);
     }
 }
 
+#define QA_SWAP_CASES                                                 \
+    case 0 * RB + 1: regSwap<T, 0, 1, C, J>(r, h, act); break;        \
+    case 0 * RB + 2: regSwap<T, 0, 2, C, J>(r, h, act); break;        \
+    case 0 * RB + 3: regSwap<T, 0, 3, C, J>(r, h, act); break;        \
+    case 1 * RB + 2: regSwap<T, 1, 2, C, J>(r, h, act); break;        \
+    case 1 * RB + 3: regSwap<T, 1, 3, C, J>(r, h, act); break;        \
+    case 2 * RB + 3: regSwap<T, 2, 3, C, J>(r, h, act); break;
+
+#define QA_CHAN_CASES(FN)                                 \
+    case 0 * RB + 1: FN<T, 0, 1>(r, f); break;           \
+    case 0 * RB + 2: FN<T, 0, 2>(r, f); break;           \
+    case 0 * RB + 3: FN<T, 0, 3>(r, f); break;           \
+    case 1 * RB + 2: FN<T, 1, 2>(r, f); break;           \
+    case 1 * RB + 3: FN<T, 1, 3>(r, f); break;           \
+    case 2 * RB + 3: FN<T, 2, 3>(r, f); break;
+
 template <typename T, bool C>
-__device__ __forceinline__ void regParity(ct<T>* r, const TOpHead& h, const double* f, int thrPar) {
-    const ct<T> f0 = cf<T>(f, 0), f1 = cf<T>(f, 1);
-#pragma unroll
-    for (int v = 0; v < RA; v++) {
-        if (!regCond<C>(v, h)) continue;
-        r[v] = cmulT(((__popc((uint32_t)v & h.rm1) & 1) ^ thrPar) ? f1 : f0, r[v]);
+__device__ __forceinline__ void dense1On(ct<T>* r, const TOpHead& h, int b, const M2<T>& m) {
+    switch (b) {
+        case 0: regDense1<T, 0, C>(r, h, m); break;
+        case 1: regDense1<T, 1, C>(r, h, m); break;
+        case 2: regDense1<T, 2, C>(r, h, m); break;
+        case 3: regDense1<T, 3, C>(r, h, m); break;
     }
 }
 
-#define QA_PAIR_CASES(FN)                                      \
-    case 0 * RB + 1: FN<T, 0, 1, C>(r, h, f); break;           \
-    case 0 * RB + 2: FN<T, 0, 2, C>(r, h, f); break;           \
-    case 0 * RB + 3: FN<T, 0, 3, C>(r, h, f); break;           \
-    case 1 * RB + 2: FN<T, 1, 2, C>(r, h, f); break;           \
-    case 1 * RB + 3: FN<T, 1, 3, C>(r, h, f); break;           \
-    case 2 * RB + 3: FN<T, 2, 3, C>(r, h, f); break;
-
-// applies one non-GROUP op to the registers of one thread; C = the op has
-// controls among the register bits
-template <typename T, bool C>
+// applies one op that may carry controls to the registers of one thread;
+// C = the op has controls among the register bits, J = among the tile bits
+// outside them (act is then this lane's part of the condition)
+template <typename T, bool C, bool J>
 __device__ __forceinline__ void applyOpC(ct<T>* r, const TOp* __restrict__ op, const OpRegs& o, uint32_t jb,
-                                         uint64_t base) {
+                                         uint64_t base, bool act) {
     const TOpHead& h = o.h;
     const double* f = o.f;
     switch (h.kind) {
-        case F_DENSE1:
-            switch (h.rb1) {
-                case 0: regDense1<T, 0, C>(r, h, f); break;
-                case 1: regDense1<T, 1, C>(r, h, f); break;
-                case 2: regDense1<T, 2, C>(r, h, f); break;
-                case 3: regDense1<T, 3, C>(r, h, f); break;
-            }
+        case F_DENSE1: {
+            const M2<T> m = m2At<T>(f);
+            dense1On<T, C>(r, h, h.rb1, J ? m2Pick(act, m) : m);
             break;
+        }
         case F_DENSE1V: {
-            // a 2x2 matrix per value of one or two non-local qubits: the
-            // tile's value picks it (uniform over the workgroup)
+            // a 2x2 matrix per value of one or two non-local qubits, all
+            // four in fx: the tile's value picks it (uniform over the
+            // workgroup)
             uint32_t w = (uint32_t)((base >> h.rb2) & 1ull);
             if (h.pad) w |= (uint32_t)((base >> (h.pad - 1)) & 1ull) << 1;
-            const double* fv = w ? op->fx + 8 * (w - 1) : f;
-            switch (h.rb1) {
-                case 0: regDense1<T, 0, C>(r, h, fv); break;
-                case 1: regDense1<T, 1, C>(r, h, fv); break;
-                case 2: regDense1<T, 2, C>(r, h, fv); break;
-                case 3: regDense1<T, 3, C>(r, h, fv); break;
-            }
+            const M2<T> m = m2At<T>(op->fx + 8 * w);
+            dense1On<T, C>(r, h, h.rb1, J ? m2Pick(act, m) : m);
             break;
         }
         case F_DENSE2: {
-            const double* fx = op->fx;
+            const M4<T> m = m4At<T>(f, op->fx);
             switch (h.rb1 * RB + h.rb2) {
-#define QA_D2(B1, B2) case B1 * RB + B2: regDense2<T, B1, B2, C>(r, h, f, fx); break;
+#define QA_D2(B1, B2) case B1 * RB + B2: regDense2<T, B1, B2, C, J>(r, h, m, act); break;
                 QA_D2(0, 1) QA_D2(0, 2) QA_D2(0, 3) QA_D2(1, 2) QA_D2(1, 3) QA_D2(2, 3)
                 QA_D2(1, 0) QA_D2(2, 0) QA_D2(3, 0) QA_D2(2, 1) QA_D2(3, 1) QA_D2(3, 2)
 #undef QA_D2
@@ -498,34 +560,46 @@ __device__ __forceinline__ void applyOpC(ct<T>* r, const TOp* __restrict__ op, c
             break;
         }
         case F_SWAP:
-            switch (h.rb1 * RB + h.rb2) { QA_PAIR_CASES(regSwap) }
+            switch (h.rb1 * RB + h.rb2) { QA_SWAP_CASES }
             break;
-        case F_DIAG1:
+        case F_DIAG1: {
+            ct<T> d0 = cf<T>(f, 0), d1 = cf<T>(f, 1);
+            if (J) {
+                d0 = pickC(act, d0, (T)1, (T)0);
+                d1 = pickC(act, d1, (T)1, (T)0);
+            }
             if (h.rb1 < 0) {
-                regDiag1T<T, C>(r, h, f, ((jb & o.jm) | (base & o.gm)) != 0);
+                regScale<T, C>(r, h, ((jb & o.jm) | (base & o.gm)) != 0 ? d1 : d0);
             } else if (h.pad) {
                 switch (h.rb1) {
-                    case 0: regDiag1R<T, 0, C, true>(r, h, f); break;
-                    case 1: regDiag1R<T, 1, C, true>(r, h, f); break;
-                    case 2: regDiag1R<T, 2, C, true>(r, h, f); break;
-                    case 3: regDiag1R<T, 3, C, true>(r, h, f); break;
+                    case 0: regDiag1R<T, 0, C, true>(r, h, d0, d1); break;
+                    case 1: regDiag1R<T, 1, C, true>(r, h, d0, d1); break;
+                    case 2: regDiag1R<T, 2, C, true>(r, h, d0, d1); break;
+                    case 3: regDiag1R<T, 3, C, true>(r, h, d0, d1); break;
                 }
             } else {
                 switch (h.rb1) {
-                    case 0: regDiag1R<T, 0, C, false>(r, h, f); break;
-                    case 1: regDiag1R<T, 1, C, false>(r, h, f); break;
-                    case 2: regDiag1R<T, 2, C, false>(r, h, f); break;
-                    case 3: regDiag1R<T, 3, C, false>(r, h, f); break;
+                    case 0: regDiag1R<T, 0, C, false>(r, h, d0, d1); break;
+                    case 1: regDiag1R<T, 1, C, false>(r, h, d0, d1); break;
+                    case 2: regDiag1R<T, 2, C, false>(r, h, d0, d1); break;
+                    case 3: regDiag1R<T, 3, C, false>(r, h, d0, d1); break;
                 }
             }
             break;
-        case F_PARITY:
-            regParity<T, C>(r, h, f, parity64((jb & o.jm) ^ (base & o.gm)));
+        }
+        case F_PARITY: {
+            ct<T> f0 = cf<T>(f, 0), f1 = cf<T>(f, 1);
+            if (J) {
+                f0 = pickC(act, f0, (T)1, (T)0);
+                f1 = pickC(act, f1, (T)1, (T)0);
+            }
+            regParity<T, C>(r, h, f0, f1, parity64((jb & o.jm) ^ (base & o.gm)));
             break;
+        }
         case F_PAULI: {
             const int thrPar = parity64((jb & o.jm) ^ (base & o.gm));
             switch (h.rm1) {
-#define QA_PAULI_CASE(M) case M: regPauli<T, M, C>(r, h, f, thrPar); break;
+#define QA_PAULI_CASE(M) case M: regPauli<T, M, C, J>(r, h, f, thrPar, act); break;
                 QA_PAULI_CASE(1) QA_PAULI_CASE(2) QA_PAULI_CASE(3) QA_PAULI_CASE(4) QA_PAULI_CASE(5)
                 QA_PAULI_CASE(6) QA_PAULI_CASE(7) QA_PAULI_CASE(8) QA_PAULI_CASE(9) QA_PAULI_CASE(10)
                 QA_PAULI_CASE(11) QA_PAULI_CASE(12) QA_PAULI_CASE(13) QA_PAULI_CASE(14) QA_PAULI_CASE(15)
@@ -536,75 +610,90 @@ __device__ __forceinline__ void applyOpC(ct<T>* r, const TOp* __restrict__ op, c
     }
 }
 
-// ops that never have controls among the register bits (PHASES, DENSE1N,
-// the channels) are applied here once; the others by applyOpC
+// a run of diagonal gates (PHASES): per-thread factors A for every
+// amplitude and B[b] for those with register bit b set (the terms come
+// sorted by that slot, with the count of each slot in a nibble of rm1),
+// multiplied into one factor per register (subset products) before they
+// reach the amplitudes
+template <typename T>
+__device__ __forceinline__ void applyPhases(ct<T>* r, const TOp* __restrict__ op, const TOpHead& h, uint32_t jb,
+                                            uint64_t base) {
+    const PhaseTerm* pt = reinterpret_cast<const PhaseTerm*>(op->b);
+    ct<T> acc[1 + RB];
+    int k = 0;
+#pragma unroll
+    for (int sl = 0; sl <= RB; sl++) {
+        acc[sl] = ct<T>{(T)1, (T)0};
+        const int ns = (int)((h.rm1 >> (4 * sl)) & 15u);
+        for (int e = 0; e < ns; e++, k++) {
+            const PhaseTerm t = pt[k];
+            const bool on = (jb & t.jm) == t.jv && (base & t.gm) == t.gv;
+            acc[sl] = cmulT(ct<T>{on ? (T)t.fre : (T)1, on ? (T)t.fim : (T)0}, acc[sl]);
+        }
+    }
+    ct<T> fac[RA];
+    fac[0] = acc[0];
+#pragma unroll
+    for (int b = 0; b < RB; b++)
+#pragma unroll
+        for (int v = 0; v < (1 << b); v++) fac[v | (1 << b)] = cmulT(acc[1 + b], fac[v]);
+#pragma unroll
+    for (int v = 0; v < RA; v++) r[v] = cmulT(fac[v], r[v]);
+}
+
+// ops that never carry controls (PHASES, DENSE1N, the channels) are applied
+// here; the others by applyOpC
 template <typename T>
 __device__ __forceinline__ void applyOp(ct<T>* r, const TOp* __restrict__ op, const OpRegs& o, uint32_t jb,
                                         uint64_t base) {
     const TOpHead& h = o.h;
     const double* f = o.f;
-    constexpr bool C = false;
     switch (h.kind) {
-        case F_PHASES: {
-            // per-thread factors: A for every amplitude, B[b] for those with
-            // register bit b set; the terms come sorted by that slot, with
-            // the count of each slot in a nibble of rm1
-            const PhaseTerm* pt = reinterpret_cast<const PhaseTerm*>(op->b);
-            ct<T> acc[1 + RB];
-            int k = 0;
-#pragma unroll
-            for (int sl = 0; sl <= RB; sl++) {
-                acc[sl] = ct<T>{(T)1, (T)0};
-                const int ns = (int)((h.rm1 >> (4 * sl)) & 15u);
-                for (int e = 0; e < ns; e++, k++) {
-                    const PhaseTerm t = pt[k];
-                    const bool on = (jb & t.jm) == t.jv && (base & t.gm) == t.gv;
-                    acc[sl] = cmulT(ct<T>{on ? (T)t.fre : (T)1, on ? (T)t.fim : (T)0}, acc[sl]);
-                }
-            }
-#pragma unroll
-            for (int v = 0; v < RA; v++) {
-                ct<T> x = cmulT(acc[0], r[v]);
-#pragma unroll
-                for (int b = 0; b < RB; b++)
-                    if (v & (1 << b)) x = cmulT(acc[1 + b], x);
-                r[v] = x;
-            }
+        case F_PHASES:
+            applyPhases<T>(r, op, h, jb, base);
             break;
-        }
         case F_DENSE1N: {
             // N = rb2 uncontrolled one-qubit matrices on register bits 0..N-1
             // in that order: straight-line code, one dispatch for N gates
             const double* fx = op->fx;
             switch (h.rb2) {
                 case 2:
-                    regDense1<T, 0, false>(r, h, f);
-                    regDense1<T, 1, false>(r, h, fx);
+                    regDense1<T, 0, false>(r, h, m2At<T>(f));
+                    regDense1<T, 1, false>(r, h, m2At<T>(fx));
                     break;
                 case 3:
-                    regDense1<T, 0, false>(r, h, f);
-                    regDense1<T, 1, false>(r, h, fx);
-                    regDense1<T, 2, false>(r, h, fx + 8);
+                    regDense1<T, 0, false>(r, h, m2At<T>(f));
+                    regDense1<T, 1, false>(r, h, m2At<T>(fx));
+                    regDense1<T, 2, false>(r, h, m2At<T>(fx + 8));
                     break;
                 case 4:
-                    regDense1<T, 0, false>(r, h, f);
-                    regDense1<T, 1, false>(r, h, fx);
-                    regDense1<T, 2, false>(r, h, fx + 8);
-                    regDense1<T, 3, false>(r, h, fx + 16);
+                    regDense1<T, 0, false>(r, h, m2At<T>(f));
+                    regDense1<T, 1, false>(r, h, m2At<T>(fx));
+                    regDense1<T, 2, false>(r, h, m2At<T>(fx + 8));
+                    regDense1<T, 3, false>(r, h, m2At<T>(fx + 16));
                     break;
             }
             break;
         }
         case F_DEPOL1:
-            switch (h.rb1 * RB + h.rb2) { QA_PAIR_CASES(regDepol1) }
+            switch (h.rb1 * RB + h.rb2) { QA_CHAN_CASES(regDepol1) }
             break;
         case F_DEPH1:
-            switch (h.rb1 * RB + h.rb2) { QA_PAIR_CASES(regDeph1) }
+            switch (h.rb1 * RB + h.rb2) { QA_CHAN_CASES(regDeph1) }
             break;
         default:
-            if (h.rcMask) applyOpC<T, true>(r, op, o, jb, base);
-            else applyOpC<T, false>(r, op, o, jb, base);
+            if (h.rcMask) applyOpC<T, true, false>(r, op, o, jb, base, true);
+            else applyOpC<T, false, false>(r, op, o, jb, base, true);
     }
+}
+
+// an op whose condition involves the tile bits outside the registers: act
+// is this lane's part of it
+template <typename T>
+__device__ __forceinline__ void applyOpLanes(ct<T>* r, const TOp* __restrict__ op, const OpRegs& o, uint32_t jb,
+                                             uint64_t base, bool act) {
+    if (o.h.rcMask) applyOpC<T, true, true>(r, op, o, jb, base, act);
+    else applyOpC<T, false, true>(r, op, o, jb, base, act);
 }
 
 __host__ __device__ __forceinline__ const int* groupSlots(const TOp& op) { return reinterpret_cast<const int*>(op.b); }
@@ -758,9 +847,15 @@ __device__ __forceinline__ void tileBody(T* __restrict__ amps, const TileArgs& a
             sj = ldsByte<T>(cur.jb);
             RegIO<T, NT, 0>::ldsLoad(r, tile, opaqueV(sj), loadOffsets(ops + o));
             inRegs = true;
-        } else if ((base & op.h.gcMask) == op.h.gcVal &&   // uniform over the workgroup
-                   (cur.jb & op.h.jcMask) == op.h.jcVal) {  // uniform over the thread's registers
-            applyOp<T>(r, ops + o, op, cur.jb, base);
+        } else if ((base & op.h.gcMask) == op.h.gcVal) {  // uniform over the workgroup
+            if (!op.h.jcMask) {
+                applyOp<T>(r, ops + o, op, cur.jb, base);
+            } else {
+                // a condition on the tile bits outside the registers: the
+                // same for all of a thread's registers, applied per lane
+                const bool act = (cur.jb & op.h.jcMask) == op.h.jcVal;
+                if (__builtin_amdgcn_ballot_w64(act)) applyOpLanes<T>(r, ops + o, op, cur.jb, base, act);
+            }
         }
     }
     if (a.lastDirect) {

@@ -25,30 +25,31 @@ __device__ __forceinline__ void tilePass(T* __restrict__ amps, const TileArgs& a
     passLayouts(row, first, a, ops, tid);
     const RegLayout& loadL = a.firstDirect ? first : row;
 
+    // tile t's base is t deposited over the non-local positions; the next
+    // tile of this workgroup (t + gridDim.x) is one masked add away
+    auto nextBase = [&](uint64_t b) { return ((b | ~a.nonLocalMask) + a.tileStep) & a.nonLocalMask; };
+    uint64_t t = blockIdx.x;
+    uint64_t base = deposit(t, a.nonLocalPos, a.numNonLocal);
     if constexpr (PF) {
         ct<T> rn[RA];
-        uint64_t t = blockIdx.x;
-        if (t < a.numTiles)
-            RegIO<T, NT, 0>::hbmLoad(rn, amps, deposit(t, a.nonLocalPos, a.numNonLocal) | loadL.tg, loadL, a.localPos);
+        if (t < a.numTiles) RegIO<T, NT, 0>::hbmLoad(rn, amps, base | loadL.tg, loadL, a.localPos);
         for (; t < a.numTiles; t += gridDim.x) {
-            const uint64_t base = deposit(t, a.nonLocalPos, a.numNonLocal);
             ct<T> r[RA];
 #pragma unroll
             for (int v = 0; v < RA; v++) r[v] = rn[v];
-            const uint64_t tn = t + gridDim.x;
-            if (tn < a.numTiles)
-                RegIO<T, NT, 0>::hbmLoad(rn, amps, deposit(tn, a.nonLocalPos, a.numNonLocal) | loadL.tg, loadL,
-                                         a.localPos);
+            const uint64_t baseN = nextBase(base);
+            if (t + gridDim.x < a.numTiles) RegIO<T, NT, 0>::hbmLoad(rn, amps, baseN | loadL.tg, loadL, a.localPos);
             tileBody<T, NT>(amps, a, ops, base, r, tile, row, first, tid);
+            base = baseN;
             // the next tile's first LDS writes must not overtake this tile's reads
             if (a.usesLds) __syncthreads();
         }
     } else {
-        for (uint64_t t = blockIdx.x; t < a.numTiles; t += gridDim.x) {
-            const uint64_t base = deposit(t, a.nonLocalPos, a.numNonLocal);
+        for (; t < a.numTiles; t += gridDim.x) {
             ct<T> r[RA];
             RegIO<T, NT, 0>::hbmLoad(r, amps, base | loadL.tg, loadL, a.localPos);
             tileBody<T, NT>(amps, a, ops, base, r, tile, row, first, tid);
+            base = nextBase(base);
             if (a.usesLds) __syncthreads();
         }
     }

@@ -277,6 +277,23 @@ int qa_statevec_calcProbsOfAllMultiQubitOutcomes(double* outProbs, const void* a
                                                  int numQubits, const int* qubits, int num);
 int qa_densmatr_calcProbsOfAllMultiQubitOutcomes(double* outProbs, const void* amps, int prec,
                                                  int numQubitsN, const int* qubits, int num);
+/* No counterpart in the reference: for callers that cannot allocate (code
+   under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc).  The statevector outcome table of
+   qa_statevec_calcProbsOfAllMultiQubitOutcomes, with the same kernels and
+   launch shape (so the same sums), written into the caller's DEVICE array
+   devOutProbs (2^num doubles) on the library stream; returns without
+   synchronising (qa_sync, or an event on qa_getStream(), orders reads).
+   It allocates nothing: its partial sums live in a 16 MiB device array that
+   each device holds for the life of the process; tables of 1..24 qubits
+   fit, anything larger is rejected with QA_ERR_ARG.  Every call on a device
+   shares that array, and every partial is written before it is read, so
+   nothing needs clearing between calls; calls whose kernels overlap on one
+   device (the same scratch in flight on two streams) race.  Gates still
+   queued for this state are an error (a flush may allocate): call
+   qa_flush(amps) first. */
+int qa_statevec_calcProbsOfAllMultiQubitOutcomesNoAlloc(double* devOutProbs, const void* amps, int prec,
+                                                        int numQubits, const int* qubits, int num);
 /* out[2] = sum conj(a_i) b_i */
 int qa_statevec_calcInnerProduct(const void* ampsA, const void* ampsB, int prec, int numQubits,
                                  double* out);

@@ -239,6 +239,11 @@ void bumpEpoch();
 double* devScratch(size_t numDoubles);
 void trimDevScratch();  // frees the device partials if they grew past 32 MiB
 double* hostScratch(size_t numDoubles);
+// the device the library is bound to (qa_init), -1 before
+int device();
+// takes the device address of the no-allocation outcome-table scratch for
+// `device`, which must be current (qa_reduce.hip; called by qa_init)
+int initNoAllocScratch(int device);
 
 }  // namespace qa
 

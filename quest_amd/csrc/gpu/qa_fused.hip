@@ -1115,6 +1115,12 @@ int flush(void* amps) {
     return rc;
 }
 
+bool pending(const void* amps) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_queues.find(const_cast<void*>(amps));
+    return it != g_queues.end() && !it->second.ops.empty();
+}
+
 int flushAll() {
     std::lock_guard<std::mutex> lk(g_mu);
     int rc = QA_OK;

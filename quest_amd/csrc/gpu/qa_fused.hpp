@@ -37,6 +37,8 @@ bool enabledFor(void* amps, int prec, int numQubits);
 int enqueue(void* amps, int prec, int numQubits, const GOp& op, bool& accepted);
 int flush(void* amps);
 int flushAll();
+// true when gates are queued for this state (a flush would launch passes)
+bool pending(const void* amps);
 void drop(void* amps);
 void shutdown();  // discards every queue and frees the program ring
 

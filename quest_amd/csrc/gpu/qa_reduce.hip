@@ -418,24 +418,52 @@ __global__ void __launch_bounds__(BLOCK) k_mergeSplitBins(const double* __restri
     out[b] = acc.s + acc.c;
 }
 
+// The launch shape of an outcome table of `num` qubits over numTerms terms:
+// register bins in `blocks` workgroups for num <= 4, else 2^num << logR
+// threads each summing every R-th term of one outcome; `partials` is the
+// scratch (doubles) the launches need besides the 2^num results
+struct OutcomeShape {
+    int blocks = 0;
+    int logR = 0;
+    int logRest = 0;
+    uint64_t threads = 0;
+    size_t partials = 0;
+};
+
+OutcomeShape outcomeShape(uint64_t numTerms, int num) {
+    OutcomeShape s;
+    const uint64_t want = (numTerms + (uint64_t)BLOCK * 4 - 1) / ((uint64_t)BLOCK * 4);
+    s.blocks = (int)(want < 1 ? 1 : (want > 256 ? 256 : want));
+    if (num <= 4) {
+        s.partials = (size_t)s.blocks * 16 * 2;
+        return s;
+    }
+    // about 2^20 threads: R partial sums per outcome
+    int logTerms = 0;
+    while ((1ull << logTerms) < numTerms) logTerms++;
+    s.logRest = logTerms - num;
+    s.logR = std::max(0, std::min(s.logRest, 20 - num));
+    s.threads = (1ull << num) << s.logR;
+    s.partials = s.logR ? 2 * (size_t)s.threads : 0;
+    return s;
+}
+
+// the launches of an outcome table on the library stream: partial sums in
+// `scratch` (at least shape.partials doubles, every one written before it
+// is read), the 2^num probabilities into the device array `res`
 template <typename T>
-int allOutcomes(double* out, const T* a, uint64_t numTerms, const int* qubits, int num, uint64_t diagStride) {
+int launchAllOutcomes(double* res, double* scratch, const OutcomeShape& s, const T* a, uint64_t numTerms,
+                      const int* qubits, int num, uint64_t diagStride) {
     TargSpec ts;
     ts.num = num;
     for (int i = 0; i < num; i++) ts.targ[i] = qubits[i];
-    uint64_t nb = 1ull << num;
-    uint64_t want = (numTerms + (uint64_t)BLOCK * 4 - 1) / ((uint64_t)BLOCK * 4);
-    int blocks = (int)(want < 1 ? 1 : (want > 256 ? 256 : want));
+    const uint64_t nb = 1ull << num;
     if (num <= 4) {
-        double* scratch = devScratch((size_t)blocks * 16 * 2 + 64);
-        double* host = hostScratch(16);
-        if (!scratch || !host) return QA_ERR_OOM;
-        double* res = scratch + (size_t)blocks * 16 * 2;
         switch (num) {
-#define QA_CASE(K)                                                                                   \
-    case K:                                                                                          \
-        k_allOutcomesReg<T, K><<<blocks, BLOCK, 0, stream()>>>(a, numTerms, ts, diagStride, scratch); \
-        k_finalizeBins<1 << K><<<1, BLOCK, 0, stream()>>>(scratch, blocks, res);                     \
+#define QA_CASE(K)                                                                                     \
+    case K:                                                                                            \
+        k_allOutcomesReg<T, K><<<s.blocks, BLOCK, 0, stream()>>>(a, numTerms, ts, diagStride, scratch); \
+        k_finalizeBins<1 << K><<<1, BLOCK, 0, stream()>>>(scratch, s.blocks, res);                     \
         break;
             QA_CASE(1)
             QA_CASE(2)
@@ -444,40 +472,82 @@ int allOutcomes(double* out, const T* a, uint64_t numTerms, const int* qubits, i
 #undef QA_CASE
         }
         QA_LAUNCH_CHECK();
-        QA_HIP(hipMemcpyAsync(host, res, nb * sizeof(double), hipMemcpyDeviceToHost, stream()));
-        QA_HIP(hipStreamSynchronize(stream()));
-        for (uint64_t b = 0; b < nb; b++) out[b] = host[b];
         return QA_OK;
     }
-    // about 2^20 threads: R partial sums per outcome
-    const int logNb = num;
-    int logTerms = 0;
-    while ((1ull << logTerms) < numTerms) logTerms++;
-    const int logRest = logTerms - num;
-    const int logR = std::max(0, std::min(logRest, 20 - logNb));
-    const uint64_t threads = nb << logR;
     // the other bits of a term's index: zeros inserted at the table's qubits
     InsertSpec rest;
     rest.num = num;
     rest.valueMask = 0;
     for (int i = 0; i < num; i++) rest.pos[i] = qubits[i];
     std::sort(rest.pos, rest.pos + num);
-    const size_t partialDoubles = logR ? 2 * (size_t)threads : 0;
-    double* scratch = devScratch(partialDoubles + (size_t)nb);
-    if (!scratch) return QA_ERR_OOM;
-    double* res = scratch + partialDoubles;
-    k_allOutcomesSplit<T><<<gridFor(threads, 1), BLOCK, 0, stream()>>>(a, 1ull << logRest, rest, ts, diagStride, logR,
-                                                                      threads, logR ? scratch : res);
+    k_allOutcomesSplit<T><<<gridFor(s.threads, 1), BLOCK, 0, stream()>>>(a, 1ull << s.logRest, rest, ts, diagStride,
+                                                                        s.logR, s.threads, s.logR ? scratch : res);
     QA_LAUNCH_CHECK();
-    if (logR) {
-        k_mergeSplitBins<<<(unsigned)((nb + BLOCK - 1) / BLOCK), BLOCK, 0, stream()>>>(scratch, logR, nb, res);
+    if (s.logR) {
+        k_mergeSplitBins<<<(unsigned)((nb + BLOCK - 1) / BLOCK), BLOCK, 0, stream()>>>(scratch, s.logR, nb, res);
         QA_LAUNCH_CHECK();
+    }
+    return QA_OK;
+}
+
+template <typename T>
+int allOutcomes(double* out, const T* a, uint64_t numTerms, const int* qubits, int num, uint64_t diagStride) {
+    const uint64_t nb = 1ull << num;
+    const OutcomeShape s = outcomeShape(numTerms, num);
+    double* scratch = devScratch(s.partials + (num <= 4 ? 64 : (size_t)nb));
+    if (!scratch) return QA_ERR_OOM;
+    double* res = scratch + s.partials;
+    int rc = launchAllOutcomes<T>(res, scratch, s, a, numTerms, qubits, num, diagStride);
+    if (rc != QA_OK) return rc;
+    if (num <= 4) {
+        double* host = hostScratch(16);
+        if (!host) return QA_ERR_OOM;
+        QA_HIP(hipMemcpyAsync(host, res, nb * sizeof(double), hipMemcpyDeviceToHost, stream()));
+        QA_HIP(hipStreamSynchronize(stream()));
+        for (uint64_t b = 0; b < nb; b++) out[b] = host[b];
+        return QA_OK;
     }
     QA_HIP(hipMemcpyAsync(out, res, nb * sizeof(double), hipMemcpyDeviceToHost, stream()));
     QA_HIP(hipStreamSynchronize(stream()));
     trimDevScratch();
     return QA_OK;
 }
+
+// ---- outcome tables without allocation -----------------------------------
+//
+// Scratch of qa_statevec_calcProbsOfAllMultiQubitOutcomesNoAlloc: a
+// file-scope device array, one copy per device for the life of the
+// process.  Its largest use is 2 * 2^20 doubles (num > 4: 2^20 threads,
+// one compensated pair each), so every table the allocating entry point
+// accepts (1..24 qubits) fits, with the same launch shape and therefore
+// the same sums.  Host code reaches it only through the address that
+// hipGetSymbolAddress gives, taken once per device by qa_init.
+constexpr size_t NOALLOC_SCRATCH_DOUBLES = (size_t)1 << 21;  // 16 MiB
+constexpr int NOALLOC_MAX_DEVICES = 64;
+__device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+double* g_scratchPtr[NOALLOC_MAX_DEVICES] = {};
+
+}  // namespace
+
+// called by qa_init with `device` current
+int initNoAllocScratch(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) {
+        setError("qa_init: device %d beyond the %d devices of the no-allocation scratch table", device,
+                 NOALLOC_MAX_DEVICES);
+        return QA_ERR_ARG;
+    }
+    if (g_scratchPtr[device]) return QA_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        setError("qa_init: hipGetSymbolAddress of the no-allocation scratch failed: %s", hipGetErrorString(e));
+        return QA_ERR_HIP;
+    }
+    g_scratchPtr[device] = static_cast<double*>(p);
+    return QA_OK;
+}
+
+namespace {
 
 bool checkArgs(const void* a, int prec, int nq, const char* caller) {
     if (!a || (prec != QA_FP32 && prec != QA_FP64) || nq < 1 || nq > 62) {
@@ -606,6 +676,32 @@ int qa_statevec_calcProbsOfAllMultiQubitOutcomes(double* out, const void* amps, 
     QA_ARG_CHECK(num >= 1 && num <= 24, "%s: %d qubits unsupported", __func__, num);
     int rc;
     QA_DISPATCH(prec, rc = allOutcomes<T>(out, (const T*)amps, 1ull << nq, qubits, num, 0));
+    return rc;
+}
+
+int qa_statevec_calcProbsOfAllMultiQubitOutcomesNoAlloc(double* devOutProbs, const void* amps, int prec, int nq,
+                                                        const int* qubits, int num) {
+    QA_REQUIRE_INIT_RO();
+    if (!checkArgs(amps, prec, nq, __func__)) return QA_ERR_ARG;
+    QA_ARG_CHECK(devOutProbs, "%s: NULL output", __func__);
+    InsertSpec chk;
+    if (!outcomeSpec(chk, nq, qubits, nullptr, num, __func__)) return QA_ERR_ARG;
+    QA_ARG_CHECK(num >= 1 && num <= 24, "%s: %d qubits unsupported", __func__, num);
+    // a flush could allocate the fusion program ring: the caller flushes first
+    QA_ARG_CHECK(!fuse::pending(amps), "%s: gates are queued for this state; call qa_flush(amps) first", __func__);
+    const int dev = device();
+    double* sp = (dev >= 0 && dev < NOALLOC_MAX_DEVICES) ? g_scratchPtr[dev] : nullptr;
+    if (!sp) {
+        setError("%s: no scratch address for device %d", __func__, dev);
+        return QA_ERR_NOT_INIT;
+    }
+    const uint64_t numTerms = 1ull << nq;
+    const OutcomeShape s = outcomeShape(numTerms, num);
+    QA_ARG_CHECK(s.partials <= NOALLOC_SCRATCH_DOUBLES,
+                 "%s: a %d-qubit table of a %d-qubit state needs %zu doubles of scratch, more than the %zu of the "
+                 "no-allocation scratch", __func__, num, nq, s.partials, NOALLOC_SCRATCH_DOUBLES);
+    int rc;
+    QA_DISPATCH(prec, rc = launchAllOutcomes<T>(devOutProbs, sp, s, (const T*)amps, numTerms, qubits, num, 0));
     return rc;
 }
 

@@ -59,6 +59,7 @@ void setError(const char* fmt, ...) {
 }
 hipStream_t stream() { return g_stream; }
 bool isInit() { return g_init; }
+int device() { return g_device; }
 
 std::atomic<uint64_t> g_epoch{1};
 void bumpEpoch() { g_epoch.fetch_add(1, std::memory_order_relaxed); }
@@ -139,6 +140,10 @@ int qa_init(int device) {
     if (!qa::devScratch(1u << 16) || !qa::hostScratch(4096)) {
         qa::setError("qa_init: scratch allocation failed");
         return QA_ERR_OOM;
+    }
+    {
+        const int rc = qa::initNoAllocScratch(device);
+        if (rc != QA_OK) return rc;
     }
     g_device = device;
     g_init = true;

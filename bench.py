@@ -352,6 +352,7 @@ def precision_bench(args) -> int:
         calls = build_step(Q, qc, circ, q)
         run_calls(calls)
         Q.syncQuESTEnv()
+        jit_ready(Q, args)
         gpu.qa_setFusionTiming(1)
         ms = timed_steps(Q, gpu, calls, 1)
         passes, fg, pms, pfl = C.c_int64(), C.c_int64(), C.c_double(), C.c_double()
@@ -418,7 +419,25 @@ def apply_fusion_args(Q, args) -> bool:
         Q.set_fusion(False)
     Q.gpu().qa_setFusionPrefetch(int(args.prefetch))
     Q.set_fusion_exact(bool(args.exact))
+    # pass kernels generated at run time: every program shape is compiled in
+    # the background at first sight (warmup steps run in the interpreter
+    # meanwhile; jit_ready() waits for the compiles before the timed steps)
+    Q.set_fusion_jit(int(args.jit), 1)
     return fused
+
+
+def jit_ready(Q, args) -> dict | None:
+    """Waits for the queued pass-kernel compiles (outside the timed region)
+    and returns the JIT counters so far."""
+    if not args.jit:
+        return None
+    t = time.perf_counter()
+    Q.fusion_jit_wait()
+    st = Q.fusion_jit_stats()
+    st["wait_seconds"] = time.perf_counter() - t
+    log(f"jit: {st['kernels']} pass kernels compiled ({st['compile_seconds']:.1f} s of compile threads, "
+        f"waited {st['wait_seconds']:.1f} s after warmup), {st['failures']} failed")
+    return st
 
 
 def main() -> int:
@@ -449,6 +468,9 @@ def main() -> int:
                          "commuting gates and merge runs on the same one or two qubits into one matrix")
     ap.add_argument("--block-min-gates", type=int, default=4,
                     help="precision mode: runs of at least this many gates on 5 qubits become FP32 MFMA blocks")
+    ap.add_argument("--jit", type=int, default=1,
+                    help="pass kernels generated at run time for each program shape (1) or every pass in the "
+                         "interpreter kernel (0)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibration-copy", action="store_true",
                     help="copy the state once (one device copy dispatch of known size) before the timed steps, "
@@ -520,6 +542,8 @@ def main() -> int:
         Q.syncQuESTEnv()
         log(f"warmup step {w}: {time.perf_counter() - t:.2f} s")
 
+    jit0 = jit_ready(Q, args)
+
     # timed region: every tile pass is bracketed by its own pair of HIP
     # events on the library stream (qa_setFusionTiming)
     gpu.qa_setFusionTiming(1)
@@ -531,6 +555,13 @@ def main() -> int:
     prog_ops, layouts = C.c_int64(), C.c_int64()
     gpu.qa_getFusionProgramStats(C.byref(prog_ops), C.byref(layouts))
     gpu.qa_setFusionTiming(0)
+    jit = None
+    if jit0 is not None:
+        j1 = Q.fusion_jit_stats()
+        jit = {"kernels": j1["kernels"], "failures": j1["failures"],
+               "compile_thread_seconds": round(j1["compile_seconds"], 2),
+               "wait_after_warmup_seconds": round(jit0["wait_seconds"], 2),
+               "timed_passes_in_generated_kernels": j1["launches"] - jit0["launches"]}
     value = args.steps * gates / (total_ms / 1e3)
     state = 16 * (1 << (2 * n if dens else n))
     pass_bytes = 2 * state
@@ -616,6 +647,7 @@ def main() -> int:
                         "gates_per_pass": (fgates.value / passes.value) if passes.value else None}
                        if fused else "off"),
         },
+        "jit": jit,
         "hbm_fraction_unfused_bytes": (args.steps * alg_bytes / (total_ms / 1e3)) / HBM_PEAK,
         "roofline": roofline,
         "unfused": unfused,

@@ -155,6 +155,23 @@ int qa_setFusionExact(int exact);
    timing on, each pass launch is bracketed by HIP events on the library
    stream and passMs sums their elapsed times (qa_getFusionStats waits for
    them) */
+/* Pass kernels generated at run time: a pass program's shape (op kinds,
+   register bits and masks, group layouts, which conditions are present) is
+   compiled with hiprtc into a straight-line kernel, cached per shape, with
+   the interpreter kernel's arithmetic bit for bit.  mode 0: off (every pass
+   in the interpreter); 1 (default): compiles on background threads once a
+   shape has been planned minSeen times (default 2), passes of a shape
+   without a kernel yet run in the interpreter; 2: compile at first use and
+   wait.  A shape whose compile fails stays interpreted
+   (qa_fusionJitLastError). */
+int qa_setFusionJit(int mode, int minSeen);
+int qa_getFusionJit(int* mode, int* minSeen);
+/* waits until every queued compile has finished */
+int qa_fusionJitWait(void);
+/* kernels compiled, compiles failed, passes run by generated kernels, and the
+   compile threads' total seconds */
+int qa_getFusionJitStats(int64_t* kernels, int64_t* failures, int64_t* launches, double* compileSeconds);
+const char* qa_fusionJitLastError(void);
 int qa_setFusionTiming(int enabled);
 int qa_getFusionStats(int64_t* passes, int64_t* gates, double* passMs, double* flops);
 /* program size of the tile passes since the last call (which resets them):

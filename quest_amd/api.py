@@ -508,6 +508,30 @@ def set_fusion_exact(exact: bool) -> None:
         raise QuESTError(f"qa_setFusionExact failed ({rc}): {_gpu_last_error()}")
 
 
+def set_fusion_jit(mode: int, min_seen: int = 2) -> None:
+    """Pass kernels generated at run time (qa_setFusionJit): mode 0 off, 1
+    background compiles once a program shape has been planned min_seen
+    times, 2 compile at first use and wait."""
+    rc = gpu().qa_setFusionJit(C.c_int(int(mode)), C.c_int(int(min_seen)))
+    if rc != 0:
+        raise QuESTError(f"qa_setFusionJit failed ({rc}): {_gpu_last_error()}")
+
+
+def fusion_jit_wait() -> None:
+    """Blocks until every queued pass-kernel compile has finished."""
+    gpu().qa_fusionJitWait()
+
+
+def fusion_jit_stats() -> dict:
+    """Generated pass kernels compiled, failed compiles, passes they ran and
+    the compile threads' seconds, since the library was initialised."""
+    k, f, n, sec = C.c_int64(), C.c_int64(), C.c_int64(), C.c_double()
+    gpu().qa_getFusionJitStats(C.byref(k), C.byref(f), C.byref(n), C.byref(sec))
+    gpu().qa_fusionJitLastError.restype = C.c_char_p
+    return {"kernels": k.value, "failures": f.value, "launches": n.value, "compile_seconds": sec.value,
+            "last_error": (gpu().qa_fusionJitLastError() or b"").decode()}
+
+
 def ensure_env() -> None:
     if not isQuESTEnvInit():
         initQuESTEnv()

@@ -36,6 +36,7 @@
 #include "qa_fused.hpp"
 #include "qa_tile.hpp"
 #include "qa_tilepass.hpp"
+#include "qa_jit.hpp"
 
 namespace qa {
 namespace fuse {
@@ -977,7 +978,11 @@ int launchPass(void* amps, const Queue& qu, const std::vector<GOp>& ops, size_t 
         setError("fusion: the tile-pass kernel has static LDS; its LDS tile would not start at address 0");
         return QA_ERR_HIP;
     }
-    const unsigned grid = (unsigned)std::min<uint64_t>(a.numTiles, residentBlocks(fn, block, lds));
+    // the pass's generated kernel when its shape has one (checked for static
+    // LDS and scratch when it was loaded), else the interpreter
+    const void* jfn = jit::kernelFor(qu.prec, nt, pf, a, prog.data(), num);
+    const unsigned grid = (unsigned)std::min<uint64_t>(
+        a.numTiles, jfn ? jit::residentBlocks(jfn, block, lds) : residentBlocks(fn, block, lds));
     a.tileStep = 0;
     for (int i = 0; i < a.numNonLocal && i < 32; i++) a.tileStep |= (uint64_t)((grid >> i) & 1u) << a.nonLocalPos[i];
     std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
@@ -991,7 +996,16 @@ int launchPass(void* amps, const Queue& qu, const std::vector<GOp>& ops, size_t 
         ev = &g_stats.pool[g_stats.used++];
         QA_HIP(hipEventRecord(ev->first, stream()));
     }
-    launchTileAny(qu.prec, nt, pf, grid, block, lds, amps, a, d);
+    if (jfn) {
+        const int e = jit::launch(jfn, grid, block, lds, amps, a, d);
+        if (e != (int)hipSuccess) {
+            setError("fusion: launch of a generated pass kernel failed: %s", hipGetErrorString((hipError_t)e));
+            return QA_ERR_HIP;
+        }
+        jit::countLaunch();
+    } else {
+        launchTileAny(qu.prec, nt, pf, grid, block, lds, amps, a, d);
+    }
     QA_LAUNCH_CHECK();
     if (ev) QA_HIP(hipEventRecord(ev->second, stream()));
     g_stats.passes++;
@@ -1139,6 +1153,7 @@ void drop(void* amps) {
 
 void shutdown() {
     std::lock_guard<std::mutex> lk(g_mu);
+    jit::shutdown();
     g_queues.clear();
     for (auto& e : g_stats.pool) {
         (void)hipEventDestroy(e.first);

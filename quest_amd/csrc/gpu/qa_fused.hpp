@@ -3,10 +3,11 @@
 
 #include <cstdint>
 
+#include "qa_kinds.hpp"
+
 namespace qa {
 namespace fuse {
 
-enum Kind : int { F_DENSE1 = 1, F_DENSE2, F_DIAG1, F_PARITY, F_PAULI, F_SWAP, F_DEPOL1, F_DEPH1, F_DENSE1V };
 
 // a queued gate in global index space
 struct GOp {

@@ -1,0 +1,453 @@
+// Pass kernels generated at run time: source generation from a pass
+// program's shape, hiprtc compiles on background threads, a per-shape
+// cache of loaded kernels, and their launches.  See qa_jit.hpp.
+#include <hip/hiprtc.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "qa_common.hpp"
+#include "qa_jit.hpp"
+
+// the device headers, embedded by the build (quest_amd/build.py)
+extern "C" const int qa_jit_num_headers;
+extern "C" const char* const qa_jit_header_names[];
+extern "C" const char* const qa_jit_header_sources[];
+
+namespace qa {
+namespace jit {
+namespace {
+
+using fuse::TOp;
+using fuse::TileArgs;
+
+enum State { NONE, QUEUED, READY, FAILED };
+
+struct Entry {
+    int seen = 0;
+    State state = NONE;
+    hipModule_t module = nullptr;
+    hipFunction_t fn = nullptr;
+    unsigned residentBlock = 0, residentValue = 0;  // residentBlocks cache (block size -> blocks)
+    size_t residentLds = 0;
+};
+
+struct Job {
+    std::string key, src;
+    int prec;
+};
+
+std::mutex g_mu;  // everything below
+std::condition_variable g_work, g_done;
+std::unordered_map<std::string, Entry> g_cache;
+std::unordered_map<const void*, Entry*> g_byFn;
+std::deque<Job> g_jobs;
+int g_inFlight = 0;
+std::vector<std::thread> g_workers;
+bool g_stop = false;
+Settings g_settings;
+Stats g_stats;
+std::string g_lastError;
+
+// stops and joins the compile threads when the library is unloaded without
+// qa_finalize (a joinable std::thread may not be destroyed); declared after
+// the state it uses, so it is destroyed first
+struct WorkerJoin {
+    ~WorkerJoin() {
+        std::vector<std::thread> workers;
+        {
+            std::lock_guard<std::mutex> lk(g_mu);
+            g_stop = true;
+            g_jobs.clear();
+            workers.swap(g_workers);
+        }
+        g_work.notify_all();
+        for (auto& w : workers) w.join();
+    }
+} g_workerJoin;
+
+// ---- shapes ---------------------------------------------------------------
+
+bool hasGc(const TOp& t) { return t.h.gcMask || t.h.gcVal; }
+bool hasJc(const TOp& t) { return t.h.jcMask || t.h.jcVal; }
+
+// the shape of one op: what its generated code writes in as constants
+std::string opShape(const TOp& t) {
+    char buf[192];
+    const int* w = reinterpret_cast<const int*>(t.b);
+    switch (t.h.kind) {
+        case fuse::F_GROUP:
+            snprintf(buf, sizeof buf, "G%d,%d,%d,%d;", w[0], w[1], w[2], w[3]);
+            break;
+        case fuse::F_PAULI_LDS:
+            snprintf(buf, sizeof buf, "L;");
+            break;
+        case fuse::F_BLOCK:
+            snprintf(buf, sizeof buf, "B%d,%d,%d,%d,%d;", w[0], w[1], w[2], w[3], w[4]);
+            break;
+        default:
+            snprintf(buf, sizeof buf, "O%d,%d,%d,%u,%u,%u,%u,%u,%d,%d;", t.h.kind, t.h.rb1, t.h.rb2, t.h.pad, t.h.rm1,
+                     t.h.rm2, t.h.rcMask, t.h.rcVal, hasGc(t) ? 1 : 0, hasJc(t) ? 1 : 0);
+    }
+    return buf;
+}
+
+const char* typeName(int prec) { return prec == QA_FP64 ? "double" : "float"; }
+
+// ---- compiles -------------------------------------------------------------
+
+bool compileOne(const Job& job, hipModule_t& mod, hipFunction_t& fn, std::string& err) {
+    hiprtcProgram p;
+    if (hiprtcCreateProgram(&p, job.src.c_str(), "qa_jit_pass.hip", qa_jit_num_headers, qa_jit_header_sources,
+                            qa_jit_header_names) != HIPRTC_SUCCESS) {
+        err = "hiprtcCreateProgram failed";
+        return false;
+    }
+    // the options the library's own tile-pass kernels are built with, but the
+    // default scheduler: straight-line programs scheduled for ILP can need
+    // more than the 256 registers of two waves per SIMD and spill
+    std::vector<const char*> opts = {"--offload-arch=gfx950", "-O3", "-std=c++17",
+                                     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                                     "-mllvm", "-simplifycfg-sink-common=false"};
+    const hiprtcResult rc = hiprtcCompileProgram(p, (int)opts.size(), opts.data());
+    if (rc != HIPRTC_SUCCESS) {
+        size_t n = 0;
+        hiprtcGetProgramLogSize(p, &n);
+        std::string log(n + 1, '\0');
+        if (n) hiprtcGetProgramLog(p, &log[0]);
+        err = std::string("hiprtc: ") + hiprtcGetErrorString(rc) + ": " + log.substr(0, 400);
+        hiprtcDestroyProgram(&p);
+        return false;
+    }
+    size_t n = 0;
+    hiprtcGetCodeSize(p, &n);
+    std::vector<char> code(n);
+    hiprtcGetCode(p, code.data());
+    hiprtcDestroyProgram(&p);
+    if (hipModuleLoadData(&mod, code.data()) != hipSuccess) {
+        err = "hipModuleLoadData failed";
+        return false;
+    }
+    if (hipModuleGetFunction(&fn, mod, "qa_jit_pass") != hipSuccess) {
+        err = "hipModuleGetFunction failed";
+        (void)hipModuleUnload(mod);
+        return false;
+    }
+    // the tile must sit at LDS address 0 (no static LDS) and the register
+    // array must not have gone to scratch memory
+    int shared = -1, local = -1;
+    (void)hipFuncGetAttribute(&shared, HIP_FUNC_ATTRIBUTE_SHARED_SIZE_BYTES, fn);
+    (void)hipFuncGetAttribute(&local, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, fn);
+    if (shared != 0 || local != 0) {
+        char buf[128];
+        snprintf(buf, sizeof buf, "generated kernel has %d bytes of static LDS and %d bytes of scratch", shared, local);
+        err = buf;
+        (void)hipModuleUnload(mod);
+        return false;
+    }
+    return true;
+}
+
+void worker(int device) {
+    (void)hipSetDevice(device);
+    for (;;) {
+        Job job;
+        {
+            std::unique_lock<std::mutex> lk(g_mu);
+            g_work.wait(lk, [] { return g_stop || !g_jobs.empty(); });
+            if (g_stop) return;
+            job = std::move(g_jobs.front());
+            g_jobs.pop_front();
+            g_inFlight++;
+        }
+        const auto t0 = std::chrono::steady_clock::now();
+        hipModule_t mod = nullptr;
+        hipFunction_t fn = nullptr;
+        std::string err;
+        const bool ok = compileOne(job, mod, fn, err);
+        const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        {
+            std::lock_guard<std::mutex> lk(g_mu);
+            Entry& e = g_cache[job.key];
+            if (ok) {
+                e.state = READY;
+                e.module = mod;
+                e.fn = fn;
+                g_byFn[(const void*)fn] = &e;
+                g_stats.kernels++;
+            } else {
+                e.state = FAILED;
+                g_stats.failures++;
+                g_lastError = err;
+                // QA_JIT_DUMP_DIR: keep the source of each kernel that failed, for diagnosis
+                if (const char* dir = std::getenv("QA_JIT_DUMP_DIR")) {
+                    char path[512];
+                    snprintf(path, sizeof path, "%s/failed_%lld.hip", dir, (long long)g_stats.failures);
+                    if (FILE* fp = std::fopen(path, "w")) {
+                        std::fprintf(fp, "// %s\n", err.c_str());
+                        std::fputs(job.src.c_str(), fp);
+                        std::fclose(fp);
+                    }
+                }
+            }
+            g_stats.compileSeconds += dt;
+            g_inFlight--;
+        }
+        g_done.notify_all();
+    }
+}
+
+// under g_mu
+void ensureWorkers() {
+    if (!g_workers.empty()) return;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    unsigned hw = std::thread::hardware_concurrency();
+    int n = (int)(hw > 2 ? hw - 1 : 1);
+    if (n > 16) n = 16;
+    g_stop = false;
+    for (int i = 0; i < n; i++) g_workers.emplace_back(worker, dev);
+}
+
+}  // namespace
+
+// ---- source generation ----------------------------------------------------
+
+std::string shapeKey(int prec, bool nt, bool pf, const TileArgs& a, const TOp* prog, size_t num) {
+    char buf[128];
+    snprintf(buf, sizeof buf, "p%d n%d f%d k%d d%d%d%d o%d|", prec, nt ? 1 : 0, pf ? 1 : 0, a.k, a.firstDirect,
+             a.lastDirect, a.usesLds, a.numOps);
+    std::string k = buf;
+    for (size_t i = 0; i < num; i++) k += opShape(prog[i]);
+    return k;
+}
+
+std::string generate(int prec, bool nt, bool pf, const TileArgs& a, const TOp* prog, size_t num) {
+    std::string s;
+    s.reserve(4096 + num * 512);
+    s += "#include \"qa_tiledev.hpp\"\n"
+         "namespace qa {\nnamespace fuse {\n"
+         "struct JitProgram {\n"
+         "    template <typename T, int NT>\n"
+         "    __device__ __forceinline__ static void run(T* __restrict__ amps, const TileArgs& a,\n"
+         "                                               const TOp* __restrict__ ops, uint64_t base, ct<T>* r,\n"
+         "                                               ct<T>* tile, const RegLayout& row, const RegLayout& first,\n"
+         "                                               uint32_t tid) {\n"
+         "        TileState st;\n"
+         "        tileBegin<T, NT>(st, a, r, tile, row, first);\n";
+    char buf[512];
+    for (size_t i = a.firstDirect ? 1 : 0; i < num; i++) {
+        const TOp& t = prog[i];
+        const int* w = reinterpret_cast<const int*>(t.b);
+        switch (t.h.kind) {
+            case fuse::F_GROUP:
+                snprintf(buf, sizeof buf,
+                         "        { const int slots[RB] = {%d, %d, %d, %d};\n"
+                         "          stepGroup<T, NT>(st, a, ops + %zu, slots, r, tile, tid); }\n",
+                         w[0], w[1], w[2], w[3], i);
+                s += buf;
+                break;
+            case fuse::F_PAULI_LDS:
+                snprintf(buf, sizeof buf,
+                         "        { const TOp* op_ = ops + %zu; u32x16 ch, cb; fetchOp(op_, ch, cb); OpRegs o;\n"
+                         "          unpackOp(ch, cb, o);\n"
+                         "          stepPauliLds<T, NT>(st, a, op_, o, ((uint64_t)cb[13] << 32) | cb[12], r, tile,\n"
+                         "                              base, tid); }\n",
+                         i);
+                s += buf;
+                break;
+            case fuse::F_BLOCK:
+                snprintf(buf, sizeof buf,
+                         "        { const TOp* op_ = ops + %zu; u32x16 ch, cb; fetchOp(op_, ch, cb);\n"
+                         "          const int bslots[BQ] = {%d, %d, %d, %d, %d};\n"
+                         "          stepBlock<T, NT>(st, op_, bslots, a.blocks + (cb[6] | ((uint64_t)cb[7] << 32)),\n"
+                         "                           a.k, r, tile, tid); }\n",
+                         i, w[0], w[1], w[2], w[3], w[4]);
+                s += buf;
+                break;
+            default:
+                // the op's arm alone, its shape fields as constants
+                snprintf(buf, sizeof buf,
+                         "        { const TOp* op_ = ops + %zu; u32x16 ch, cb; fetchOp(op_, ch, cb); OpRegs o;\n"
+                         "          unpackOp(ch, cb, o);\n"
+                         "          o.h.kind = %d; o.h.rb1 = %d; o.h.rb2 = %d; o.h.pad = %uu; o.h.rm1 = %uu;\n"
+                         "          o.h.rm2 = %uu; o.h.rcMask = %uu; o.h.rcVal = %uu;\n"
+                         "          stepFixed<T, %d, %d, %d, %uu, %uu, %s, %s, %s>(st, op_, o, r, base); }\n",
+                         i, t.h.kind, t.h.rb1, t.h.rb2, t.h.pad, t.h.rm1, t.h.rm2, t.h.rcMask, t.h.rcVal, t.h.kind,
+                         t.h.rb1, t.h.rb2, t.h.pad, t.h.rm1, hasGc(t) ? "true" : "false", hasJc(t) ? "true" : "false",
+                         t.h.rcMask ? "true" : "false");
+                s += buf;
+        }
+    }
+    s += "        tileEnd<T, NT>(st, a, amps, r, tile, row, base);\n"
+         "    }\n"
+         "};\n"
+         "}  // namespace fuse\n}  // namespace qa\n";
+    snprintf(buf, sizeof buf,
+             "extern \"C\" __global__ void __launch_bounds__(512)%s\n"
+             "qa_jit_pass(%s* __restrict__ amps, qa::fuse::TileArgs a, const qa::fuse::TOp* __restrict__ ops) {\n"
+             "    qa::fuse::tilePassLoop<%s, %d, %d, qa::fuse::JitProgram>(amps, a, ops);\n"
+             "}\n",
+             pf ? "" : " __attribute__((amdgpu_waves_per_eu(3)))", typeName(prec), typeName(prec), nt ? 1 : 0,
+             pf ? 1 : 0);
+    s += buf;
+    return s;
+}
+
+// ---- the cache ------------------------------------------------------------
+
+Settings settings() {
+    std::lock_guard<std::mutex> lk(g_mu);
+    return g_settings;
+}
+
+void setSettings(const Settings& s) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_settings = s;
+}
+
+const void* kernelFor(int prec, bool nt, bool pf, const TileArgs& a, const TOp* prog, size_t num) {
+    Settings s = settings();
+    if (s.mode == 0) return nullptr;
+    const std::string key = shapeKey(prec, nt, pf, a, prog, num);
+    std::unique_lock<std::mutex> lk(g_mu);
+    Entry& e = g_cache[key];  // element references survive rehashing
+    e.seen++;
+    if (e.state == READY) return (const void*)e.fn;
+    if (e.state == NONE && (s.mode == 2 || e.seen >= s.minSeen)) {
+        e.state = QUEUED;
+        lk.unlock();
+        std::string src = generate(prec, nt, pf, a, prog, num);
+        lk.lock();
+        g_jobs.push_back(Job{key, std::move(src), prec});
+        ensureWorkers();
+        g_work.notify_one();
+    }
+    if (s.mode == 2 && e.state == QUEUED) g_done.wait(lk, [&] { return e.state == READY || e.state == FAILED; });
+    return e.state == READY ? (const void*)e.fn : nullptr;
+}
+
+int launch(const void* kernel, unsigned grid, unsigned block, size_t lds, void* amps, const TileArgs& a,
+           const TOp* ops) {
+    void* am = amps;
+    TileArgs aa = a;
+    const TOp* op = ops;
+    void* args[] = {&am, &aa, &op};
+    return (int)hipModuleLaunchKernel((hipFunction_t)kernel, grid, 1, 1, block, 1, 1, (unsigned)lds, stream(), args,
+                                      nullptr);
+}
+
+unsigned residentBlocks(const void* kernel, unsigned block, size_t lds) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_byFn.find(kernel);
+    Entry* e = it == g_byFn.end() ? nullptr : it->second;
+    if (e && e->residentBlock == block && e->residentLds == lds && e->residentValue) return e->residentValue;
+    int perCU = 0, dev = 0, cus = 0;
+    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, (hipFunction_t)kernel, (int)block, lds) !=
+            hipSuccess ||
+        perCU < 1)
+        perCU = 1;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;
+    const unsigned r = (unsigned)(perCU * cus);
+    if (e) {
+        e->residentBlock = block;
+        e->residentLds = lds;
+        e->residentValue = r;
+    }
+    return r;
+}
+
+void waitIdle() {
+    std::unique_lock<std::mutex> lk(g_mu);
+    g_done.wait(lk, [] { return g_jobs.empty() && g_inFlight == 0; });
+}
+
+Stats stats() {
+    std::lock_guard<std::mutex> lk(g_mu);
+    return g_stats;
+}
+
+void countLaunch() {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_stats.launches++;
+}
+
+const char* lastError() {
+    static thread_local std::string copy;
+    std::lock_guard<std::mutex> lk(g_mu);
+    copy = g_lastError;
+    return copy.c_str();
+}
+
+void shutdown() {
+    std::vector<std::thread> workers;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_stop = true;
+        g_jobs.clear();
+        workers.swap(g_workers);
+    }
+    g_work.notify_all();
+    for (auto& w : workers) w.join();
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (auto& kv : g_cache)
+        if (kv.second.module) (void)hipModuleUnload(kv.second.module);
+    g_cache.clear();
+    g_byFn.clear();
+    g_inFlight = 0;
+    g_stop = false;
+    g_stats = Stats();
+    g_lastError.clear();
+}
+
+}  // namespace jit
+}  // namespace qa
+
+using namespace qa;
+
+extern "C" {
+
+int qa_setFusionJit(int mode, int minSeen) {
+    QA_ARG_CHECK(mode >= 0 && mode <= 2, "qa_setFusionJit: mode must be 0 (off), 1 (background) or 2 (at first use)");
+    QA_ARG_CHECK(minSeen >= 1, "qa_setFusionJit: minSeen must be at least 1");
+    jit::Settings s;
+    s.mode = mode;
+    s.minSeen = minSeen;
+    jit::setSettings(s);
+    return QA_OK;
+}
+
+int qa_getFusionJit(int* mode, int* minSeen) {
+    const jit::Settings s = jit::settings();
+    if (mode) *mode = s.mode;
+    if (minSeen) *minSeen = s.minSeen;
+    return QA_OK;
+}
+
+int qa_fusionJitWait(void) {
+    jit::waitIdle();
+    return QA_OK;
+}
+
+int qa_getFusionJitStats(int64_t* kernels, int64_t* failures, int64_t* launches, double* compileSeconds) {
+    const jit::Stats s = jit::stats();
+    if (kernels) *kernels = s.kernels;
+    if (failures) *failures = s.failures;
+    if (launches) *launches = s.launches;
+    if (compileSeconds) *compileSeconds = s.compileSeconds;
+    return QA_OK;
+}
+
+const char* qa_fusionJitLastError(void) { return jit::lastError(); }
+
+}  // extern "C"

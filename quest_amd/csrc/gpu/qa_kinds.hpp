@@ -1,0 +1,10 @@
+// The gate kinds of the fusion queue (host planner and device tile code).
+#pragma once
+
+namespace qa {
+namespace fuse {
+
+enum Kind : int { F_DENSE1 = 1, F_DENSE2, F_DIAG1, F_PARITY, F_PAULI, F_SWAP, F_DEPOL1, F_DEPH1, F_DENSE1V };
+
+}  // namespace fuse
+}  // namespace qa

@@ -1,7 +1,8 @@
-// The tile-pass kernels: one workgroup per tile (persistent), the tile's
-// amplitudes in registers, register-layout changes through LDS.  Included
-// by qa_fused.hip (complex64 instances) and qa_fused_f64.hip (complex128
-// instances, compiled with their own scheduling options).
+// The interpreter tile-pass kernels: persistent workgroups, a tile's
+// amplitudes in registers, register-layout changes through LDS, each op's
+// kind read from the program at run time.  Included by qa_fused.hip
+// (complex64 instances) and qa_fused_f64.hip (complex128 instances,
+// compiled with their own scheduling options).
 #pragma once
 
 #include "qa_common.hpp"
@@ -11,60 +12,16 @@
 namespace qa {
 namespace fuse {
 
-// NT: non-temporal HBM loads/stores, used when the contiguous low run of a
-// tile row is at least 2^STREAM_LOW_BIT amplitudes.  Persistent workgroups;
-// PF = 1: each prefetches its next tile into registers before working on
-// the current one, so HBM traffic overlaps the register and LDS work (16
-// more registers of amplitudes per thread); PF = 0: load, work, store.
-template <typename T, int NT, int PF>
-__device__ __forceinline__ void tilePass(T* __restrict__ amps, const TileArgs& a, const TOp* __restrict__ ops) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    ct<T>* tile = reinterpret_cast<ct<T>*>(smem);
-    const uint32_t tid = threadIdx.x;
-    RegLayout row, first;
-    passLayouts(row, first, a, ops, tid);
-    const RegLayout& loadL = a.firstDirect ? first : row;
-
-    // tile t's base is t deposited over the non-local positions; the next
-    // tile of this workgroup (t + gridDim.x) is one masked add away
-    auto nextBase = [&](uint64_t b) { return ((b | ~a.nonLocalMask) + a.tileStep) & a.nonLocalMask; };
-    uint64_t t = blockIdx.x;
-    uint64_t base = deposit(t, a.nonLocalPos, a.numNonLocal);
-    if constexpr (PF) {
-        ct<T> rn[RA];
-        if (t < a.numTiles) RegIO<T, NT, 0>::hbmLoad(rn, amps, base | loadL.tg, loadL, a.localPos);
-        for (; t < a.numTiles; t += gridDim.x) {
-            ct<T> r[RA];
-#pragma unroll
-            for (int v = 0; v < RA; v++) r[v] = rn[v];
-            const uint64_t baseN = nextBase(base);
-            if (t + gridDim.x < a.numTiles) RegIO<T, NT, 0>::hbmLoad(rn, amps, baseN | loadL.tg, loadL, a.localPos);
-            tileBody<T, NT>(amps, a, ops, base, r, tile, row, first, tid);
-            base = baseN;
-            // the next tile's first LDS writes must not overtake this tile's reads
-            if (a.usesLds) __syncthreads();
-        }
-    } else {
-        for (; t < a.numTiles; t += gridDim.x) {
-            ct<T> r[RA];
-            RegIO<T, NT, 0>::hbmLoad(r, amps, base | loadL.tg, loadL, a.localPos);
-            tileBody<T, NT>(amps, a, ops, base, r, tile, row, first, tid);
-            base = nextBase(base);
-            if (a.usesLds) __syncthreads();
-        }
-    }
-}
-
 // the prefetching form needs 16 more registers of amplitudes per thread;
 // the lean form is held to a register budget of three waves per SIMD
 template <typename T, int NT>
 __global__ void __launch_bounds__(512) k_tilePass(T* __restrict__ amps, TileArgs a, const TOp* __restrict__ ops) {
-    tilePass<T, NT, 1>(amps, a, ops);
+    tilePassLoop<T, NT, 1, Interpreter>(amps, a, ops);
 }
 template <typename T, int NT>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3)))
 k_tilePassLean(T* __restrict__ amps, TileArgs a, const TOp* __restrict__ ops) {
-    tilePass<T, NT, 0>(amps, a, ops);
+    tilePassLoop<T, NT, 0, Interpreter>(amps, a, ops);
 }
 
 // [precision FP32/FP64][non-temporal][prefetch]

@@ -18,6 +18,16 @@
 #include <vector>
 
 #include "qa_tile.hpp"
+#include "qa_jit.hpp"
+
+#include <hip/hiprtc.h>
+#include <map>
+#include <string>
+
+// the device headers the library embeds for its generated pass kernels
+extern "C" const int qa_jit_num_headers;
+extern "C" const char* const qa_jit_header_names[];
+extern "C" const char* const qa_jit_header_sources[];
 
 using namespace qa::fuse;
 typedef std::complex<double> cx;
@@ -247,15 +257,84 @@ static GOp randomGate(Rng& r, int n, bool dens, int hot) {
     return g;
 }
 
+// With "jit": every planned pass's program (complex128, and complex64 with
+// MFMA blocks) also gets its generated kernel source; programs of equal
+// shape key must have equal source (what makes the per-shape kernel cache
+// sound), and up to maxCompiles distinct sources are compiled for gfx950
+// with hiprtc, as the library does at run time (loading needs a GPU and is
+// left to the GPU tests).
+struct JitCheck {
+    bool on = false;
+    int maxCompiles = 0, compiled = 0;
+    std::map<std::string, std::string> byKey;
+    int check(const GOp* ops, size_t num, int n, uint64_t L, int low, bool exact, int c) {
+        for (int prec : {QA_FP64, QA_FP32}) {
+            TileArgs a;
+            std::vector<TOp> prog;
+            std::vector<float> blocks;
+            if (prec == QA_FP32) planPassBlocks(ops, num, n, L, low, 2, a, prog, blocks);
+            else planPass(ops, num, n, L, low, a, prog, !exact);
+            setLdsOffsets(prog.data(), prog.size(), a, prec);
+            const bool nt = low >= 3;
+            const std::string key = qa::jit::shapeKey(prec, nt, true, a, prog.data(), prog.size());
+            const std::string src = qa::jit::generate(prec, nt, true, a, prog.data(), prog.size());
+            auto it = byKey.find(key);
+            if (it != byKey.end()) {
+                if (it->second != src) {
+                    std::printf("case %d: two programs of one shape key generated different sources\n", c);
+                    return 1;
+                }
+                continue;
+            }
+            byKey[key] = src;
+            if (const char* dir = std::getenv("QA_JIT_DUMP_DIR")) {
+                // the source, for compiling outside (test infrastructure)
+                char path[512];
+                std::snprintf(path, sizeof path, "%s/pass_%04zu.hip", dir, byKey.size());
+                if (FILE* fp = std::fopen(path, "w")) {
+                    std::fputs(src.c_str(), fp);
+                    std::fclose(fp);
+                }
+            }
+            if (compiled >= maxCompiles) continue;
+            hiprtcProgram p;
+            hiprtcCreateProgram(&p, src.c_str(), "qa_jit_pass.hip", qa_jit_num_headers, qa_jit_header_sources,
+                                qa_jit_header_names);
+            const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature",
+                                  "-Xclang", "-packed-fp32-ops", "-mllvm", "-simplifycfg-sink-common=false"};
+            const hiprtcResult rc = hiprtcCompileProgram(p, 9, opts);
+            if (rc != HIPRTC_SUCCESS) {
+                size_t ln = 0;
+                hiprtcGetProgramLogSize(p, &ln);
+                std::string log(ln + 1, '\0');
+                hiprtcGetProgramLog(p, &log[0]);
+                std::printf("case %d: generated kernel does not compile (%s):\n%s\n", c, hiprtcGetErrorString(rc),
+                            log.substr(0, 2000).c_str());
+                return 1;
+            }
+            hiprtcDestroyProgram(&p);
+            compiled++;
+        }
+        return 0;
+    }
+};
+
 int main(int argc, char** argv) {
     const int cases = argc > 1 ? std::atoi(argv[1]) : 200;
     const uint64_t seed0 = argc > 2 ? std::strtoull(argv[2], nullptr, 10) : 1;
+    JitCheck jit;
+    if (argc > 3 && std::string(argv[3]) == "jit") {
+        jit.on = true;
+        jit.maxCompiles = argc > 4 ? std::atoi(argv[4]) : 4;
+    }
     int checked = 0;
     for (int c = 0; c < cases; c++) {
         Rng r(seed0 * 1000003ull + (uint64_t)c);
         const bool dens = c % 4 == 3;
-        const int n = dens ? 2 * (3 + r.below(3)) : 6 + r.below(5);
-        const int kMax = std::min(n, 4 + r.below(5));
+        // QA_PLANNER_WIDE: wider states and tiles (up to 14 qubits, 13-qubit tiles)
+        const bool wide = std::getenv("QA_PLANNER_WIDE") != nullptr;
+        const int n = wide ? (dens ? 2 * (5 + r.below(3)) : 11 + r.below(4)) : (dens ? 2 * (3 + r.below(3)) : 6 + r.below(5));
+        const int kMax = wide ? std::min(n, 9 + r.below(5)) : std::min(n, 4 + r.below(5));
         const int low = r.below(std::max(1, kMax - 1));
         const bool exact = c % 5 == 4;
         const int num = 20 + r.below(120);
@@ -313,6 +392,7 @@ int main(int argc, char** argv) {
                     return 1;
                 }
             }
+            if (jit.on && jit.check(planned.data(), planned.size(), n, L, low, exact, c)) return 1;
             // every planned op's dense targets are local to the pass
             for (const GOp& g : planned) {
                 if (denseMask(g) & ~L) {
@@ -341,6 +421,8 @@ int main(int argc, char** argv) {
         }
         checked++;
     }
+    if (jit.on)
+        std::printf("jit check: %zu shapes, %d generated kernels compiled\n", jit.byKey.size(), jit.compiled);
     std::printf("planner check: %d cases agree\n", checked);
     return 0;
 }

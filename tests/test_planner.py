@@ -25,7 +25,7 @@ def planner_check(tmp_path_factory):
         pytest.skip("hipcc not available")
     exe = tmp_path_factory.mktemp("planner") / "planner_check"
     cmd = [HIPCC, "-std=c++17", "-O2", f"-I{ROOT / 'quest_amd' / 'csrc' / 'gpu'}", f"-I{ROOT / 'include'}",
-           str(ROOT / "tests" / "planner_check.cpp"), "-o", str(exe), f"-L{GPU_LIB.parent}", "-lquest_amd_gpu",
+           str(ROOT / "tests" / "planner_check.cpp"), "-o", str(exe), f"-L{GPU_LIB.parent}", "-lquest_amd_gpu", "-lhiprtc",
            f"-Wl,-rpath,{GPU_LIB.parent}"]
     subprocess.run(cmd, check=True, capture_output=True, text=True, timeout=300)
     return exe
@@ -36,3 +36,13 @@ def test_planned_passes_act_like_the_queue(planner_check, seed):
     out = subprocess.run([str(planner_check), "300", str(seed)], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "300 cases agree" in out.stdout
+
+
+def test_generated_pass_kernels_compile_and_cache_soundly(planner_check):
+    """Every planned pass program also gets its generated kernel source:
+    programs with equal shape keys must generate equal sources (the per-shape
+    kernel cache depends on it), and a few distinct sources are compiled for
+    gfx950 with hiprtc, as the library does at run time."""
+    out = subprocess.run([str(planner_check), "60", "5", "jit", "4"], capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "4 generated kernels compiled" in out.stdout

@@ -42,7 +42,9 @@ struct Entry {
 };
 
 struct Job {
-    std::string key, src;
+    std::string key;
+    std::string src;      // the kernel as asked for
+    std::string srcLean;  // the same program without next-tile prefetch (or empty)
     int prec;
 };
 
@@ -80,6 +82,13 @@ struct WorkerJoin {
 bool hasGc(const TOp& t) { return t.h.gcMask || t.h.gcVal; }
 bool hasJc(const TOp& t) { return t.h.jcMask || t.h.jcVal; }
 
+// the number of terms of a PHASES op (a nibble per slot in rm1)
+int phaseCount(const TOp& t) {
+    int n = 0;
+    for (int sl = 0; sl <= fuse::RB; sl++) n += (int)((t.h.rm1 >> (4 * sl)) & 15u);
+    return n;
+}
+
 // the shape of one op: what its generated code writes in as constants
 std::string opShape(const TOp& t) {
     char buf[192];
@@ -94,6 +103,21 @@ std::string opShape(const TOp& t) {
         case fuse::F_BLOCK:
             snprintf(buf, sizeof buf, "B%d,%d,%d,%d,%d;", w[0], w[1], w[2], w[3], w[4]);
             break;
+        case fuse::F_PHASES: {
+            // every term's slot and condition masks are written in; only
+            // the factors are read from the program
+            // (and the op's own non-local condition, empty as planned)
+            snprintf(buf, sizeof buf, "P%llx,%llx,%d|", (unsigned long long)t.h.gcMask,
+                     (unsigned long long)t.h.gcVal, hasJc(t) ? 1 : 0);
+            std::string k = buf;
+            const fuse::PhaseTerm* pt = reinterpret_cast<const fuse::PhaseTerm*>(t.b);
+            for (int i = 0; i < phaseCount(t); i++) {
+                snprintf(buf, sizeof buf, "%x,%x,%x,%llx,%llx,", pt[i].rm, pt[i].jm, pt[i].jv,
+                         (unsigned long long)pt[i].gm, (unsigned long long)pt[i].gv);
+                k += buf;
+            }
+            return k + ";";
+        }
         default:
             snprintf(buf, sizeof buf, "O%d,%d,%d,%u,%u,%u,%u,%u,%d,%d;", t.h.kind, t.h.rb1, t.h.rb2, t.h.pad, t.h.rm1,
                      t.h.rm2, t.h.rcMask, t.h.rcVal, hasGc(t) ? 1 : 0, hasJc(t) ? 1 : 0);
@@ -105,9 +129,9 @@ const char* typeName(int prec) { return prec == QA_FP64 ? "double" : "float"; }
 
 // ---- compiles -------------------------------------------------------------
 
-bool compileOne(const Job& job, hipModule_t& mod, hipFunction_t& fn, std::string& err) {
+bool compileSource(const std::string& src, hipModule_t& mod, hipFunction_t& fn, std::string& err) {
     hiprtcProgram p;
-    if (hiprtcCreateProgram(&p, job.src.c_str(), "qa_jit_pass.hip", qa_jit_num_headers, qa_jit_header_sources,
+    if (hiprtcCreateProgram(&p, src.c_str(), "qa_jit_pass.hip", qa_jit_num_headers, qa_jit_header_sources,
                             qa_jit_header_names) != HIPRTC_SUCCESS) {
         err = "hiprtcCreateProgram failed";
         return false;
@@ -155,6 +179,18 @@ bool compileOne(const Job& job, hipModule_t& mod, hipFunction_t& fn, std::string
         return false;
     }
     return true;
+}
+
+// A program too large for the registers of the prefetching kernel (it
+// would spill to scratch) is compiled again without the prefetch, which
+// frees the 64 registers of the next tile.
+bool compileOne(const Job& job, hipModule_t& mod, hipFunction_t& fn, std::string& err) {
+    if (compileSource(job.src, mod, fn, err)) return true;
+    if (job.srcLean.empty()) return false;
+    std::string err2;
+    if (compileSource(job.srcLean, mod, fn, err2)) return true;
+    err += "; without prefetch: " + err2;
+    return false;
 }
 
 void worker(int device) {
@@ -238,10 +274,16 @@ std::string generate(int prec, bool nt, bool pf, const TileArgs& a, const TOp* p
          "namespace qa {\nnamespace fuse {\n"
          "struct JitProgram {\n"
          "    template <typename T, int NT>\n"
-         "    __device__ __forceinline__ static void run(T* __restrict__ amps, const TileArgs& a,\n"
+         "    __device__ __forceinline__ static void run(T* __restrict__ amps, const TileArgs& args,\n"
          "                                               const TOp* __restrict__ ops, uint64_t base, ct<T>* r,\n"
          "                                               ct<T>* tile, const RegLayout& row, const RegLayout& first,\n"
          "                                               uint32_t tid) {\n"
+         // the program and the pass arguments addressed through a value that
+         // changes per tile (base >> 62 is 0 for every state): loop-invariant
+         // addresses would be hoisted out of the tile loop and, one per op,
+         // held in scalar registers for the whole loop
+         "        ops += (size_t)(base >> 62);\n"
+         "        const TileArgs& a = *(&args + (size_t)(base >> 62));\n"
          "        TileState st;\n"
          "        tileBegin<T, NT>(st, a, r, tile, row, first);\n";
     char buf[512];
@@ -274,6 +316,39 @@ std::string generate(int prec, bool nt, bool pf, const TileArgs& a, const TOp* p
                          i, w[0], w[1], w[2], w[3], w[4]);
                 s += buf;
                 break;
+            case fuse::F_PHASES: {
+                if (hasJc(t)) {
+                    // planned PHASES ops carry no op-level lane condition
+                    // (their terms hold every condition); keep the interpreter
+                    s.clear();
+                    return s;
+                }
+                // each term as straight-line code with its condition masks
+                // written in (the interpreter's applyPhases, unrolled)
+                const fuse::PhaseTerm* pt = reinterpret_cast<const fuse::PhaseTerm*>(t.b);
+                snprintf(buf, sizeof buf,
+                         "        { const PhaseTerm* pt = reinterpret_cast<const PhaseTerm*>((ops + %zu)->b);\n"
+                         "          const uint32_t jb = st.cur.jb;\n"
+                         "          if ((base & 0x%llxull) == 0x%llxull) {\n"
+                         "          ct<T> acc[1 + RB];\n"
+                         "          for (int s_ = 0; s_ <= RB; s_++) acc[s_] = ct<T>{(T)1, (T)0};\n",
+                         i, (unsigned long long)t.h.gcMask, (unsigned long long)t.h.gcVal);
+                s += buf;
+                int k = 0;
+                for (int sl = 0; sl <= fuse::RB; sl++) {
+                    const int ns = (int)((t.h.rm1 >> (4 * sl)) & 15u);
+                    for (int e = 0; e < ns; e++, k++) {
+                        snprintf(buf, sizeof buf,
+                                 "          phaseTerm(acc[%d], (jb & 0x%xu) == 0x%xu && (base & 0x%llxull) == 0x%llxull,\n"
+                                 "                    pt[%d].fre, pt[%d].fim);\n",
+                                 sl, pt[k].jm, pt[k].jv, (unsigned long long)pt[k].gm, (unsigned long long)pt[k].gv,
+                                 k, k);
+                        s += buf;
+                    }
+                }
+                s += "          phasesFinish<T>(r, acc); } }\n";
+                break;
+            }
             default:
                 // the op's arm alone, its shape fields as constants
                 snprintf(buf, sizeof buf,
@@ -293,12 +368,11 @@ std::string generate(int prec, bool nt, bool pf, const TileArgs& a, const TOp* p
          "};\n"
          "}  // namespace fuse\n}  // namespace qa\n";
     snprintf(buf, sizeof buf,
-             "extern \"C\" __global__ void __launch_bounds__(512)%s\n"
+             "extern \"C\" __global__ void __launch_bounds__(512)\n"
              "qa_jit_pass(%s* __restrict__ amps, qa::fuse::TileArgs a, const qa::fuse::TOp* __restrict__ ops) {\n"
              "    qa::fuse::tilePassLoop<%s, %d, %d, qa::fuse::JitProgram>(amps, a, ops);\n"
              "}\n",
-             pf ? "" : " __attribute__((amdgpu_waves_per_eu(3)))", typeName(prec), typeName(prec), nt ? 1 : 0,
-             pf ? 1 : 0);
+             typeName(prec), typeName(prec), nt ? 1 : 0, pf ? 1 : 0);
     s += buf;
     return s;
 }
@@ -327,8 +401,13 @@ const void* kernelFor(int prec, bool nt, bool pf, const TileArgs& a, const TOp* 
         e.state = QUEUED;
         lk.unlock();
         std::string src = generate(prec, nt, pf, a, prog, num);
+        std::string lean = pf ? generate(prec, nt, false, a, prog, num) : std::string();
         lk.lock();
-        g_jobs.push_back(Job{key, std::move(src), prec});
+        if (src.empty()) {
+            e.state = FAILED;
+            return nullptr;
+        }
+        g_jobs.push_back(Job{key, std::move(src), std::move(lean), prec});
         ensureWorkers();
         g_work.notify_one();
     }

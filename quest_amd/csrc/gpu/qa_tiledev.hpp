@@ -613,24 +613,16 @@ __device__ __forceinline__ void applyOpC(ct<T>* r, const TOp* __restrict__ op, c
 // sorted by that slot, with the count of each slot in a nibble of rm1),
 // multiplied into one factor per register (subset products) before they
 // reach the amplitudes
-// (NIB >= 0: the slot counts as a constant, so the term loops unroll)
-template <typename T, int NIB = -1>
-__device__ __forceinline__ void applyPhases(ct<T>* r, const TOp* __restrict__ op, const TOpHead& h, uint32_t jb,
-                                            uint64_t base) {
-    const PhaseTerm* pt = reinterpret_cast<const PhaseTerm*>(op->b);
-    const uint32_t nib = NIB >= 0 ? (uint32_t)NIB : h.rm1;
-    ct<T> acc[1 + RB];
-    int k = 0;
-#pragma unroll
-    for (int sl = 0; sl <= RB; sl++) {
-        acc[sl] = ct<T>{(T)1, (T)0};
-        const int ns = (int)((nib >> (4 * sl)) & 15u);
-        for (int e = 0; e < ns; e++, k++) {
-            const PhaseTerm t = pt[k];
-            const bool on = (jb & t.jm) == t.jv && (base & t.gm) == t.gv;
-            acc[sl] = cmulT(ct<T>{on ? (T)t.fre : (T)1, on ? (T)t.fim : (T)0}, acc[sl]);
-        }
-    }
+
+// one term: acc <- (its condition holds ? its factor : 1) * acc
+template <typename T>
+__device__ __forceinline__ void phaseTerm(ct<T>& acc, bool on, double fre, double fim) {
+    acc = cmulT(ct<T>{on ? (T)fre : (T)1, on ? (T)fim : (T)0}, acc);
+}
+
+// acc[0] for every register, acc[1 + b] for registers with bit b set
+template <typename T>
+__device__ __forceinline__ void phasesFinish(ct<T>* r, const ct<T>* acc) {
     ct<T> fac[RA];
     fac[0] = acc[0];
 #pragma unroll
@@ -639,6 +631,24 @@ __device__ __forceinline__ void applyPhases(ct<T>* r, const TOp* __restrict__ op
         for (int v = 0; v < (1 << b); v++) fac[v | (1 << b)] = cmulT(acc[1 + b], fac[v]);
 #pragma unroll
     for (int v = 0; v < RA; v++) r[v] = cmulT(fac[v], r[v]);
+}
+
+template <typename T>
+__device__ __forceinline__ void applyPhases(ct<T>* r, const TOp* __restrict__ op, const TOpHead& h, uint32_t jb,
+                                            uint64_t base) {
+    const PhaseTerm* pt = reinterpret_cast<const PhaseTerm*>(op->b);
+    ct<T> acc[1 + RB];
+    int k = 0;
+#pragma unroll
+    for (int sl = 0; sl <= RB; sl++) {
+        acc[sl] = ct<T>{(T)1, (T)0};
+        const int ns = (int)((h.rm1 >> (4 * sl)) & 15u);
+        for (int e = 0; e < ns; e++, k++) {
+            const PhaseTerm t = pt[k];
+            phaseTerm(acc[sl], (jb & t.jm) == t.jv && (base & t.gm) == t.gv, t.fre, t.fim);
+        }
+    }
+    phasesFinish<T>(r, acc);
 }
 
 // ops that never carry controls (PHASES, DENSE1N, the channels) are applied
@@ -735,7 +745,7 @@ __device__ __forceinline__ void applyFixed(ct<T>* r, const TOp* __restrict__ op,
     } else if constexpr (KIND == F_PAULI) {
         regPauli<T, (int)RM1, C, J>(r, h, f, parity64((jb & o.jm) ^ (base & o.gm)), act);
     } else if constexpr (KIND == F_PHASES) {
-        applyPhases<T, (int)RM1>(r, op, h, jb, base);
+        applyPhases<T>(r, op, h, jb, base);
     } else if constexpr (KIND == F_DENSE1N) {
         const double* fx = op->fx;
         regDense1<T, 0, false>(r, h, m2At<T>(f));

@@ -46,3 +46,26 @@ def test_generated_pass_kernels_compile_and_cache_soundly(planner_check):
     out = subprocess.run([str(planner_check), "60", "5", "jit", "4"], capture_output=True, text=True, timeout=600)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "4 generated kernels compiled" in out.stdout
+
+
+@pytest.fixture(scope="module")
+def jit_bench_check(tmp_path_factory):
+    if not Path(HIPCC).exists():
+        pytest.skip("hipcc not available")
+    exe = tmp_path_factory.mktemp("jitbench") / "jit_bench_check"
+    cmd = [HIPCC, "-std=c++17", "-O2", f"-I{ROOT / 'quest_amd' / 'csrc' / 'gpu'}", f"-I{ROOT / 'include'}",
+           str(ROOT / "tests" / "jit_bench_check.cpp"), "-o", str(exe), f"-L{GPU_LIB.parent}", "-lquest_amd_gpu",
+           "-lhiprtc", f"-Wl,-rpath,{GPU_LIB.parent}"]
+    subprocess.run(cmd, check=True, capture_output=True, text=True, timeout=300)
+    return exe
+
+
+def test_benchmark_pass_kernels_compile_without_scratch(jit_bench_check):
+    """The 30-qubit benchmark family's pass programs (a few layers, the QFT
+    and some gadgets), planned as the default planner plans them: every
+    distinct shape's generated kernel compiles for gfx950 and uses no
+    scratch memory (with its prefetch, or without it where the registers
+    run out, as the library falls back)."""
+    out = subprocess.run([str(jit_bench_check), "4", "8", "1000"], capture_output=True, text=True, timeout=900)
+    assert out.returncode == 0, out.stdout[-3000:] + out.stderr[-2000:]
+    assert "0 with scratch or failed" in out.stdout

@@ -56,26 +56,58 @@ std::deque<Job> g_jobs;
 int g_inFlight = 0;
 std::vector<std::thread> g_workers;
 bool g_stop = false;
+bool g_exiting = false;  // set at exit: no compile threads start again
+int g_exitRegs = 0;      // exit handlers registered by the compile threads
 Settings g_settings;
 Stats g_stats;
 std::string g_lastError;
 
-// stops and joins the compile threads when the library is unloaded without
-// qa_finalize (a joinable std::thread may not be destroyed); declared after
-// the state it uses, so it is destroyed first
-struct WorkerJoin {
-    ~WorkerJoin() {
-        std::vector<std::thread> workers;
-        {
-            std::lock_guard<std::mutex> lk(g_mu);
-            g_stop = true;
-            g_jobs.clear();
-            workers.swap(g_workers);
-        }
-        g_work.notify_all();
-        for (auto& w : workers) w.join();
+// Stops and joins the compile threads.  At process exit the handlers
+// registered with atexit and the static destructors run in the reverse
+// order of their registration, and hiprtc loads the compiler library (with
+// its static state) only at its first compile, after this library's own
+// statics: a handler registered at load would run after the compiler's
+// state is gone, while a compile thread may still be inside it.  So
+// ensureWorkers() compiles one empty program on the calling thread before
+// it starts the threads and then registers this handler, and the compile
+// threads register it again after each of the first 16 compiles (for state
+// the compiler creates lazily).  It is idempotent; a compile in flight at
+// exit is waited for.
+void stopWorkers(bool exiting) {
+    std::vector<std::thread> workers;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_stop = true;
+        g_exiting = g_exiting || exiting;
+        g_jobs.clear();
+        workers.swap(g_workers);
     }
+    g_work.notify_all();
+    for (auto& w : workers)
+        if (w.get_id() != std::this_thread::get_id()) w.join();
+        else w.detach();
+}
+
+// the same for a library unloaded without qa_finalize and without exit
+// (a joinable std::thread may not be destroyed); declared after the state
+// it uses, so it is destroyed first
+struct WorkerJoin {
+    ~WorkerJoin() { stopWorkers(true); }
 } g_workerJoin;
+
+void stopAtExit() { stopWorkers(true); }
+void registerExitStop() { (void)std::atexit(stopAtExit); }
+
+// loads and initialises hiprtc's compiler on the calling thread
+void warmCompiler() {
+    hiprtcProgram p;
+    if (hiprtcCreateProgram(&p, "extern \"C\" __global__ void qa_jit_warm() {}", "qa_jit_warm.hip", 0, nullptr,
+                            nullptr) != HIPRTC_SUCCESS)
+        return;
+    const char* opts[] = {"--offload-arch=gfx950", "-O0"};
+    (void)hiprtcCompileProgram(p, 2, opts);
+    hiprtcDestroyProgram(&p);
+}
 
 // ---- shapes ---------------------------------------------------------------
 
@@ -210,6 +242,7 @@ void worker(int device) {
         hipFunction_t fn = nullptr;
         std::string err;
         const bool ok = compileOne(job, mod, fn, err);
+        bool again = false;
         const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         {
             std::lock_guard<std::mutex> lk(g_mu);
@@ -237,19 +270,24 @@ void worker(int device) {
             }
             g_stats.compileSeconds += dt;
             g_inFlight--;
+            again = g_exitRegs < 16;
+            g_exitRegs += again ? 1 : 0;
         }
+        if (again) registerExitStop();
         g_done.notify_all();
     }
 }
 
 // under g_mu
 void ensureWorkers() {
-    if (!g_workers.empty()) return;
+    if (!g_workers.empty() || g_exiting) return;
     int dev = 0;
     (void)hipGetDevice(&dev);
     unsigned hw = std::thread::hardware_concurrency();
     int n = (int)(hw > 2 ? hw - 1 : 1);
     if (n > 16) n = 16;
+    warmCompiler();
+    registerExitStop();
     g_stop = false;
     for (int i = 0; i < n; i++) g_workers.emplace_back(worker, dev);
 }
@@ -397,7 +435,7 @@ const void* kernelFor(int prec, bool nt, bool pf, const TileArgs& a, const TOp* 
     Entry& e = g_cache[key];  // element references survive rehashing
     e.seen++;
     if (e.state == READY) return (const void*)e.fn;
-    if (e.state == NONE && (s.mode == 2 || e.seen >= s.minSeen)) {
+    if (e.state == NONE && !g_exiting && (s.mode == 2 || e.seen >= s.minSeen)) {
         e.state = QUEUED;
         lk.unlock();
         std::string src = generate(prec, nt, pf, a, prog, num);
@@ -469,15 +507,7 @@ const char* lastError() {
 }
 
 void shutdown() {
-    std::vector<std::thread> workers;
-    {
-        std::lock_guard<std::mutex> lk(g_mu);
-        g_stop = true;
-        g_jobs.clear();
-        workers.swap(g_workers);
-    }
-    g_work.notify_all();
-    for (auto& w : workers) w.join();
+    stopWorkers(false);
     std::lock_guard<std::mutex> lk(g_mu);
     for (auto& kv : g_cache)
         if (kv.second.module) (void)hipModuleUnload(kv.second.module);

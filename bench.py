@@ -87,28 +87,50 @@ def build_step(Q, qc, circ, q):
     return calls
 
 
-def cpu_baseline(circ, n: int, layers: int = 3) -> dict | None:
-    """Bounded sample of the same workload on the host cores: the first
-    `layers` layers of the step (each n applyCompMatr1, n/2
-    applyTwoQubitPhaseShift and one 3-control applyMultiControlledCompMatr1)
-    at n qubits, through QuEST's own CPU build if present, else through the
-    oracle's C restatement."""
+def cpu_baseline(circ, n: int) -> dict | None:
+    """Bounded, stratified sample of the step on the host cores, through
+    QuEST's own CPU build if present, else through the oracle's C
+    restatement, each segment timed on its own: the first 2 layers (their
+    applyCompMatr1, applyTwoQubitPhaseShift and 3-control
+    applyMultiControlledCompMatr1 calls), the QFT's first 2 Hadamards with
+    their phase shifts in the order QuEST's applyQuantumFourierTransform
+    issues them, 4 of the step's Pauli gadgets, and 3 of its reductions (2
+    qubit probabilities, 1 Pauli expectation).  The step's time is
+    estimated from the segments' rates weighted by the step's own counts
+    (layer gates, QFT gates, gadgets, reductions); value = the step's gates
+    over that time, so it reflects the step's mix rather than one gate
+    kind."""
+    import math
     from quest_amd import circuits as qc
-    sample_ops, seen = [], 0
+    layer_ops, seen = [], 0
     for op in circ.ops:
         if op[0] not in ("U1", "PS2", "CU1"):
             break
-        sample_ops.append(op)
+        layer_ops.append(op)
         if op[0] == "CU1":
             seen += 1
-            if seen == layers:
+            if seen == 2:
                 break
+    qft_ops = []
+    for t in (n - 1, n - 2):
+        qft_ops.append(("H", t))
+        qft_ops += [("PS2", t, t - m - 1, math.pi / (1 << (m + 1))) for m in range(t)]
+    gadget_ops = [op for op in circ.ops if op[0] == "PG"][:4]
+    red_ops = [op for op in circ.ops if op[0] == "OUT_PROB"][:2] + [op for op in circ.ops if op[0] == "OUT_EXPEC"][:1]
+    segments = [("layers", layer_ops), ("qft", qft_ops), ("gadgets", gadget_ops), ("reductions", red_ops)]
+    step = {"layers": sum(1 for op in circ.ops if op[0] in ("U1", "PS2", "CU1")),
+            "qft": (n + n * (n - 1) // 2 + n // 2) if any(op[0] == "FQFT" for op in circ.ops) else 0,
+            "gadgets": sum(1 for op in circ.ops if op[0] == "PG"),
+            "reductions": sum(1 for op in circ.ops if op[0].startswith("OUT_"))}
+    step_gates = step["layers"] + step["qft"] + step["gadgets"]
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     driver = ROOT / "oracle" / "_ref" / "circuit_driver"
-    desc = f"first {layers} layers ({len(sample_ops)} gates) of the step on a {n}-qubit FP64 statevector"
+    secs = {}
+    kind = "port"
     if driver.exists():
-        lines = [f"QUREG {n} 0", "INITPLUS", "TIMER_START"] + [qc.op_to_line(op) for op in sample_ops]
-        lines.append("TIMER_STOP sample")
+        lines = [f"QUREG {n} 0", "INITPLUS"]
+        for name, ops in segments:
+            lines += ["TIMER_START"] + [qc.op_to_line(op) for op in ops] + [f"TIMER_STOP {name}"]
         with tempfile.NamedTemporaryFile("w", suffix=".txt", delete=False) as f:
             f.write("\n".join(lines) + "\n")
             path = f.name
@@ -119,20 +141,31 @@ def cpu_baseline(circ, n: int, layers: int = 3) -> dict | None:
         finally:
             os.unlink(path)
         if out.returncode == 0:
-            secs = [float(l.split()[2]) for l in out.stdout.splitlines() if l.startswith("T sample")]
-            if secs:
-                return {"value": len(sample_ops) / secs[0], "unit": "gates/s", "cores": threads,
-                        "kind": "reference", "sample": desc + f", QuEST v4.1.0 FP64 OpenMP build, {secs[0]:.2f} s"}
-        log(f"reference CPU build failed ({out.returncode}); falling back to the oracle port")
-    from oracle import qoracle as O
-    st = O.State(n)
-    st.a[:] = 1 / (2 ** (n / 2))
-    t0 = time.perf_counter()
-    for op in sample_ops:
-        O.apply_op(st, op, [])
-    secs = time.perf_counter() - t0
-    return {"value": len(sample_ops) / secs, "unit": "gates/s", "cores": threads, "kind": "port",
-            "sample": desc + f", C restatement (OpenMP), {secs:.2f} s"}
+            for l in out.stdout.splitlines():
+                if l.startswith("T "):
+                    secs[l.split()[1]] = float(l.split()[2])
+            kind = "reference"
+        else:
+            log(f"reference CPU build failed ({out.returncode}); falling back to the oracle port")
+    if len(secs) != len(segments):
+        from oracle import qoracle as O
+        st = O.State(n)
+        st.a[:] = 1 / (2 ** (n / 2))
+        secs, kind = {}, "port"
+        for name, ops in segments:
+            t0 = time.perf_counter()
+            for op in ops:
+                O.apply_op(st, op, [])
+            secs[name] = time.perf_counter() - t0
+    est = sum(step[name] * secs[name] / max(1, len(ops)) for name, ops in segments)
+    sample = (f"{n}-qubit FP64 statevector, segments timed apart: 2 layers ({len(layer_ops)} gates, "
+              f"{secs['layers']:.2f} s), QFT start ({len(qft_ops)} gates, {secs['qft']:.2f} s), "
+              f"{len(gadget_ops)} Pauli gadgets ({secs['gadgets']:.2f} s), {len(red_ops)} reductions "
+              f"({secs['reductions']:.2f} s); step time estimated from the segment rates at the step's counts "
+              f"({step['layers']} layer gates, {step['qft']} QFT gates, {step['gadgets']} gadgets, "
+              f"{step['reductions']} reductions) = {est:.1f} s; "
+              + ("QuEST v4.1.0 FP64 OpenMP build" if kind == "reference" else "C restatement (OpenMP)"))
+    return {"value": step_gates / est, "unit": "gates/s", "cores": threads, "kind": kind, "sample": sample}
 
 
 def kernel_sweep(Q, n: int, reps: int = 5) -> None:
@@ -231,6 +264,7 @@ def serve_bench(args) -> int:
     lib = S._lib()
     eng = S.Engine(args.arena_gib << 30, max_active=args.max_active, tile_qubits=args.tile, low_qubits=4,
                    precision=1)
+    eng.set_graphs(bool(args.graphs))
     # warm the engine's kernels on a few requests outside the measurement
     for c in reqs[:4]:
         eng.wait(eng.submit(c.num_qubits, c.ops, S.paulis_of(c)))
@@ -262,6 +296,7 @@ def serve_bench(args) -> int:
         lat.append(L.value)
     wall = time.perf_counter() - t0
     st = eng.stats()
+    st["graphs"] = eng.graph_stats()
     eng.close()
     lat = np.array(lat)
     value = len(reqs) / wall
@@ -304,7 +339,8 @@ def serve_bench(args) -> int:
         "config": {"workload": (f"{len(reqs)} variational requests, qubits U{{10..24}}, depth U{{4..32}}, "
                                 "RY+RZ on every qubit then a CZ ladder per layer, 8 Pauli expectations each, "
                                 f"Poisson arrivals at {args.rate}/s"),
-                   "arena_gib": args.arena_gib, "max_active": args.max_active, "tile_qubits": args.tile},
+                   "arena_gib": args.arena_gib, "max_active": args.max_active, "tile_qubits": args.tile,
+                   "graphs": bool(args.graphs)},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "max": float(lat.max())},
         "gates_per_s": total_gates / wall,
@@ -419,6 +455,7 @@ def apply_fusion_args(Q, args) -> bool:
         Q.set_fusion(False)
     Q.gpu().qa_setFusionPrefetch(int(args.prefetch))
     Q.set_fusion_exact(bool(args.exact))
+    Q.set_fusion_span(int(args.span))
     # pass kernels generated at run time: every program shape is compiled in
     # the background at first sight (warmup steps run in the interpreter
     # meanwhile; jit_ready() waits for the compiles before the timed steps)
@@ -448,6 +485,8 @@ def main() -> int:
     ap.add_argument("--max-active", type=int, default=256, help="serve mode: requests running at once")
     ap.add_argument("--tile", type=int, default=13, help="serve mode: tile qubits")
     ap.add_argument("--baseline-sample", type=int, default=24, help="serve mode: requests run sequentially")
+    ap.add_argument("--graphs", type=int, default=0,
+                    help="serve mode: launch each engine step as one captured HIP graph (1) or launch by launch (0)")
     ap.add_argument("--mode", choices=["circuit", "density", "serve", "precision", "sweep"], default="circuit",
                     help="circuit: the headline 30-qubit statevector workload; density: the 14-qubit noisy "
                          "density-matrix workload; sweep: per-target dense-kernel timing + copy calibration")
@@ -471,6 +510,12 @@ def main() -> int:
     ap.add_argument("--jit", type=int, default=1,
                     help="pass kernels generated at run time for each program shape (1) or every pass in the "
                          "interpreter kernel (0)")
+    ap.add_argument("--span", type=int, default=12,
+                    help="span passes for runs of Pauli/phase gadgets over cosets of 2^SPAN amplitudes "
+                         "(10..13), 0 = off")
+    ap.add_argument("--tune", type=int, default=0,
+                    help="run the fusion autotuner (qa_tuneFusion) on one step's gates before the warmup and use "
+                         "its choice of tile, low and span qubits")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibration-copy", action="store_true",
                     help="copy the state once (one device copy dispatch of known size) before the timed steps, "
@@ -535,6 +580,26 @@ def main() -> int:
         Q.destroyQureg(Q.createCloneQureg(q))
         Q.syncQuESTEnv()
 
+    tuner = None
+    if args.tune and fused:
+        # the fusion autotuner on one step's gates (queued whole, then timed
+        # under candidate shapes on a scratch copy of the state)
+        t = time.perf_counter()
+        Q.set_fusion(True, fusion_cfg[1], fusion_cfg[2], 4096)
+        gate_ops = [op for op in circ.ops if not op[0].startswith("OUT_")]
+        for f, a in build_step(Q, qc, qc.Circuit(circ.num_qubits, circ.dens, gate_ops, circ.init, circ.name), q):
+            f(*a)
+        tuner = Q.tune_fusion(q, retune=True)
+        cfg = Q.get_fusion()
+        Q.set_fusion(True, cfg[1], cfg[2], fusion_cfg[3])
+        fusion_cfg = Q.get_fusion()
+        args.span = tuner["best"]["span_qubits"]
+        tuner["seconds"] = time.perf_counter() - t
+        for tr in tuner["trials"]:
+            log(f"tuner: tile {tr['tile_qubits']} low {tr['low_qubits']} span {tr['span_qubits']} "
+                f"blocks {tr['block_min_gates']}: {tr['ms']:.1f} ms")
+        log(f"tuner: chose {tuner['best']} in {tuner['seconds']:.1f} s")
+
     for w in range(args.warmup):
         t = time.perf_counter()
         for f, a in calls:
@@ -554,6 +619,7 @@ def main() -> int:
     gpu.qa_getFusionStats(C.byref(passes), C.byref(fgates), C.byref(pass_ms), C.byref(pass_flops))
     prog_ops, layouts = C.c_int64(), C.c_int64()
     gpu.qa_getFusionProgramStats(C.byref(prog_ops), C.byref(layouts))
+    span = Q.fusion_span_stats()
     gpu.qa_setFusionTiming(0)
     jit = None
     if jit0 is not None:
@@ -567,14 +633,18 @@ def main() -> int:
     pass_bytes = 2 * state
     log(f"timed: {total_ms:.1f} ms (wall {wall * 1e3:.1f} ms) for {args.steps} steps = {value:.1f} gates/s")
 
+    # tile passes (the dominant kernel) and span passes (runs of gadgets) apart
+    tile_passes = passes.value - span["passes"]
+    tile_ms = pass_ms.value - span["pass_ms"]
     roofline = None
-    if passes.value:
-        avg_s = pass_ms.value / 1e3 / passes.value
+    if tile_passes:
+        avg_s = tile_ms / 1e3 / tile_passes
         achieved = pass_bytes / avg_s
-        log(f"tile passes: {passes.value} ({fgates.value} gates, {fgates.value / passes.value:.2f} per pass), "
-            f"avg {avg_s * 1e3:.3f} ms = {achieved / 1e9:.0f} GB/s; passes are "
-            f"{100 * pass_ms.value / total_ms:.1f}% of the timed region; per pass {prog_ops.value / passes.value:.1f} "
-            f"program ops, {layouts.value / passes.value:.2f} register-layout changes")
+        log(f"tile passes: {tile_passes} ({fgates.value - span['gates']} gates, "
+            f"{(fgates.value - span['gates']) / tile_passes:.2f} per pass), avg {avg_s * 1e3:.3f} ms = "
+            f"{achieved / 1e9:.0f} GB/s; passes are {100 * pass_ms.value / total_ms:.1f}% of the timed region; "
+            f"per pass {(prog_ops.value - span['gates']) / tile_passes:.1f} program ops, "
+            f"{layouts.value / tile_passes:.2f} register-layout changes")
         traffic = None
         tfile = ROOT / "profiles" / f"k_tilePass_{n}q_fp64_traffic.json"
         if tfile.exists():
@@ -586,18 +656,27 @@ def main() -> int:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
             "traffic": traffic,
-            "kernel": "k_tilePass<double> (fused gate pass)",
+            "kernel": ("tile pass (qa_jit_pass generated per program shape, else k_tilePass<double>)"
+                       if args.jit else "k_tilePass<double> (fused gate pass)"),
             "bytes_per_launch": pass_bytes,
             "avg_launch_ms": avg_s * 1e3,
-            "launches": passes.value,
-            "program_ops_per_launch": prog_ops.value / passes.value,
-            "layout_changes_per_launch": layouts.value / passes.value,
-            "share_of_step_time": pass_ms.value / total_ms,
+            "launches": tile_passes,
+            "program_ops_per_launch": (prog_ops.value - span["gates"]) / tile_passes,
+            "layout_changes_per_launch": layouts.value / tile_passes,
+            "share_of_step_time": tile_ms / total_ms,
             # the same launches against the FP64 vector peak: the gates a
             # pass applies are FP64 arithmetic on the registers
             "fp64_TFLOPs": pass_flops.value / (pass_ms.value / 1e3) / 1e12,
             "fp64_frac": pass_flops.value / (pass_ms.value / 1e3) / FP64_PEAK,
         }
+    span_out = None
+    if span["passes"]:
+        avg_s = span["pass_ms"] / 1e3 / span["passes"]
+        span_out = {"kernel": "k_spanPass<double>", "launches": span["passes"], "gates": span["gates"],
+                    "avg_launch_ms": avg_s * 1e3, "achieved_GBps": pass_bytes / avg_s / 1e9,
+                    "frac": pass_bytes / avg_s / HBM_PEAK, "share_of_step_time": span["pass_ms"] / total_ms}
+        log(f"span passes: {span['passes']} ({span['gates']} gates), avg {avg_s * 1e3:.3f} ms = "
+            f"{pass_bytes / avg_s / 1e9:.0f} GB/s")
 
     unfused = None
     if fused and not args.no_unfused and not dens:
@@ -643,13 +722,16 @@ def main() -> int:
             "state_bytes": state,
             "parallelism": "single GPU",
             "fusion": ({"tile_qubits": fusion_cfg[1], "low_qubits": fusion_cfg[2], "max_queue": fusion_cfg[3],
-                        "exact": bool(args.exact), "passes_per_step": passes.value / args.steps,
+                        "exact": bool(args.exact), "span_qubits": args.span,
+                        "passes_per_step": passes.value / args.steps,
                         "gates_per_pass": (fgates.value / passes.value) if passes.value else None}
                        if fused else "off"),
         },
         "jit": jit,
+        "tuner": tuner,
         "hbm_fraction_unfused_bytes": (args.steps * alg_bytes / (total_ms / 1e3)) / HBM_PEAK,
         "roofline": roofline,
+        "span_passes": span_out,
         "unfused": unfused,
         "cpu_baseline": cpu,
     }

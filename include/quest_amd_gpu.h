@@ -178,6 +178,45 @@ int qa_getFusionStats(int64_t* passes, int64_t* gates, double* passMs, double* f
    ops the passes executed (gates after merging, plus layout changes) and,
    among them, register-layout changes through LDS */
 int qa_getFusionProgramStats(int64_t* ops, int64_t* layouts);
+/* span passes over cosets of 2^spanQubits amplitudes (10..13; 12, the
+   default: two workgroups per CU overlap one's LDS work with the other's
+   HBM traffic) or none (0).  A run of uncontrolled Pauli gadgets, Pauli
+   tensors and phase gadgets at the front of the queue whose X/Y masks, with
+   the always-local low qubits, span at most spanQubits dimensions over
+   GF(2) may be applied as one span pass: each workgroup loads one coset of
+   that subspace into LDS and applies the run in queue order with the
+   single-gate arithmetic, so the result is bit-identical to the gates one
+   by one.  The flush takes it instead of the next tile pass when it holds
+   more gates.  (No counterpart in the reference, which applies each gadget
+   as its own kernel: quest/src/gpu/gpu_subroutines.cpp:780 and :865.) */
+int qa_setFusionSpan(int spanQubits);
+/* Fusion autotuner (no counterpart in the reference, which has no fusion).
+   Queue a representative batch of gates for the state `amps` (one
+   iteration of the circuit, with maxQueue large enough to hold it), then
+   call: the batch is applied to a scratch copy of the state under
+   candidate fusion shapes -- by coordinate descent from the current
+   configuration over tileQubits (13, 12, 11), lowQubits (2, 3, 4),
+   spanQubits (13, 12, 0) and, for complex64 states, the MFMA block
+   threshold (0, 4) -- once to warm (generated kernels are compiled and
+   waited for) and once timed with HIP events; the fastest becomes the
+   fusion configuration (maxQueue is kept) and the queued batch is then
+   applied to `amps` with it.  Choices are cached per device, precision,
+   qubit count and gate mix (tenths of the batch per gate kind); a cached
+   choice is applied without timing unless flags & 1.  trials[0..maxTrials)
+   receives the shapes tried, in order, with the batch's milliseconds (-1
+   where a queued gate does not fit the shape), *numTrials their count and
+   *best the index of the one chosen.  Needs device memory for one more
+   copy of the state; holds the fusion lock (other threads' gates wait)
+   while it runs. */
+typedef struct {
+    int tileQubits, lowQubits, spanQubits, blockMinGates;
+    double ms;
+} qa_fusionTrial;
+int qa_tuneFusion(void* amps, int flags, qa_fusionTrial* trials, int maxTrials, int* numTrials, int* best);
+/* span passes among the passes, the gates they applied and (with timing
+   on) their summed elapsed milliseconds, since the last call (which resets
+   them; qa_getFusionStats counts them too) */
+int qa_getFusionSpanStats(int64_t* passes, int64_t* gates, double* passMs);
 
 /* ------------------------------------------------------------------ */
 /* statevector gates             (gpu_subroutines.hpp:33-70)          */

@@ -89,6 +89,18 @@ int qa_serveStats(void* engine, int64_t* steps, int64_t* passes, int64_t* admitt
 /* Milliseconds the engine thread has spent planning and launching steps,
    and waiting for steps to finish on the GPU. */
 int qa_serveTimes(void* engine, double* planMs, double* waitMs);
+/* enabled = 1: each step's copies and launches (program upload, one batched
+   tile pass per tile size, the expectation reduction and its merge, the
+   result copy) are captured into a HIP graph and launched as one graph;
+   each of the engine's step slots keeps one executable graph per launch
+   signature (upload or not, number of pass launches, reduction or not) and
+   updates it in place (hipGraphExecUpdate) with the new step's sizes and
+   arguments, instantiating anew when the update is refused.  0 (the
+   default): launched one by one.
+   Takes effect from the next step; results are the same either way. */
+int qa_serveSetGraphs(void* engine, int enabled);
+/* Graph launches, instantiations and in-place updates since creation. */
+int qa_serveGraphStats(void* engine, int64_t* launches, int64_t* instantiations, int64_t* updates);
 
 #ifdef __cplusplus
 }

@@ -481,6 +481,45 @@ def set_fusion(enabled: bool, tile_qubits: int = 12, low_qubits: int = 4, max_qu
     setQuESTFusion(int(bool(enabled)), int(tile_qubits), int(low_qubits), int(max_queue))
 
 
+def set_fusion_span(span_qubits=12) -> None:
+    """Span passes for runs of Pauli and phase gadgets over cosets of
+    2^span_qubits amplitudes (10..13; True = 12, the default), or none
+    (False / 0)."""
+    q = 12 if span_qubits is True else int(span_qubits)
+    rc = gpu().qa_setFusionSpan(q)
+    if rc != 0:
+        raise QuESTError(f"qa_setFusionSpan failed ({rc}): {_gpu_last_error()}")
+
+
+class FusionTrial(C.Structure):
+    _fields_ = [("tileQubits", C.c_int), ("lowQubits", C.c_int), ("spanQubits", C.c_int),
+                ("blockMinGates", C.c_int), ("ms", C.c_double)]
+
+
+def tune_fusion(qureg, retune: bool = False) -> dict:
+    """Runs the fusion autotuner on the gates queued for `qureg` (see
+    qa_tuneFusion) and returns {"trials": [...], "best": {...}}."""
+    trials = (FusionTrial * 32)()
+    num, best = C.c_int(), C.c_int()
+    rc = gpu().qa_tuneFusion(C.c_void_p(qureg.gpuAmps), int(bool(retune)), trials, 32, C.byref(num),
+                             C.byref(best))
+    if rc != 0:
+        raise QuESTError(f"qa_tuneFusion failed ({rc}): {_gpu_last_error()}")
+    out = [{"tile_qubits": t.tileQubits, "low_qubits": t.lowQubits, "span_qubits": t.spanQubits,
+            "block_min_gates": t.blockMinGates, "ms": t.ms} for t in trials[:min(num.value, 32)]]
+    return {"trials": out, "best": out[best.value] if 0 <= best.value < len(out) else None}
+
+
+def fusion_span_stats() -> dict:
+    """Span passes launched, the gates they applied and (with pass timing
+    on) their milliseconds since the last call (which resets them)."""
+    passes, gates, ms = C.c_int64(), C.c_int64(), C.c_double()
+    rc = gpu().qa_getFusionSpanStats(C.byref(passes), C.byref(gates), C.byref(ms))
+    if rc != 0:
+        raise QuESTError(f"qa_getFusionSpanStats failed ({rc}): {_gpu_last_error()}")
+    return {"passes": passes.value, "gates": gates.value, "pass_ms": ms.value}
+
+
 def fusion_stats() -> dict:
     """Tile passes launched, the gates they applied, their milliseconds (with
     pass timing on), the flops and program ops they executed and their

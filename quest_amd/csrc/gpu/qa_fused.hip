@@ -67,11 +67,14 @@ Config g_cfg;  // disabled by default
 // around each tile-pass launch (for measuring the pass kernel alone)
 struct Stats {
     int64_t passes = 0, gates = 0;
+    int64_t spanPasses = 0, spanGates = 0;  // among them, span passes and their gates
     int64_t progOps = 0, layouts = 0;  // program entries of the passes, and their GROUP ops
     double flops = 0;  // algorithmic real floating-point operations of the passes
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
+    std::vector<char> poolSpan;  // the pass of pool[i] was a span pass
     size_t used = 0;
+    double passMs = 0, spanMs = 0;  // elapsed times collected from the pool
 };
 Stats g_stats;
 
@@ -135,6 +138,7 @@ bool ldsTileAtZero(const void* fn) {
 }
 
 std::vector<TOp> g_prog;  // program of the pass being launched (under g_mu)
+std::vector<SpanOp> g_spanProg;  // ops of the span pass being planned (under g_mu)
 
 // packed MFMA block matrices: staged per pass, uploaded through a ring like
 // the op ring (a wrap waits for the stream)
@@ -918,6 +922,101 @@ void setLdsOffsets(TOp* prog, size_t num, TileArgs& a, int prec) {
     offsets(cur, a.lastOff);
 }
 
+bool spanEligible(const GOp& g) { return (g.kind == F_PAULI || g.kind == F_PARITY) && g.cMask == 0; }
+
+size_t spanRun(const GOp* ops, size_t num, int n, int k, int low, SpanArgs& a, std::vector<SpanOp>& prog) {
+    prog.clear();
+    if (k < 10 || k > 13 || n <= k || low < 0 || low > k - RB) return 0;
+    // basis vectors in reduced echelon form: vec[j] has bit piv[j], which no
+    // other basis vector has
+    uint64_t vec[MAXQ];
+    int piv[MAXQ];
+    int dim = 0;
+    for (int q = 0; q < low; q++) {
+        vec[dim] = 1ull << q;
+        piv[dim++] = q;
+    }
+    auto add = [&](uint64_t m) {  // m reduced (no pivot bits), nonzero
+        const int p = 63 - __builtin_clzll(m);
+        for (int j = 0; j < dim; j++)
+            if ((vec[j] >> p) & 1ull) vec[j] ^= m;
+        vec[dim] = m;
+        piv[dim++] = p;
+    };
+    size_t run = 0;
+    for (; run < num && run < (size_t)MAX_SPAN_OPS; run++) {
+        const GOp& g = ops[run];
+        if (!spanEligible(g)) break;
+        if (g.kind != F_PAULI) continue;
+        uint64_t m = g.m1;
+        for (int j = 0; j < dim; j++)
+            if ((m >> piv[j]) & 1ull) m ^= vec[j];
+        if (!m) continue;
+        if (dim == k) break;
+        add(m);
+    }
+    if (run == 0) return 0;
+    // the rest of the k dimensions: unit vectors of the lowest free qubits
+    uint64_t pivMask = 0;
+    for (int j = 0; j < dim; j++) pivMask |= 1ull << piv[j];
+    for (int q = 0; dim < k && q < n; q++)
+        if (!((pivMask >> q) & 1ull)) {
+            add(1ull << q);
+            pivMask |= 1ull << q;
+        }
+    // coordinates in pivot order: the lanes of a wave hold the lowest
+    // coordinates, so the unit vectors of the low qubits come first
+    for (int i = 1; i < dim; i++)
+        for (int j = i; j > 0 && piv[j - 1] > piv[j]; j--) {
+            std::swap(piv[j - 1], piv[j]);
+            std::swap(vec[j - 1], vec[j]);
+        }
+    std::memset(&a, 0, sizeof(a));
+    a.k = k;
+    for (int j = 0; j < k; j++) a.basis[j] = vec[j];
+    for (int i = 0; i < RA; i++) {
+        uint64_t t = 0;
+        for (int b = 0; b < RB; b++)
+            if ((i >> b) & 1) t ^= vec[k - RB + b];
+        a.top[i] = t;
+    }
+    for (int q = 0; q < n; q++)
+        if (!((pivMask >> q) & 1ull)) a.repPos[a.numRep++] = q;
+    a.repMask = ((n == 64) ? ~0ull : ((1ull << n) - 1ull)) & ~pivMask;
+    a.numCosets = 1ull << (n - k);
+    a.numOps = (int)run;
+    // a mask in V has coordinate j equal to its bit piv[j]; a sign mask's
+    // coordinate j is its parity over b_j
+    auto coords = [&](uint64_t m) {
+        uint32_t c = 0;
+        for (int j = 0; j < k; j++) c |= (uint32_t)((m >> piv[j]) & 1ull) << j;
+        return c;
+    };
+    auto parities = [&](uint64_t z) {
+        uint32_t d = 0;
+        for (int j = 0; j < k; j++) d |= (uint32_t)(__builtin_popcountll(vec[j] & z) & 1) << j;
+        return d;
+    };
+    for (size_t i = 0; i < run; i++) {
+        const GOp& g = ops[i];
+        SpanOp o;
+        std::memset(&o, 0, sizeof(o));
+        o.kind = g.kind;
+        if (g.kind == F_PAULI) {
+            o.c = coords(g.m1);
+            o.top = 31u - (uint32_t)__builtin_clz(o.c);
+            o.d = parities(g.m2);
+            o.z = g.m2;
+        } else {
+            o.d = parities(g.m1);
+            o.z = g.m1;
+        }
+        std::memcpy(o.f, g.f, 4 * sizeof(double));
+        prog.push_back(o);
+    }
+    return run;
+}
+
 // launches one pass applying ops [begin, end) of `ops`, which stand for
 // numGates queued gates
 int launchPass(void* amps, const Queue& qu, const std::vector<GOp>& ops, size_t begin, size_t end, uint64_t localMask,
@@ -993,6 +1092,8 @@ int launchPass(void* amps, const Queue& qu, const std::vector<GOp>& ops, size_t 
             QA_HIP(hipEventCreate(&e.second));
             g_stats.pool.push_back(e);
         }
+        g_stats.poolSpan.resize(g_stats.pool.size());
+        g_stats.poolSpan[g_stats.used] = 0;
         ev = &g_stats.pool[g_stats.used++];
         QA_HIP(hipEventRecord(ev->first, stream()));
     }
@@ -1016,6 +1117,72 @@ int launchPass(void* amps, const Queue& qu, const std::vector<GOp>& ops, size_t 
     return QA_OK;
 }
 
+// adds the elapsed times of the timed passes so far to passMs (and, for
+// span passes, spanMs) and frees their events for reuse; under g_mu
+int collectTimes() {
+    Stats& st = g_stats;
+    for (size_t i = 0; i < st.used; i++) {
+        float f = 0;
+        QA_HIP(hipEventSynchronize(st.pool[i].second));
+        QA_HIP(hipEventElapsedTime(&f, st.pool[i].first, st.pool[i].second));
+        st.passMs += f;
+        if (i < st.poolSpan.size() && st.poolSpan[i]) st.spanMs += f;
+    }
+    st.used = 0;
+    return QA_OK;
+}
+
+// launches one span pass applying ops [begin, begin + prog.size()) of `ops`
+// as planned by spanRun
+int launchSpan(void* amps, const Queue& qu, const std::vector<GOp>& ops, size_t begin, const SpanArgs& plan,
+               const std::vector<SpanOp>& prog) {
+    SpanArgs a = plan;
+    const size_t bytes = prog.size() * sizeof(SpanOp);
+    const size_t slots = (bytes + sizeof(TOp) - 1) / sizeof(TOp);
+    if (g_ringPos + slots > RING_OPS) {
+        QA_HIP(hipStreamSynchronize(stream()));
+        g_ringPos = 0;
+    }
+    std::memcpy(g_ringHost + g_ringPos, prog.data(), bytes);
+    QA_HIP(hipMemcpyAsync(g_ringDev + g_ringPos, g_ringHost + g_ringPos, bytes, hipMemcpyHostToDevice, stream()));
+    const SpanOp* d = reinterpret_cast<const SpanOp*>(g_ringDev + g_ringPos);
+    g_ringPos += slots;
+    const unsigned block = 1u << (a.k - RB);
+    const size_t lds = ldsBytes(qu.prec, a.k);
+    const bool nt = std::min(g_cfg.lowQubits, a.k) >= STREAM_LOW_BIT;
+    const void* fn = spanKernel(qu.prec, nt);
+    if (!ldsTileAtZero(fn)) {
+        setError("fusion: the span-pass kernel has static LDS");
+        return QA_ERR_HIP;
+    }
+    const unsigned grid = (unsigned)std::min<uint64_t>(a.numCosets, residentBlocks(fn, block, lds));
+    a.repStep = 0;
+    for (int i = 0; i < a.numRep && i < 32; i++) a.repStep |= (uint64_t)((grid >> i) & 1u) << a.repPos[i];
+    std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
+    if (g_stats.timing) {
+        if (g_stats.used == g_stats.pool.size()) {
+            std::pair<hipEvent_t, hipEvent_t> e;
+            QA_HIP(hipEventCreate(&e.first));
+            QA_HIP(hipEventCreate(&e.second));
+            g_stats.pool.push_back(e);
+        }
+        g_stats.poolSpan.resize(g_stats.pool.size());
+        g_stats.poolSpan[g_stats.used] = 1;
+        ev = &g_stats.pool[g_stats.used++];
+        QA_HIP(hipEventRecord(ev->first, stream()));
+    }
+    launchSpanAny(qu.prec, nt, grid, block, lds, amps, a, d);
+    QA_LAUNCH_CHECK();
+    if (ev) QA_HIP(hipEventRecord(ev->second, stream()));
+    g_stats.passes++;
+    g_stats.gates += (int64_t)prog.size();
+    g_stats.spanPasses++;
+    g_stats.spanGates += (int64_t)prog.size();
+    g_stats.progOps += (int64_t)prog.size();
+    for (size_t i = begin; i < begin + prog.size(); i++) g_stats.flops += opFlops(ops[i], qu.numQubits);
+    return QA_OK;
+}
+
 // greedy planner: in queue order, extend the local set while the next gate's
 // dense targets still fit the tile, then launch the run as one pass; unless
 // exact, gates that do not fit are passed over, later gates that commute
@@ -1032,6 +1199,15 @@ int flushQueue(void* amps, Queue& qu) {
     const int low = std::min(g_cfg.lowQubits, kMax);
     std::vector<GOp>& ops = qu.ops;
     int rc = QA_OK;
+    SpanArgs span;
+    std::vector<SpanOp>& spanProg = g_spanProg;
+    // a span pass instead of the next tile pass when it takes more gates
+    auto spanFirst = [&](size_t from, size_t tileGates) {
+        const int ks = std::min(g_cfg.spanQubits, n - 1);
+        if (ks < 10) return (size_t)0;
+        const size_t r = spanRun(ops.data() + from, ops.size() - from, n, ks, std::min(low, ks - RB), span, spanProg);
+        return (r >= 2 && r > tileGates) ? r : (size_t)0;
+    };
     if (g_cfg.exact) {
         size_t i = 0;
         while (i < ops.size() && rc == QA_OK) {
@@ -1041,6 +1217,11 @@ int flushQueue(void* amps, Queue& qu) {
                 setError("fusion: a gate needs more local qubits than the tile holds");
                 rc = QA_ERR_ARG;
                 break;
+            }
+            if (const size_t r = spanFirst(i, run)) {
+                rc = launchSpan(amps, qu, ops, i, span, spanProg);
+                i += r;
+                continue;
             }
             rc = launchPass(amps, qu, ops, i, i + run, L, run);
             i += run;
@@ -1058,6 +1239,11 @@ int flushQueue(void* amps, Queue& qu) {
             setError("fusion: a gate needs more local qubits than the tile holds");
             rc = QA_ERR_ARG;
             break;
+        }
+        if (const size_t r = spanFirst(0, picked.size())) {
+            rc = launchSpan(amps, qu, ops, 0, span, spanProg);
+            ops.erase(ops.begin(), ops.begin() + (ptrdiff_t)r);
+            continue;
         }
         pass.clear();
         rest.clear();
@@ -1172,6 +1358,216 @@ void shutdown() {
     g_blockPos = 0;
 }
 
+// lets the tile-pass and span-pass kernels use the LDS of a 2^tileQubits
+// tile (complex128 tiles are the larger ones)
+int allowTileLds(int tileQubits) {
+    int dev = 0, maxLds = 0;
+    QA_HIP(hipGetDevice(&dev));
+    QA_HIP(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    const size_t need = ldsBytes(QA_FP64, tileQubits);
+    QA_ARG_CHECK(need <= (size_t)maxLds, "fusion: a %d-qubit tile needs %zu bytes of LDS, the device allows %d",
+                 tileQubits, need, maxLds);
+    if (need > 65536) {
+        for (int v = 0; v < 8; v++) {
+            const void* fn = tileKernel((v & 4) ? QA_FP64 : QA_FP32, v & 2, v & 1);
+            QA_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+        }
+        for (int v = 0; v < 4; v++)
+            QA_HIP(hipFuncSetAttribute(spanKernel((v & 2) ? QA_FP64 : QA_FP32, v & 1),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+    }
+    return QA_OK;
+}
+
+// ---- autotuner ----------------------------------------------------------------
+
+struct TuneChoice {
+    int tile, low, span, blocks;
+    double ms;
+};
+std::map<std::string, TuneChoice> g_tuned;  // (device, precision, qubits, gate mix) -> choice
+
+// the gate mix of a batch: tenths of its gates per kind group
+std::string tuneKey(int dev, int prec, int n, const std::vector<GOp>& ops) {
+    int cnt[6] = {0, 0, 0, 0, 0, 0};
+    for (const GOp& g : ops) {
+        const int k = g.kind;
+        const int grp = k == F_DENSE1 || k == F_DENSE1V ? 0
+                        : k == F_DENSE2                  ? 1
+                        : k == F_DIAG1                   ? 2
+                        : k == F_SWAP                    ? 3
+                        : k == F_PAULI || k == F_PARITY  ? 4
+                                                         : 5;
+        cnt[grp]++;
+    }
+    char buf[160];
+    const double tot = ops.empty() ? 1.0 : (double)ops.size();
+    snprintf(buf, sizeof buf, "dev%d p%d n%d mix%d.%d.%d.%d.%d.%d", dev, prec, n, (int)std::lround(10 * cnt[0] / tot),
+             (int)std::lround(10 * cnt[1] / tot), (int)std::lround(10 * cnt[2] / tot),
+             (int)std::lround(10 * cnt[3] / tot), (int)std::lround(10 * cnt[4] / tot),
+             (int)std::lround(10 * cnt[5] / tot));
+    return buf;
+}
+
+// whether every gate of the batch is taken by a queue of this shape (the
+// queue runs a gate whose dense part does not fit a tile unfused)
+bool batchFits(const std::vector<GOp>& ops, int n, int tile, int low) {
+    const int kMax = std::min(n, tile);
+    const uint64_t lowMask = (1ull << std::min(low, kMax)) - 1ull;
+    for (const GOp& g : ops) {
+        const uint64_t d = denseMask(g);
+        if (__builtin_popcountll(d | lowMask) > kMax || (__builtin_popcountll(d) > RB && g.kind != F_PAULI))
+            return false;
+    }
+    return true;
+}
+
+}  // namespace fuse
+}  // namespace qa
+
+extern "C" int qa_tuneFusion(void* amps, int flags, qa_fusionTrial* trials, int maxTrials, int* numTrials,
+                             int* best) {
+    using namespace qa;
+    using namespace qa::fuse;
+    QA_REQUIRE_INIT();
+    QA_ARG_CHECK(amps && numTrials && best && (trials || maxTrials == 0) && maxTrials >= 0,
+                 "qa_tuneFusion: null argument");
+    *numTrials = 0;
+    *best = -1;
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_queues.find(amps);
+    QA_ARG_CHECK(it != g_queues.end() && !it->second.ops.empty(),
+                 "qa_tuneFusion: no gates are queued for this state (queue a representative batch first)");
+    Queue& qu = it->second;
+    const int n = qu.numQubits, prec = qu.prec;
+    const std::vector<GOp> batch = qu.ops;
+    int dev = 0;
+    QA_HIP(hipGetDevice(&dev));
+    const std::string key = tuneKey(dev, prec, n, batch);
+    const Config start = g_cfg;
+    auto report = [&](const TuneChoice& c) {
+        if (*numTrials < maxTrials) {
+            qa_fusionTrial& t = trials[*numTrials];
+            t.tileQubits = c.tile;
+            t.lowQubits = c.low;
+            t.spanQubits = c.span;
+            t.blockMinGates = c.blocks;
+            t.ms = c.ms;
+        }
+        return (*numTrials)++;
+    };
+    auto apply = [&](const TuneChoice& c) {
+        g_cfg.tileQubits = c.tile;
+        g_cfg.lowQubits = c.low;
+        g_cfg.spanQubits = c.span;
+        g_cfg.blockMinGates = c.blocks;
+    };
+    auto cached = g_tuned.find(key);
+    if (!(flags & 1) && cached != g_tuned.end()) {
+        int rc = allowTileLds(cached->second.tile);
+        if (rc != QA_OK) return rc;
+        apply(cached->second);
+        *best = report(cached->second);
+        return flushQueue(amps, qu);
+    }
+    // a scratch copy of the state: every candidate applies the batch to it
+    const size_t bytes = (prec == QA_FP64 ? 16 : 8) << n;
+    void* scratch = nullptr;
+    if (hipMalloc(&scratch, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        setError("qa_tuneFusion: no room for a %zu-byte scratch copy of the state", bytes);
+        return QA_ERR_OOM;
+    }
+    const jit::Settings js = jit::settings();
+    if (js.mode == 1) jit::setSettings(jit::Settings{1, 1});
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = QA_OK;
+    std::map<std::tuple<int, int, int, int>, double> seen;
+    // the batch's milliseconds under one shape: once to warm (the generated
+    // kernels of its passes are compiled meanwhile and waited for), once timed
+    auto time = [&](TuneChoice c) -> double {
+        const auto k = std::make_tuple(c.tile, c.low, c.span, c.blocks);
+        auto f = seen.find(k);
+        if (f != seen.end()) return f->second;
+        double ms = -1;
+        if (batchFits(batch, n, c.tile, c.low) && allowTileLds(c.tile) == QA_OK) {
+            apply(c);
+            for (int run = 0; run < 2 && rc == QA_OK; run++) {
+                Queue q;
+                q.prec = prec;
+                q.numQubits = n;
+                q.ops = batch;
+                if (run == 1) rc = hipEventRecord(e0, stream()) == hipSuccess ? QA_OK : QA_ERR_HIP;
+                if (rc == QA_OK) rc = flushQueue(scratch, q);
+                if (run == 0) {
+                    jit::waitIdle();
+                    continue;
+                }
+                if (rc == QA_OK) rc = hipEventRecord(e1, stream()) == hipSuccess ? QA_OK : QA_ERR_HIP;
+                float f32 = 0;
+                if (rc == QA_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&f32, e0, e1) != hipSuccess))
+                    rc = QA_ERR_HIP;
+                ms = f32;
+            }
+        }
+        c.ms = ms;
+        report(c);
+        seen[k] = ms;
+        return ms;
+    };
+    TuneChoice cur{start.tileQubits, start.lowQubits, start.spanQubits, start.blockMinGates, -1};
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+        hipMemcpyAsync(scratch, amps, bytes, hipMemcpyDeviceToDevice, stream()) != hipSuccess)
+        rc = QA_ERR_HIP;
+    double bestMs = rc == QA_OK ? time(cur) : -1;
+    // coordinate descent: tile size, then the always-local low qubits, then
+    // the span-pass coset size, then (complex64) the MFMA block threshold
+    const std::vector<std::vector<int>> values = {{13, 12, 11}, {2, 3, 4}, {13, 12, 0}, {0, 4}};
+    for (int dim = 0; dim < 4 && rc == QA_OK; dim++) {
+        if (dim == 3 && prec != QA_FP32) break;
+        for (int v : values[dim]) {
+            TuneChoice c = cur;
+            (dim == 0 ? c.tile : dim == 1 ? c.low : dim == 2 ? c.span : c.blocks) = v;
+            if (c.low > c.tile - 2 || c.tile > n) continue;
+            const double ms = time(c);
+            if (rc != QA_OK) break;
+            if (ms >= 0 && (bestMs < 0 || ms < bestMs)) {
+                bestMs = ms;
+                cur = c;
+            }
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(stream());
+    (void)hipFree(scratch);
+    jit::setSettings(js);
+    if (rc == QA_OK && bestMs < 0) {
+        setError("qa_tuneFusion: no candidate shape takes every queued gate");
+        rc = QA_ERR_ARG;
+    }
+    if (rc != QA_OK) {
+        g_cfg = start;
+        (void)allowTileLds(start.tileQubits);
+        return rc;
+    }
+    cur.ms = bestMs;
+    apply(cur);
+    rc = allowTileLds(cur.tile);
+    if (rc != QA_OK) return rc;
+    g_tuned[key] = cur;
+    for (int i = 0; i < std::min(*numTrials, maxTrials); i++) {
+        const qa_fusionTrial& t = trials[i];
+        if (t.tileQubits == cur.tile && t.lowQubits == cur.low && t.spanQubits == cur.span &&
+            t.blockMinGates == cur.blocks)
+            *best = i;
+    }
+    return flushQueue(amps, qu);
+}
+
+namespace qa {
+namespace fuse {
+
 const Config& config() { return g_cfg; }
 
 }  // namespace fuse
@@ -1186,21 +1582,8 @@ int qa_setFusion(int enabled, int tileQubits, int lowQubits, int maxQueue) {
     QA_ARG_CHECK(tileQubits >= qa::fuse::RB && tileQubits <= 13, "qa_setFusion: tileQubits must be in [4, 13]");
     QA_ARG_CHECK(lowQubits >= 0 && lowQubits <= tileQubits - 2, "qa_setFusion: lowQubits must be in [0, tileQubits-2]");
     QA_ARG_CHECK(maxQueue >= 1 && maxQueue <= 4096, "qa_setFusion: maxQueue must be in [1, 4096]");
-    {
-        int dev = 0, maxLds = 0;
-        QA_HIP(hipGetDevice(&dev));
-        QA_HIP(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-        // FP64 tiles are the larger ones
-        size_t need = qa::fuse::ldsBytes(QA_FP64, tileQubits);
-        QA_ARG_CHECK(need <= (size_t)maxLds, "qa_setFusion: a %d-qubit tile needs %zu bytes of LDS, the device allows %d",
-                     tileQubits, need, maxLds);
-        if (need > 65536) {
-            for (int v = 0; v < 8; v++) {
-                const void* fn = qa::fuse::tileKernel((v & 4) ? QA_FP64 : QA_FP32, v & 2, v & 1);
-                QA_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
-            }
-        }
-    }
+    rc = qa::fuse::allowTileLds(tileQubits);
+    if (rc != QA_OK) return rc;
     std::lock_guard<std::mutex> lk(qa::fuse::g_mu);
     qa::fuse::g_cfg.enabled = enabled != 0;
     qa::fuse::g_cfg.tileQubits = tileQubits;
@@ -1226,6 +1609,8 @@ int qa_setFusionTiming(int enabled) {
     qa::fuse::g_stats.used = 0;
     qa::fuse::g_stats.passes = qa::fuse::g_stats.gates = 0;
     qa::fuse::g_stats.progOps = qa::fuse::g_stats.layouts = 0;
+    qa::fuse::g_stats.spanPasses = qa::fuse::g_stats.spanGates = 0;
+    qa::fuse::g_stats.passMs = qa::fuse::g_stats.spanMs = 0;
     qa::fuse::g_stats.flops = 0;
     return QA_OK;
 }
@@ -1235,19 +1620,14 @@ int qa_getFusionStats(int64_t* passes, int64_t* gates, double* passMs, double* f
     QA_FLUSH_ALL();
     std::lock_guard<std::mutex> lk(qa::fuse::g_mu);
     qa::fuse::Stats& st = qa::fuse::g_stats;
-    double total = 0;
-    for (size_t i = 0; i < st.used; i++) {
-        float f = 0;
-        QA_HIP(hipEventSynchronize(st.pool[i].second));
-        QA_HIP(hipEventElapsedTime(&f, st.pool[i].first, st.pool[i].second));
-        total += f;
-    }
+    const int rc = qa::fuse::collectTimes();
+    if (rc != QA_OK) return rc;
     if (passes) *passes = st.passes;
     if (gates) *gates = st.gates;
-    if (passMs) *passMs = total;
+    if (passMs) *passMs = st.passMs;
     if (flops) *flops = st.flops;
-    st.used = 0;
     st.passes = st.gates = 0;
+    st.passMs = 0;
     st.flops = 0;
     return QA_OK;
 }
@@ -1268,6 +1648,31 @@ int qa_setFusionPrefetch(int enabled) {
     QA_FLUSH_ALL();
     std::lock_guard<std::mutex> lk(qa::fuse::g_mu);
     qa::fuse::g_cfg.prefetch = enabled != 0;
+    return QA_OK;
+}
+
+int qa_setFusionSpan(int spanQubits) {
+    QA_REQUIRE_INIT();
+    QA_ARG_CHECK(spanQubits == 0 || (spanQubits >= 10 && spanQubits <= 13),
+                 "qa_setFusionSpan: spanQubits must be 0 (off) or in [10, 13]");
+    QA_FLUSH_ALL();
+    std::lock_guard<std::mutex> lk(qa::fuse::g_mu);
+    qa::fuse::g_cfg.spanQubits = spanQubits;
+    return QA_OK;
+}
+
+int qa_getFusionSpanStats(int64_t* passes, int64_t* gates, double* passMs) {
+    QA_REQUIRE_INIT();
+    QA_FLUSH_ALL();
+    std::lock_guard<std::mutex> lk(qa::fuse::g_mu);
+    qa::fuse::Stats& st = qa::fuse::g_stats;
+    const int rc = qa::fuse::collectTimes();
+    if (rc != QA_OK) return rc;
+    if (passes) *passes = st.spanPasses;
+    if (gates) *gates = st.spanGates;
+    if (passMs) *passMs = st.spanMs;
+    st.spanPasses = st.spanGates = 0;
+    st.spanMs = 0;
     return QA_OK;
 }
 

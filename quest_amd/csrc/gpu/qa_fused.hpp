@@ -28,6 +28,8 @@ struct Config {
     int minQubits = 1;
     bool prefetch = true;  // tile passes prefetch each workgroup's next tile into registers
     bool exact = false;    // passes apply gates in queue order, each with its single-gate arithmetic
+    int spanQubits = 12;   // runs of Pauli and phase gadgets may go to span passes over cosets of
+                           // 2^spanQubits amplitudes (0 = off)
     int blockMinGates = 0; // complex64 states: runs of at least this many gates on 5 qubits are
                            // multiplied into one 32x32 block applied with FP32 MFMA (0 = off)
 };

@@ -282,6 +282,9 @@ struct StepBuf {
     double* outHost = nullptr;
     size_t outCap = 0;
     hipEvent_t done = nullptr;
+    // this slot's executable graphs (graphs on), one per launch signature:
+    // (upload, number of pass launches, reduction) -> graph
+    std::map<int, hipGraphExec_t> execs;
     std::vector<Request*> finishing;
 };
 
@@ -311,7 +314,9 @@ struct Engine {
     std::deque<int> inflight;
     std::thread th;
 
+    std::atomic<bool> graphs{false};  // launch each step as a captured HIP graph
     std::atomic<int64_t> nSteps{0}, nPasses{0}, nAdmitted{0}, nCompleted{0}, maxRunning{0};
+    std::atomic<int64_t> nGraphLaunches{0}, nGraphInstances{0}, nGraphUpdates{0};
     std::atomic<int64_t> planNs{0}, waitNs{0};  // engine thread: planning + launching, waiting for steps
 };
 
@@ -357,6 +362,52 @@ bool reserve(StepBuf& b, size_t bytes, size_t chunks, size_t outs, std::string& 
 template <typename T>
 const void* passKernel() {
     return (const void*)k_batchPass<T>;
+}
+
+// Captures the step's launches (`issue`) into a graph and launches it as
+// one of the slot's executable graphs: the one of the step's launch
+// signature, updated in place (hipGraphExecUpdate: new grid sizes,
+// arguments and copy sizes), else a new instantiation.  The slot is never
+// in flight here (the engine retires a slot's step before reusing it).  The
+// graph holds copies and kernels only.
+template <typename Issue>
+bool launchGraph(Engine& E, StepBuf& sb, int signature, Issue& issue, std::string& err) {
+    if (!hipOk(hipStreamBeginCapture(E.stream, hipStreamCaptureModeThreadLocal), err, "serve: begin capture"))
+        return false;
+    const bool ok = issue();
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(E.stream, &g);
+    if (!ok || ec != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (ok) hipOk(ec, err, "serve: end capture");
+        return false;
+    }
+    bool ready = false;
+    hipGraphExec_t& exec = sb.execs[signature];
+    if (exec) {
+        hipGraphNode_t errNode = nullptr;
+        hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+        if (hipGraphExecUpdate(exec, g, &errNode, &res) == hipSuccess && res == hipGraphExecUpdateSuccess) {
+            ready = true;
+            E.nGraphUpdates++;
+        } else {
+            (void)hipGetLastError();
+            (void)hipGraphExecDestroy(exec);
+            exec = nullptr;
+        }
+    }
+    if (!ready) {
+        if (!hipOk(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0), err, "serve: graph instantiation")) {
+            exec = nullptr;
+            (void)hipGraphDestroy(g);
+            return false;
+        }
+        E.nGraphInstances++;
+    }
+    (void)hipGraphDestroy(g);
+    if (!hipOk(hipGraphLaunch(exec, E.stream), err, "serve: graph launch")) return false;
+    E.nGraphLaunches++;
+    return true;
 }
 
 // plans and launches one step; false with `err` set on failure
@@ -453,10 +504,22 @@ bool launchStep(Engine& E, StepBuf& sb, std::string& err) {
     const size_t jobsAt = (opsAt + opBytes + 63) & ~size_t(63);
     if (jobBytes) std::memcpy(sb.host + jobsAt, jobs.data(), jobBytes);
     const size_t total = jobsAt + jobBytes;
-    if (total && !hipOk(hipMemcpyAsync(sb.dev, sb.host, total, hipMemcpyHostToDevice, E.stream), err, "serve: upload"))
-        return false;
 
-    const TOp* devOps = reinterpret_cast<const TOp*>(sb.dev + opsAt);
+    // launch shapes, worked out before any launch (a captured step may not
+    // query the runtime for them)
+    struct PassLaunch {
+        const PassDesc* dp;
+        int num;
+        uint64_t tiles;
+        unsigned grid, block;
+        size_t lds;
+    };
+    std::vector<PassLaunch> launches;
+    const void* fn = E.prec == QA_FP64 ? passKernel<double>() : passKernel<float>();
+    if (!groupAt.empty() && !ldsTileAtZero(fn)) {
+        err = "serve: the pass kernel has static LDS; its LDS tile would not start at address 0";
+        return false;
+    }
     for (auto& g : groupAt) {
         const int k = g.first;
         const std::vector<PassDesc>& v = byK[k];
@@ -466,37 +529,54 @@ bool launchStep(Engine& E, StepBuf& sb, std::string& err) {
             tiles += pd.a.numTiles;
             lds = lds || pd.a.usesLds;
         }
-        const unsigned block = 1u << (k - RB);
-        const size_t ldsBytes = lds ? (E.ampBytes << k) : 0;
-        const void* fn = E.prec == QA_FP64 ? passKernel<double>() : passKernel<float>();
-        if (!ldsTileAtZero(fn)) {
-            err = "serve: the pass kernel has static LDS; its LDS tile would not start at address 0";
+        PassLaunch l;
+        l.block = 1u << (k - RB);
+        l.lds = lds ? (E.ampBytes << k) : 0;
+        l.grid = (unsigned)std::min<uint64_t>(tiles, residentBlocks(fn, l.block, l.lds));
+        l.dp = reinterpret_cast<const PassDesc*>(sb.dev) + g.second;
+        l.num = (int)v.size();
+        l.tiles = tiles;
+        launches.push_back(l);
+    }
+    const TOp* devOps = reinterpret_cast<const TOp*>(sb.dev + opsAt);
+    const ExpJob* dj = reinterpret_cast<const ExpJob*>(sb.dev + jobsAt);
+    const unsigned expGrid = (unsigned)std::min<uint64_t>(chunks, 8192);
+
+    // the step's copies and launches on the engine stream, in order
+    auto issue = [&]() -> bool {
+        if (total && !hipOk(hipMemcpyAsync(sb.dev, sb.host, total, hipMemcpyHostToDevice, E.stream), err,
+                            "serve: upload"))
             return false;
+        for (const PassLaunch& l : launches) {
+            if (E.prec == QA_FP64)
+                k_batchPass<double><<<l.grid, l.block, l.lds, E.stream>>>(l.dp, l.num, devOps, l.tiles);
+            else
+                k_batchPass<float><<<l.grid, l.block, l.lds, E.stream>>>(l.dp, l.num, devOps, l.tiles);
+            if (!hipOk(hipGetLastError(), err, "serve: pass launch")) return false;
         }
-        const unsigned grid = (unsigned)std::min<uint64_t>(tiles, residentBlocks(fn, block, ldsBytes));
-        const PassDesc* dp = reinterpret_cast<const PassDesc*>(sb.dev) + g.second;
-        if (E.prec == QA_FP64)
-            k_batchPass<double><<<grid, block, ldsBytes, E.stream>>>(dp, (int)v.size(), devOps, tiles);
-        else
-            k_batchPass<float><<<grid, block, ldsBytes, E.stream>>>(dp, (int)v.size(), devOps, tiles);
-        if (!hipOk(hipGetLastError(), err, "serve: pass launch")) return false;
-        E.nPasses += (int64_t)v.size();
+        if (!jobs.empty()) {
+            if (E.prec == QA_FP64)
+                k_batchExpec<double><<<expGrid, BLOCK, 0, E.stream>>>(dj, (int)jobs.size(), chunks, sb.partials);
+            else
+                k_batchExpec<float><<<expGrid, BLOCK, 0, E.stream>>>(dj, (int)jobs.size(), chunks, sb.partials);
+            if (!hipOk(hipGetLastError(), err, "serve: expectation launch")) return false;
+            k_batchExpecFinal<<<(unsigned)jobs.size(), BLOCK, 0, E.stream>>>(dj, sb.partials, sb.outDev);
+            if (!hipOk(hipGetLastError(), err, "serve: expectation merge launch")) return false;
+            if (!hipOk(hipMemcpyAsync(sb.outHost, sb.outDev, (size_t)outs * sizeof(double), hipMemcpyDeviceToHost,
+                                      E.stream),
+                       err, "serve: result copy"))
+                return false;
+        }
+        return true;
+    };
+    const bool any = total || !launches.empty() || !jobs.empty();
+    if (any && E.graphs.load()) {
+        const int signature = (total ? 1 : 0) | (jobs.empty() ? 0 : 2) | ((int)launches.size() << 2);
+        if (!launchGraph(E, sb, signature, issue, err)) return false;
+    } else if (any && !issue()) {
+        return false;
     }
-    if (!jobs.empty()) {
-        const ExpJob* dj = reinterpret_cast<const ExpJob*>(sb.dev + jobsAt);
-        const unsigned grid = (unsigned)std::min<uint64_t>(chunks, 8192);
-        if (E.prec == QA_FP64)
-            k_batchExpec<double><<<grid, BLOCK, 0, E.stream>>>(dj, (int)jobs.size(), chunks, sb.partials);
-        else
-            k_batchExpec<float><<<grid, BLOCK, 0, E.stream>>>(dj, (int)jobs.size(), chunks, sb.partials);
-        if (!hipOk(hipGetLastError(), err, "serve: expectation launch")) return false;
-        k_batchExpecFinal<<<(unsigned)jobs.size(), BLOCK, 0, E.stream>>>(dj, sb.partials, sb.outDev);
-        if (!hipOk(hipGetLastError(), err, "serve: expectation merge launch")) return false;
-        if (!hipOk(hipMemcpyAsync(sb.outHost, sb.outDev, (size_t)outs * sizeof(double), hipMemcpyDeviceToHost,
-                                  E.stream),
-                   err, "serve: result copy"))
-            return false;
-    }
+    for (const PassLaunch& l : launches) E.nPasses += (int64_t)l.num;
     if (!hipOk(hipEventRecord(sb.done, E.stream), err, "serve: step event")) return false;
     E.nSteps++;
     return true;
@@ -771,6 +851,8 @@ int qa_serveDestroy(void* engine) {
         if (s.outDev) (void)hipFree(s.outDev);
         if (s.outHost) (void)hipHostFree(s.outHost);
         if (s.done) (void)hipEventDestroy(s.done);
+        for (auto& kv : s.execs)
+            if (kv.second) (void)hipGraphExecDestroy(kv.second);
     }
     (void)hipFree(E->arena);
     (void)hipStreamDestroy(E->stream);
@@ -853,6 +935,21 @@ int qa_serveWait(void* engine, int64_t id, double* expectations, double* latency
     if (latencyMs) *latencyMs = std::chrono::duration<double, std::milli>(r->finished - r->submitted).count();
     E->byId.erase(it);
     delete r;
+    return QA_OK;
+}
+
+int qa_serveSetGraphs(void* engine, int enabled) {
+    QA_ARG_CHECK(engine, "qa_serveSetGraphs: null engine");
+    ((Engine*)engine)->graphs = enabled != 0;
+    return QA_OK;
+}
+
+int qa_serveGraphStats(void* engine, int64_t* launches, int64_t* instantiations, int64_t* updates) {
+    QA_ARG_CHECK(engine, "qa_serveGraphStats: null engine");
+    Engine* E = (Engine*)engine;
+    if (launches) *launches = E->nGraphLaunches;
+    if (instantiations) *instantiations = E->nGraphInstances;
+    if (updates) *updates = E->nGraphUpdates;
     return QA_OK;
 }
 

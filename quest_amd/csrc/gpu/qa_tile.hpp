@@ -7,6 +7,7 @@
 
 #include "qa_common.hpp"
 #include "qa_fused.hpp"
+#include "qa_span.hpp"
 #include "qa_tiledev.hpp"
 
 namespace qa {
@@ -88,6 +89,26 @@ void planPassBlocks(const GOp* ops, size_t num, int numQubits, uint64_t localMas
 // whose first op is a GROUP), for amplitudes of precision prec: the fx words
 // of its GROUP, PAULI_LDS and BLOCK ops, and a.rowOff / a.lastOff
 void setLdsOffsets(TOp* prog, size_t num, TileArgs& a, int prec);
+
+// ---- span passes (qa_span.hpp) -----------------------------------------------
+
+constexpr int MAX_SPAN_OPS = 128;
+
+// true for the ops a span pass takes: Pauli gadgets and tensors, and phase
+// gadgets, without controls
+bool spanEligible(const GOp& g);
+
+// the longest run at the front of ops[0..num) (at most MAX_SPAN_OPS) of
+// span-eligible ops whose X/Y masks, with the `low` lowest qubits, span at
+// most k dimensions, planned as one span pass over cosets of 2^k
+// amplitudes: fills every field of `a` but repStep and the ops in `prog`.
+// Returns the run's length (0 when n <= k or k is outside [10, 13]).
+size_t spanRun(const GOp* ops, size_t num, int numQubits, int k, int low, SpanArgs& a, std::vector<SpanOp>& prog);
+
+// the span-pass kernel and its launch (qa_span.hip)
+const void* spanKernel(int prec, bool nt);
+void launchSpanAny(int prec, bool nt, unsigned grid, unsigned block, size_t lds, void* amps, const SpanArgs& a,
+                   const SpanOp* ops);
 
 }  // namespace fuse
 }  // namespace qa

@@ -117,6 +117,16 @@ class Engine:
         out.update(plan_ms=pm.value, wait_ms=wm.value)
         return out
 
+    def set_graphs(self, enabled: bool) -> None:
+        """Launch each engine step as one captured HIP graph (True) or launch
+        by launch (False, the default)."""
+        _check(self._lib.qa_serveSetGraphs(self._h, int(bool(enabled))), "qa_serveSetGraphs")
+
+    def graph_stats(self) -> dict:
+        v = [C.c_int64() for _ in range(3)]
+        _check(self._lib.qa_serveGraphStats(self._h, *[C.byref(x) for x in v]), "qa_serveGraphStats")
+        return dict(zip(["launches", "instantiations", "updates"], [x.value for x in v]))
+
     def close(self) -> None:
         if self._h:
             _check(self._lib.qa_serveDestroy(self._h), "qa_serveDestroy")
@@ -152,6 +162,8 @@ def _lib():
         lib.qa_serveWait.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         lib.qa_serveStats.argtypes = [C.c_void_p] + [C.POINTER(C.c_int64)] * 5
         lib.qa_serveTimes.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        lib.qa_serveSetGraphs.argtypes = [C.c_void_p, C.c_int]
+        lib.qa_serveGraphStats.argtypes = [C.c_void_p] + [C.POINTER(C.c_int64)] * 3
         lib.qa_lastError.restype = C.c_char_p
         _LIB = lib
     return _LIB

@@ -319,9 +319,215 @@ struct JitCheck {
     }
 };
 
+// ---- span passes ----------------------------------------------------------
+
+static int rankGf2(std::vector<uint64_t> v) {
+    int rank = 0;
+    for (int bit = 63; bit >= 0; bit--) {
+        size_t p = v.size();
+        for (size_t i = 0; i < v.size(); i++)
+            if ((v[i] >> bit) & 1ull) {
+                p = i;
+                break;
+            }
+        if (p == v.size()) continue;
+        const uint64_t pv = v[p];
+        v.erase(v.begin() + (ptrdiff_t)p);
+        for (auto& x : v)
+            if ((x >> bit) & 1ull) x ^= pv;
+        rank++;
+    }
+    return rank;
+}
+
+// a gadget over random qubits: PAULI (X/Y somewhere) or PARITY
+static GOp randomGadget(Rng& r, int n, int maxQubits) {
+    GOp g;
+    const int k = 1 + r.below(maxQubits);
+    uint64_t x = 0, y = 0, z = 0;
+    for (int j = 0; j < k; j++) {
+        const uint64_t b = 1ull << r.below(n);
+        if ((x | y | z) & b) continue;
+        const int c = r.below(3);
+        (c == 0 ? x : c == 1 ? y : z) |= b;
+    }
+    const double th = r.unit() * 3;
+    if (x | y) {
+        g.kind = F_PAULI;
+        g.m1 = x | y;
+        g.m2 = y | z;
+        g.f[0] = std::cos(th);
+        g.f[3] = std::sin(th);
+    } else {
+        g.kind = F_PARITY;
+        g.m1 = z ? z : 1ull;
+        g.f[0] = std::cos(th);
+        g.f[1] = -std::sin(th);
+        g.f[2] = std::cos(th);
+        g.f[3] = std::sin(th);
+    }
+    return g;
+}
+
+static uint64_t depositBits(uint64_t v, const int* pos, int num) {
+    uint64_t out = 0;
+    for (int i = 0; i < num; i++) out |= ((v >> i) & 1ull) << pos[i];
+    return out;
+}
+
+// one span pass on the full state, following the kernel's indexing: a grid
+// of `grid` workgroups, each walking its cosets by masked adds; thread t,
+// register i holds coordinate t + i * 2^(k-4) at rep ^ own(t) ^ top[i].
+// Every amplitude must be visited exactly once.
+static bool emulateSpan(const SpanArgs& a, const std::vector<SpanOp>& prog, uint64_t grid, std::vector<cx>& amps,
+                        int c) {
+    const uint32_t nth = 1u << (a.k - RB), size = 1u << a.k, half = size / 2;
+    std::vector<uint8_t> seen(amps.size(), 0);
+    std::vector<cx> tile(size);
+    const uint64_t step = depositBits(grid, a.repPos, a.numRep);
+    for (uint64_t b = 0; b < grid && b < a.numCosets; b++) {
+        uint64_t rep = depositBits(b, a.repPos, a.numRep);
+        for (uint64_t w = b; w < a.numCosets; w += grid) {
+            std::vector<uint64_t> addr(size);
+            for (uint32_t t = 0; t < nth; t++) {
+                uint64_t own = 0;
+                for (int j = 0; j < a.k - RB; j++)
+                    if ((t >> j) & 1u) own ^= a.basis[j];
+                for (int i = 0; i < RA; i++) {
+                    const uint32_t s = t + (uint32_t)i * nth;
+                    addr[s] = rep ^ own ^ a.top[i];
+                    if (addr[s] >= amps.size() || seen[addr[s]]++) {
+                        std::printf("case %d: coset %llu visits amplitude %llu twice or out of range\n", c,
+                                    (unsigned long long)w, (unsigned long long)addr[s]);
+                        return false;
+                    }
+                    tile[s] = amps[addr[s]];
+                }
+            }
+            for (const SpanOp& o : prog) {
+                const int rp = parity(rep & o.z);
+                const cx f0(o.f[0], o.f[1]), f1(o.f[2], o.f[3]);
+                if (o.kind == F_PARITY) {
+                    for (uint32_t s = 0; s < size; s++) tile[s] *= (parity(s & o.d) ^ rp) ? f1 : f0;
+                    continue;
+                }
+                const uint32_t lowMask = (1u << o.top) - 1u;
+                for (uint32_t j = 0; j < half; j++) {
+                    const uint32_t s = ((j & ~lowMask) << 1) | (j & lowMask), v = s ^ o.c;
+                    const double sA = (parity(s & o.d) ^ rp) ? -1 : 1, sB = (parity(v & o.d) ^ rp) ? -1 : 1;
+                    const cx xa = tile[s], xb = tile[v];
+                    tile[s] = f0 * xa + f1 * sB * xb;
+                    tile[v] = f0 * xb + f1 * sA * xa;
+                }
+            }
+            for (uint32_t s = 0; s < size; s++) amps[addr[s]] = tile[s];
+            rep = ((rep | ~a.repMask) + step) & a.repMask;
+        }
+    }
+    for (size_t i = 0; i < seen.size(); i++)
+        if (seen[i] != 1) {
+            std::printf("case %d: amplitude %zu visited %d times\n", c, i, (int)seen[i]);
+            return false;
+        }
+    return true;
+}
+
+// random runs of gadgets (with an occasional op a span pass does not take)
+// planned by spanRun as a flush does, each span pass emulated on the full
+// state; must agree with the gates one by one, and each run must be the
+// longest whose X/Y masks with the low qubits span at most k dimensions
+static int spanMain(int cases, uint64_t seed0) {
+    int checked = 0, spanPasses = 0;
+    for (int c = 0; c < cases; c++) {
+        Rng r(seed0 * 7919ull + (uint64_t)c);
+        const int n = 11 + r.below(6);
+        const int k = 10 + r.below(std::min(4, n - 10));
+        const int low = r.below(4);
+        const int num = 10 + r.below(60);
+        std::vector<GOp> queue;
+        for (int i = 0; i < num; i++) {
+            if (r.below(12) == 0) {
+                GOp g = randomGadget(r, n, 6);
+                g.cMask = g.cVal = 0;
+                if (r.below(2)) {
+                    g.kind = F_DENSE1;
+                    g.t1 = r.below(n);
+                    g.m1 = g.m2 = 0;
+                    randomUnitary2(r, g.f);
+                } else {
+                    // a controlled gadget: not taken
+                    uint64_t free = ((1ull << n) - 1ull) & ~(g.m1 | g.m2);
+                    if (!free) continue;
+                    int q = 0;
+                    while (!((free >> q) & 1ull)) q++;
+                    g.cMask = g.cVal = 1ull << q;
+                }
+                queue.push_back(g);
+                continue;
+            }
+            queue.push_back(randomGadget(r, n, 1 + r.below(std::min(n, 12))));
+        }
+        std::vector<cx> want(1ull << n);
+        for (auto& x : want) x = cx(r.unit(), r.unit());
+        std::vector<cx> got = want;
+        for (const GOp& g : queue) ::apply(g, want);
+        std::vector<GOp> ops = queue;
+        SpanArgs a;
+        std::vector<SpanOp> prog;
+        while (!ops.empty()) {
+            const size_t run = spanRun(ops.data(), ops.size(), n, k, low, a, prog);
+            std::vector<uint64_t> masks;
+            for (int q = 0; q < low; q++) masks.push_back(1ull << q);
+            for (size_t i = 0; i < run; i++)
+                if (ops[i].kind == F_PAULI) masks.push_back(ops[i].m1);
+            if (run && rankGf2(masks) > k) {
+                std::printf("case %d: a run of %zu spans %d > %d dimensions\n", c, run, rankGf2(masks), k);
+                return 1;
+            }
+            if (run < ops.size() && run < (size_t)MAX_SPAN_OPS && spanEligible(ops[run])) {
+                masks.push_back(ops[run].m1);
+                if (ops[run].kind != F_PAULI || rankGf2(masks) <= k) {
+                    std::printf("case %d: the run of %zu stops before an op that fits\n", c, run);
+                    return 1;
+                }
+            }
+            if (run == 0) {
+                if (spanEligible(ops[0])) {
+                    std::printf("case %d: an eligible op starts no run\n", c);
+                    return 1;
+                }
+                ::apply(ops[0], got);
+                ops.erase(ops.begin());
+                continue;
+            }
+            if (prog.size() != run || a.numOps != (int)run || a.k != k) {
+                std::printf("case %d: plan of %zu ops has %zu ops\n", c, run, prog.size());
+                return 1;
+            }
+            const uint64_t grid = 1 + (uint64_t)r.below((int)std::min<uint64_t>(a.numCosets, 40));
+            if (!emulateSpan(a, prog, grid, got, c)) return 1;
+            spanPasses++;
+            ops.erase(ops.begin(), ops.begin() + (ptrdiff_t)run);
+        }
+        double err = 0, norm = 0;
+        for (size_t i = 0; i < got.size(); i++) {
+            err = std::max(err, std::abs(got[i] - want[i]));
+            norm = std::max(norm, std::abs(want[i]));
+        }
+        if (!(err <= 1e-12 * std::max(1.0, norm))) {
+            std::printf("case %d (n=%d k=%d low=%d gates=%zu): max error %.3e\n", c, n, k, low, queue.size(), err);
+            return 1;
+        }
+        checked++;
+    }
+    std::printf("span check: %d cases agree (%d span passes)\n", checked, spanPasses);
+    return 0;
+}
+
 int main(int argc, char** argv) {
     const int cases = argc > 1 ? std::atoi(argv[1]) : 200;
     const uint64_t seed0 = argc > 2 ? std::strtoull(argv[2], nullptr, 10) : 1;
+    if (argc > 3 && std::string(argv[3]) == "span") return spanMain(cases, seed0);
     JitCheck jit;
     if (argc > 3 && std::string(argv[3]) == "jit") {
         jit.on = true;
@@ -349,7 +555,7 @@ int main(int argc, char** argv) {
         std::vector<cx> a(1ull << n);
         for (auto& x : a) x = cx(r.unit(), r.unit());
         std::vector<cx> want = a, got = a;
-        for (const GOp& g : queue) apply(g, want);
+        for (const GOp& g : queue) ::apply(g, want);
 
         std::vector<GOp> ops = queue, pass, rest, merged, planned;
         std::vector<uint32_t> picked;

@@ -78,14 +78,18 @@ def test_generated_kernels_match_the_interpreter(api, name, make, shape, exact, 
     mid = api.fusion_jit_stats()
     assert mid["failures"] == before["failures"], mid["last_error"]
     passes = api.fusion_stats()["passes"]
+    api.fusion_span_stats()
     got, gres = run(api, c, shape, exact, precision, blocks)
     after = api.fusion_jit_stats()
     passes2 = api.fusion_stats()["passes"]
+    # span passes (runs of gadgets) have a kernel of their own
+    spans = api.fusion_span_stats()["passes"]
     say(f"  {passes} passes; {mid['kernels'] - before['kernels']} kernels compiled "
         f"({mid['compile_seconds'] - before['compile_seconds']:.1f} s of compile threads); "
-        f"second run {after['launches'] - mid['launches']} of {passes2} passes in generated kernels")
+        f"second run {after['launches'] - mid['launches']} of {passes2 - spans} tile passes in generated kernels "
+        f"({spans} span passes)")
     assert passes2 == passes and passes > 0
-    assert after["launches"] - mid["launches"] == passes2
+    assert after["launches"] - mid["launches"] == passes2 - spans
     assert np.array_equal(got, want), f"max |diff| {np.max(np.abs(got - want))}"
     assert len(gres) == len(wres)
     for (lg, vg), (lw, vw) in zip(gres, wres):
@@ -102,3 +106,35 @@ def test_jit_settings_validation(api):
     m, s = C.c_int(), C.c_int()
     g.qa_getFusionJit(C.byref(m), C.byref(s))
     assert (m.value, s.value) == (0, 2)
+
+
+EXIT_SCRIPT = r"""
+import sys
+sys.path[:0] = [{root!r}, {tests!r}]
+from quest_amd import api, circuits as qc
+from test_gpu_fusion import fusion_mix
+api.ensure_env()
+api.set_fusion(True, 6, 2, 2048)
+api.set_fusion_jit(1, 1)
+c = fusion_mix(12, 160, 21)
+q = api.createCustomPrecisionQureg(c.num_qubits, 0, 2)
+qc.run(c, q)
+print("exiting with", api.fusion_jit_stats()["kernels"], "kernels compiled", flush=True)
+"""
+
+
+def test_exit_with_compiles_in_flight(tmp_path):
+    """A program that ends while pass kernels are still being compiled in
+    the background, without qa_finalize (as a QuEST program through the
+    accelerator bridge does), must exit cleanly: the compile threads are
+    stopped before the HIP runtime and the compiler are torn down."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    script = tmp_path / "exit_in_flight.py"
+    script.write_text(EXIT_SCRIPT.format(root=root, tests=os.path.join(root, "tests")))
+    say("jit exit case: 12-qubit fusion mix, every shape queued at first sight, exit without finalize")
+    proc = subprocess.run([sys.executable, str(script)], capture_output=True, text=True, timeout=110)
+    say(proc.stdout.strip()[-300:])
+    assert proc.returncode == 0, proc.stderr[-2000:]
+    assert "exiting with" in proc.stdout

@@ -134,3 +134,37 @@ def test_empty_circuit_reads_the_zero_state(api):
         rid = eng.submit(n, [], paulis)
         got, _ = eng.wait(rid)
     assert list(got) == [1.0, 0.0, 1.0], got
+
+
+@pytest.mark.parametrize("precision", [2, 1])
+def test_graph_steps_match_eager_steps(api, precision):
+    """Steps launched as captured HIP graphs (each step slot's executable
+    graph updated in place between steps, or instantiated anew when the
+    launches change) give bitwise the results of steps launched one by one.
+    The graph engine runs many steps (each a graph replay), with requests of
+    1-3 tile sizes per step, and eager steps of the same engine run in
+    between (graphs switched off and on again mid-stream)."""
+    say(f"[gpu] serve graphs vs eager precision={precision}")
+    reqs = S.serving_requests(40, 21, qubits=(8, 20), depth=(2, 12))
+    with S.Engine(1 << 30, max_active=16, tile_qubits=12, precision=precision) as eng:
+        want = [run_alone(eng, c) for c in reqs]
+    with S.Engine(1 << 30, max_active=16, tile_qubits=12, precision=precision) as eng:
+        eng.set_graphs(True)
+        got = []
+        for i, c in enumerate(reqs):
+            if i == 15:
+                eng.set_graphs(False)
+            if i == 20:
+                eng.set_graphs(True)
+            got.append(run_alone(eng, c))
+        # and a concurrent batch: steps with several requests and tile sizes
+        ids = [eng.submit(c.num_qubits, c.ops, S.paulis_of(c)) for c in reqs]
+        batch = [eng.wait(rid)[0] for rid in ids]
+        gs = eng.graph_stats()
+        st = eng.stats()
+    say(f"  {st['steps']} steps, graphs: {gs}")
+    assert gs["launches"] > 0 and gs["updates"] > 0, gs
+    assert gs["instantiations"] + gs["updates"] == gs["launches"], gs
+    for i, c in enumerate(reqs):
+        assert np.array_equal(got[i], want[i]), (i, c.name)
+        assert np.array_equal(batch[i], want[i]), (i, c.name)

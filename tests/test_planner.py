@@ -38,6 +38,18 @@ def test_planned_passes_act_like_the_queue(planner_check, seed):
     assert "300 cases agree" in out.stdout
 
 
+@pytest.mark.parametrize("seed", [1, 2])
+def test_span_passes_act_like_the_gadgets(planner_check, seed):
+    """Runs of Pauli and phase gadgets planned as span passes (cosets of the
+    span of their X/Y masks, applied in coordinates) and emulated with the
+    kernel's indexing: every amplitude visited once, results equal to the
+    gadgets one by one to rounding, and every run as long as k dimensions
+    allow."""
+    out = subprocess.run([str(planner_check), "300", str(seed), "span"], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "300 cases agree" in out.stdout
+
+
 def test_generated_pass_kernels_compile_and_cache_soundly(planner_check):
     """Every planned pass program also gets its generated kernel source:
     programs with equal shape keys must generate equal sources (the per-shape

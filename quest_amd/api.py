@@ -488,7 +488,7 @@ def set_fusion_span(span_qubits=12) -> None:
     q = 12 if span_qubits is True else int(span_qubits)
     rc = gpu().qa_setFusionSpan(q)
     if rc != 0:
-        raise QuESTError(f"qa_setFusionSpan failed ({rc}): {_gpu_last_error()}")
+        raise QuESTError("qa_setFusionSpan", f"failed ({rc}): {_gpu_last_error()}")
 
 
 class FusionTrial(C.Structure):
@@ -504,7 +504,7 @@ def tune_fusion(qureg, retune: bool = False) -> dict:
     rc = gpu().qa_tuneFusion(C.c_void_p(qureg.gpuAmps), int(bool(retune)), trials, 32, C.byref(num),
                              C.byref(best))
     if rc != 0:
-        raise QuESTError(f"qa_tuneFusion failed ({rc}): {_gpu_last_error()}")
+        raise QuESTError("qa_tuneFusion", f"failed ({rc}): {_gpu_last_error()}")
     out = [{"tile_qubits": t.tileQubits, "low_qubits": t.lowQubits, "span_qubits": t.spanQubits,
             "block_min_gates": t.blockMinGates, "ms": t.ms} for t in trials[:min(num.value, 32)]]
     return {"trials": out, "best": out[best.value] if 0 <= best.value < len(out) else None}
@@ -516,7 +516,7 @@ def fusion_span_stats() -> dict:
     passes, gates, ms = C.c_int64(), C.c_int64(), C.c_double()
     rc = gpu().qa_getFusionSpanStats(C.byref(passes), C.byref(gates), C.byref(ms))
     if rc != 0:
-        raise QuESTError(f"qa_getFusionSpanStats failed ({rc}): {_gpu_last_error()}")
+        raise QuESTError("qa_getFusionSpanStats", f"failed ({rc}): {_gpu_last_error()}")
     return {"passes": passes.value, "gates": gates.value, "pass_ms": ms.value}
 
 
@@ -528,11 +528,11 @@ def fusion_stats() -> dict:
     passes, gates, ms, flops = C.c_int64(), C.c_int64(), C.c_double(), C.c_double()
     rc = gpu().qa_getFusionStats(C.byref(passes), C.byref(gates), C.byref(ms), C.byref(flops))
     if rc != 0:
-        raise QuESTError(f"qa_getFusionStats failed ({rc}): {_gpu_last_error()}")
+        raise QuESTError("qa_getFusionStats", f"failed ({rc}): {_gpu_last_error()}")
     ops, layouts = C.c_int64(), C.c_int64()
     rc = gpu().qa_getFusionProgramStats(C.byref(ops), C.byref(layouts))
     if rc != 0:
-        raise QuESTError(f"qa_getFusionProgramStats failed ({rc}): {_gpu_last_error()}")
+        raise QuESTError("qa_getFusionProgramStats", f"failed ({rc}): {_gpu_last_error()}")
     return {"passes": passes.value, "gates": gates.value, "pass_ms": ms.value, "flops": flops.value,
             "ops": ops.value, "layouts": layouts.value}
 
@@ -544,7 +544,7 @@ def set_fusion_exact(exact: bool) -> None:
     one or two qubits into one matrix."""
     rc = gpu().qa_setFusionExact(C.c_int(int(bool(exact))))
     if rc != 0:
-        raise QuESTError(f"qa_setFusionExact failed ({rc}): {_gpu_last_error()}")
+        raise QuESTError("qa_setFusionExact", f"failed ({rc}): {_gpu_last_error()}")
 
 
 def set_fusion_jit(mode: int, min_seen: int = 2) -> None:
@@ -553,7 +553,7 @@ def set_fusion_jit(mode: int, min_seen: int = 2) -> None:
     times, 2 compile at first use and wait."""
     rc = gpu().qa_setFusionJit(C.c_int(int(mode)), C.c_int(int(min_seen)))
     if rc != 0:
-        raise QuESTError(f"qa_setFusionJit failed ({rc}): {_gpu_last_error()}")
+        raise QuESTError("qa_setFusionJit", f"failed ({rc}): {_gpu_last_error()}")
 
 
 def fusion_jit_wait() -> None:

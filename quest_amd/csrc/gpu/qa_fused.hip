@@ -25,6 +25,8 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -1441,6 +1443,13 @@ extern "C" int qa_tuneFusion(void* amps, int flags, qa_fusionTrial* trials, int 
     Queue& qu = it->second;
     const int n = qu.numQubits, prec = qu.prec;
     const std::vector<GOp> batch = qu.ops;
+    // other states' queued gates were taken under the current shape: apply
+    // them before it changes
+    for (auto& kv : g_queues)
+        if (kv.first != amps && !kv.second.ops.empty()) {
+            const int r = flushQueue(kv.first, kv.second);
+            if (r != QA_OK) return r;
+        }
     int dev = 0;
     QA_HIP(hipGetDevice(&dev));
     const std::string key = tuneKey(dev, prec, n, batch);
@@ -1471,9 +1480,9 @@ extern "C" int qa_tuneFusion(void* amps, int flags, qa_fusionTrial* trials, int 
         return flushQueue(amps, qu);
     }
     // a scratch copy of the state: every candidate applies the batch to it
-    const size_t bytes = (prec == QA_FP64 ? 16 : 8) << n;
+    const size_t bytes = (size_t)(prec == QA_FP64 ? 16 : 8) << n;
     void* scratch = nullptr;
-    if (hipMalloc(&scratch, bytes) != hipSuccess) {
+    if (hipMalloc(&scratch, bytes) != hipSuccess || !scratch) {
         (void)hipGetLastError();
         setError("qa_tuneFusion: no room for a %zu-byte scratch copy of the state", bytes);
         return QA_ERR_OOM;
@@ -1513,6 +1522,12 @@ extern "C" int qa_tuneFusion(void* amps, int flags, qa_fusionTrial* trials, int 
         c.ms = ms;
         report(c);
         seen[k] = ms;
+        // QA_TUNE_VERBOSE: a line per shape timed, for long tuning runs
+        if (std::getenv("QA_TUNE_VERBOSE")) {
+            std::fprintf(stderr, "qa_tuneFusion: tile %d low %d span %d blocks %d: %.2f ms\n", c.tile, c.low, c.span,
+                         c.blocks, ms);
+            std::fflush(stderr);
+        }
         return ms;
     };
     TuneChoice cur{start.tileQubits, start.lowQubits, start.spanQubits, start.blockMinGates, -1};

@@ -141,6 +141,84 @@ struct FProbsOfQubits {
     }
 };
 
+// Both outcome probabilities of every qubit of a statevector of 10..PQALL
+// qubits in one read.  A wave reads 1024-amplitude chunks, lane l the
+// amplitudes chunk*1024 + 64 i + l (i < 16), so in each amplitude's index
+// bits 0-5 are the lane's, bits 6-9 are i's and the higher bits the
+// chunk's.  Each lane keeps: its total (bits 0-5 are constant for the lane;
+// the workgroup merge splits the totals by lane bit), the zero- and
+// one-sums of bits 6-9 (sums of 8 of its 16 terms per chunk), and for each
+// higher bit the zero- and one-sums of its chunk totals.  Every outcome is
+// a sum of its own terms, never a difference.  The merges are compensated;
+// a lane's chunk totals for the high bits are plain FP64 sums (at most a
+// few hundred per lane).
+constexpr int PQALL = 32;
+constexpr int PQALL_VALUES = 2 * PQALL;  // (zero, one) per qubit, merged 8 at a time
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_probsOfEveryQubit(const T* __restrict__ a, int nq, uint64_t numChunks,
+                                                             double* __restrict__ partials) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t waves = (uint64_t)gridDim.x * (BLOCK / 64);
+    KB tot = {0, 0};
+    KB mid[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) mid[j] = KB{0, 0};
+    double hi[2 * (PQALL - 10)];
+#pragma unroll
+    for (int j = 0; j < 2 * (PQALL - 10); j++) hi[j] = 0;
+    for (uint64_t c = (uint64_t)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6); c < numChunks; c += waves) {
+        const uint64_t base = (c << 10) | (uint64_t)lane;
+        double p[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) p[i] = norm2(ldd(a, base + 64 * i));
+        double g = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) g += p[i];
+        kadd(tot, g);
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            double z = 0, o = 0;
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                if ((i >> b) & 1) o += p[i];
+                else z += p[i];
+            }
+            kadd(mid[2 * b], z);
+            kadd(mid[2 * b + 1], o);
+        }
+#pragma unroll
+        for (int b = 0; b < PQALL - 10; b++) {
+            const bool one = (c >> b) & 1ull;
+            hi[2 * b] += one ? 0.0 : g;
+            hi[2 * b + 1] += one ? g : 0.0;
+        }
+    }
+    // value 2q + o: qubit q's outcome o; merged across the workgroup in
+    // groups of 8, written group-major: partials[((grp * blocks) + block) * 8 + v][2]
+#pragma unroll
+    for (int grp = 0; grp < PQALL_VALUES / 8; grp++) {
+        KB t[8];
+#pragma unroll
+        for (int v = 0; v < 8; v++) {
+            const int val = grp * 8 + v, q = val >> 1, o = val & 1;
+            if (q < 6) t[v] = (((lane >> q) & 1) == o) ? tot : KB{0, 0};
+            else if (q < 10) t[v] = mid[2 * (q - 6) + o];
+            else t[v] = KB{hi[2 * (q - 10) + o], 0};
+        }
+        blockMerge<8>(t);
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int v = 0; v < 8; v++) {
+                const size_t at = (((size_t)grp * gridDim.x + blockIdx.x) * 8 + v) * 2;
+                partials[at] = t[v].s;
+                partials[at + 1] = t[v].c;
+            }
+        // the next group's merge reuses the workgroup's LDS
+        __syncthreads();
+    }
+    (void)nq;
+}
+
 template <typename T>
 struct FNormAll {  // |a_n|^2
     const T* a;
@@ -617,6 +695,36 @@ int qa_statevec_calcProbsOfEveryQubit(const void* amps, int prec, int nq, double
     QA_FLUSH_ALL();
     if (!checkArgs(amps, prec, nq, __func__)) return QA_ERR_ARG;
     QA_ARG_CHECK(out, "%s: NULL output", __func__);
+    if (nq >= 10 && nq <= PQALL) {
+        // every qubit in one read (k_probsOfEveryQubit)
+        const uint64_t chunks = 1ull << (nq - 10);
+        const int blocks = (int)std::min<uint64_t>(RED_BLOCKS, (chunks + BLOCK / 64 - 1) / (BLOCK / 64));
+        const int groups = PQALL_VALUES / 8;
+        const size_t resOff = (size_t)groups * blocks * 8 * 2;
+        double* scratch = devScratch(resOff + PQALL_VALUES);
+        double* host = hostScratch(PQALL_VALUES);
+        if (!scratch || !host) {
+            setError("reduction scratch unavailable");
+            return QA_ERR_OOM;
+        }
+        if (prec == QA_FP64)
+            k_probsOfEveryQubit<double><<<blocks, BLOCK, 0, stream()>>>((const double*)amps, nq, chunks, scratch);
+        else
+            k_probsOfEveryQubit<float><<<blocks, BLOCK, 0, stream()>>>((const float*)amps, nq, chunks, scratch);
+        QA_LAUNCH_CHECK();
+        for (int g = 0; g < groups; g++)
+            k_finalize<8><<<1, BLOCK, 0, stream()>>>(scratch + (size_t)g * blocks * 8 * 2, blocks,
+                                                     scratch + resOff + 8 * g);
+        QA_LAUNCH_CHECK();
+        QA_HIP(hipMemcpyAsync(host, scratch + resOff, PQALL_VALUES * sizeof(double), hipMemcpyDeviceToHost,
+                              stream()));
+        QA_HIP(hipStreamSynchronize(stream()));
+        for (int q = 0; q < nq; q++) {
+            out[2 * q] = host[2 * q];
+            out[2 * q + 1] = host[2 * q + 1];
+        }
+        return QA_OK;
+    }
     for (int q0 = 0; q0 < nq; q0 += PQ) {
         double v[2 * PQ];
         int rc;

@@ -251,6 +251,33 @@ def test_qubit_probabilities_cache_follows_the_state(api):
         api.destroyQureg(q)
 
 
+@pytest.mark.parametrize("n,precision", [(10, 2), (11, 1), (16, 2), (20, 2), (23, 1), (26, 2)])
+def test_every_qubit_probabilities_in_one_read(api, n, precision):
+    """qa_statevec_calcProbsOfEveryQubit (one read of the state for up to 32
+    qubits) against the marginals of the amplitudes, summed in FP64 with
+    numpy: both outcomes of every qubit, within 1e-13 (complex128) and 1e-7
+    (complex64, whose amplitudes are rounded)."""
+    import ctypes as C
+    say(f"[gpu] every-qubit probabilities n={n} precision={precision}")
+    q = api.createCustomPrecisionQureg(n, 0, precision)
+    try:
+        api.set_amps(q, qc.random_state(n, 100 + n))
+        amps = api.get_amps(q).astype(np.complex128)
+        out = np.zeros(2 * n)
+        rc = api.gpu().qa_statevec_calcProbsOfEveryQubit(C.c_void_p(q.gpuAmps), precision, n,
+                                                          out.ctypes.data_as(C.c_void_p))
+        assert rc == 0
+        p = np.abs(amps) ** 2
+        idx = np.arange(1 << n)
+        tol = 1e-13 if precision == 2 else 1e-7
+        for t in range(n):
+            one = ((idx >> t) & 1).astype(bool)
+            assert abs(out[2 * t] - p[~one].sum()) <= tol, t
+            assert abs(out[2 * t + 1] - p[one].sum()) <= tol, t
+    finally:
+        api.destroyQureg(q)
+
+
 def test_qureg_size_limits(api):
     """Sizes at and past the limits are input errors, never a crash: no
     qubits, more index bits than one GPU addresses, and a state larger than

@@ -74,6 +74,38 @@ def test_tuned_exact_run_matches_unfused(api):
         api.destroyQureg(q)
 
 
+def test_tune_at_28_qubits(api):
+    """The tuner's scratch copy of a 28-qubit complex128 state is 4 GiB (a
+    size that overflows 32-bit arithmetic): tuned in exact mode, the state
+    must be the unfused one -- every qubit's probabilities equal."""
+    n = 28
+    ops = []
+    rng = qc.SplitMix64(5)
+    for i in range(24):
+        t = rng.randint(n)
+        ops.append(("U1", t, qc.haar_unitary(rng, 2)))
+        if i % 3 == 0:
+            k = 4 + rng.randint(6)
+            ops.append(("PG", qc.random_pauli_chars(rng, k, allow_identity=False), rng.sample(n, k), rng.angle()))
+    c = qc.Circuit(n, False, ops, ("plus",), "tune28")
+    say(f"tune case: {n} qubits, {len(ops)} gates")
+    probs = []
+    for tuned in (False, True):
+        api.set_fusion(tuned, 13, 3, 4096)
+        api.set_fusion_exact(True)
+        api.set_fusion_jit(0)
+        q = api.createQureg(n)
+        try:
+            q, _ = qc.run(c, q)
+            if tuned:
+                res = api.tune_fusion(q, retune=True)
+                say(f"  chose {res['best']}")
+            probs.append([api.calcProbOfQubitOutcome(q, t, 0) for t in range(n)])
+        finally:
+            api.destroyQureg(q)
+    assert probs[0] == probs[1]
+
+
 def test_tune_needs_queued_gates(api):
     api.set_fusion(True)
     q = api.createQureg(12)

@@ -1,8 +1,15 @@
 // Pass kernels generated at run time: source generation from a pass
-// program's shape, hiprtc compiles on background threads, a per-shape
-// cache of loaded kernels, and their launches.  See qa_jit.hpp.
-#include <hip/hiprtc.h>
+// program's shape, compiles in qa_jitc child processes driven by background
+// threads, a per-shape cache of loaded kernels, and their launches.  See
+// qa_jit.hpp.
+#include <dlfcn.h>
+#include <signal.h>
+#include <spawn.h>
+#include <sys/socket.h>
+#include <sys/wait.h>
+#include <unistd.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstdio>
@@ -18,10 +25,7 @@
 #include "qa_common.hpp"
 #include "qa_jit.hpp"
 
-// the device headers, embedded by the build (quest_amd/build.py)
-extern "C" const int qa_jit_num_headers;
-extern "C" const char* const qa_jit_header_names[];
-extern "C" const char* const qa_jit_header_sources[];
+extern char** environ;
 
 namespace qa {
 namespace jit {
@@ -57,22 +61,21 @@ int g_inFlight = 0;
 std::vector<std::thread> g_workers;
 bool g_stop = false;
 bool g_exiting = false;  // set at exit: no compile threads start again
-int g_exitRegs = 0;      // exit handlers registered by the compile threads
+bool g_exitRegistered = false;
 Settings g_settings;
 Stats g_stats;
 std::string g_lastError;
 
-// Stops and joins the compile threads.  At process exit the handlers
-// registered with atexit and the static destructors run in the reverse
-// order of their registration, and hiprtc loads the compiler library (with
-// its static state) only at its first compile, after this library's own
-// statics: a handler registered at load would run after the compiler's
-// state is gone, while a compile thread may still be inside it.  So
-// ensureWorkers() compiles one empty program on the calling thread before
-// it starts the threads and then registers this handler, and the compile
-// threads register it again after each of the first 16 compiles (for state
-// the compiler creates lazily).  It is idempotent; a compile in flight at
-// exit is waited for.
+// Stops and joins the compile threads; at exit (exiting) also kills the
+// compiler processes in flight, whose results would not be used.  Compiles
+// run in child processes (qa_jitc), so nothing of the compiler lives in
+// this process; a compile thread only waits on its child, and a thread
+// whose child was killed ends at once.  Registered with atexit when the
+// threads start (the HIP runtime the threads load modules with is
+// initialised before, so it is torn down after this runs) and run by
+// qa_finalize (stopWorkers(false)).
+std::vector<pid_t> g_children;  // compiler processes in flight (under g_mu)
+
 void stopWorkers(bool exiting) {
     std::vector<std::thread> workers;
     {
@@ -81,6 +84,8 @@ void stopWorkers(bool exiting) {
         g_exiting = g_exiting || exiting;
         g_jobs.clear();
         workers.swap(g_workers);
+        if (exiting)
+            for (pid_t c : g_children) (void)kill(c, SIGKILL);
     }
     g_work.notify_all();
     for (auto& w : workers)
@@ -96,17 +101,98 @@ struct WorkerJoin {
 } g_workerJoin;
 
 void stopAtExit() { stopWorkers(true); }
-void registerExitStop() { (void)std::atexit(stopAtExit); }
 
-// loads and initialises hiprtc's compiler on the calling thread
-void warmCompiler() {
-    hiprtcProgram p;
-    if (hiprtcCreateProgram(&p, "extern \"C\" __global__ void qa_jit_warm() {}", "qa_jit_warm.hip", 0, nullptr,
-                            nullptr) != HIPRTC_SUCCESS)
-        return;
-    const char* opts[] = {"--offload-arch=gfx950", "-O0"};
-    (void)hiprtcCompileProgram(p, 2, opts);
-    hiprtcDestroyProgram(&p);
+// the compiler program: QA_JITC, else qa_jitc beside this library
+std::string compilerPath() {
+    if (const char* p = std::getenv("QA_JITC")) return p;
+    Dl_info info;
+    if (dladdr((const void*)&compilerPath, &info) && info.dli_fname) {
+        std::string lib = info.dli_fname;
+        const size_t slash = lib.rfind('/');
+        return (slash == std::string::npos ? std::string(".") : lib.substr(0, slash)) + "/qa_jitc";
+    }
+    return "qa_jitc";
+}
+
+bool sendAll(int fd, const void* p, size_t n) {
+    const char* c = (const char*)p;
+    while (n) {
+        const ssize_t w = send(fd, c, n, MSG_NOSIGNAL);
+        if (w <= 0) return false;
+        c += w;
+        n -= (size_t)w;
+    }
+    return true;
+}
+bool recvAll(int fd, void* p, size_t n) {
+    char* c = (char*)p;
+    while (n) {
+        const ssize_t r = recv(fd, c, n, 0);
+        if (r <= 0) return false;
+        c += r;
+        n -= (size_t)r;
+    }
+    return true;
+}
+
+// compiles one source in a qa_jitc child process: the code object, or
+// false with `err` set
+bool runCompiler(const std::string& src, std::vector<char>& code, std::string& err) {
+    static const std::string path = compilerPath();
+    int sv[2];
+    if (socketpair(AF_UNIX, SOCK_STREAM | SOCK_CLOEXEC, 0, sv) != 0) {
+        err = "socketpair failed";
+        return false;
+    }
+    posix_spawn_file_actions_t fa;
+    posix_spawn_file_actions_init(&fa);
+    posix_spawn_file_actions_adddup2(&fa, sv[1], 0);
+    posix_spawn_file_actions_adddup2(&fa, sv[1], 1);
+    // the child gets no other descriptor of this process (device files included)
+    posix_spawn_file_actions_addclosefrom_np(&fa, 3);
+    char* argv[] = {const_cast<char*>(path.c_str()), nullptr};
+    pid_t pid = -1;
+    int rc;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        rc = g_exiting ? -1 : posix_spawn(&pid, path.c_str(), &fa, nullptr, argv, environ);
+        if (rc == 0) g_children.push_back(pid);
+    }
+    posix_spawn_file_actions_destroy(&fa);
+    close(sv[1]);
+    if (rc != 0) {
+        close(sv[0]);
+        err = "cannot start the kernel compiler " + path;
+        return false;
+    }
+    const uint64_t n = src.size();
+    uint32_t status = 99;
+    uint64_t len = 0;
+    bool ok = sendAll(sv[0], &n, 8) && sendAll(sv[0], src.data(), src.size());
+    shutdown(sv[0], SHUT_WR);
+    ok = ok && recvAll(sv[0], &status, 4) && recvAll(sv[0], &len, 8) && len <= (256u << 20);
+    std::vector<char> payload;
+    if (ok) {
+        payload.resize(len);
+        ok = recvAll(sv[0], payload.data(), len);
+    }
+    close(sv[0]);
+    int ws = 0;
+    (void)waitpid(pid, &ws, 0);
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_children.erase(std::remove(g_children.begin(), g_children.end(), pid), g_children.end());
+    }
+    if (!ok) {
+        err = "the kernel compiler " + path + " ended without a result";
+        return false;
+    }
+    if (status != 0) {
+        err.assign(payload.begin(), payload.end());
+        return false;
+    }
+    code.swap(payload);
+    return true;
 }
 
 // ---- shapes ---------------------------------------------------------------
@@ -162,33 +248,15 @@ const char* typeName(int prec) { return prec == QA_FP64 ? "double" : "float"; }
 // ---- compiles -------------------------------------------------------------
 
 bool compileSource(const std::string& src, hipModule_t& mod, hipFunction_t& fn, std::string& err) {
-    hiprtcProgram p;
-    if (hiprtcCreateProgram(&p, src.c_str(), "qa_jit_pass.hip", qa_jit_num_headers, qa_jit_header_sources,
-                            qa_jit_header_names) != HIPRTC_SUCCESS) {
-        err = "hiprtcCreateProgram failed";
-        return false;
+    std::vector<char> code;
+    if (!runCompiler(src, code, err)) return false;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        if (g_exiting) {
+            err = "the process is exiting";
+            return false;
+        }
     }
-    // the options the library's own tile-pass kernels are built with, but the
-    // default scheduler: straight-line programs scheduled for ILP can need
-    // more than the 256 registers of two waves per SIMD and spill
-    std::vector<const char*> opts = {"--offload-arch=gfx950", "-O3", "-std=c++17",
-                                     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-                                     "-mllvm", "-simplifycfg-sink-common=false"};
-    const hiprtcResult rc = hiprtcCompileProgram(p, (int)opts.size(), opts.data());
-    if (rc != HIPRTC_SUCCESS) {
-        size_t n = 0;
-        hiprtcGetProgramLogSize(p, &n);
-        std::string log(n + 1, '\0');
-        if (n) hiprtcGetProgramLog(p, &log[0]);
-        err = std::string("hiprtc: ") + hiprtcGetErrorString(rc) + ": " + log.substr(0, 400);
-        hiprtcDestroyProgram(&p);
-        return false;
-    }
-    size_t n = 0;
-    hiprtcGetCodeSize(p, &n);
-    std::vector<char> code(n);
-    hiprtcGetCode(p, code.data());
-    hiprtcDestroyProgram(&p);
     if (hipModuleLoadData(&mod, code.data()) != hipSuccess) {
         err = "hipModuleLoadData failed";
         return false;
@@ -242,7 +310,6 @@ void worker(int device) {
         hipFunction_t fn = nullptr;
         std::string err;
         const bool ok = compileOne(job, mod, fn, err);
-        bool again = false;
         const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         {
             std::lock_guard<std::mutex> lk(g_mu);
@@ -270,10 +337,7 @@ void worker(int device) {
             }
             g_stats.compileSeconds += dt;
             g_inFlight--;
-            again = g_exitRegs < 16;
-            g_exitRegs += again ? 1 : 0;
         }
-        if (again) registerExitStop();
         g_done.notify_all();
     }
 }
@@ -286,8 +350,10 @@ void ensureWorkers() {
     unsigned hw = std::thread::hardware_concurrency();
     int n = (int)(hw > 2 ? hw - 1 : 1);
     if (n > 16) n = 16;
-    warmCompiler();
-    registerExitStop();
+    if (!g_exitRegistered) {
+        (void)std::atexit(stopAtExit);
+        g_exitRegistered = true;
+    }
     g_stop = false;
     for (int i = 0; i < n; i++) g_workers.emplace_back(worker, dev);
 }

@@ -11,7 +11,8 @@
 // the remaining masks stay in the program the kernel reads, so one kernel
 // serves every pass of the same shape, with the interpreter's arithmetic
 // bit for bit.  Kernels are cached per shape for the life of the library;
-// compiles run on background threads, and a pass whose shape has no kernel
+// compiles run in child processes (qa_jitc, hiprtc) driven by background
+// threads, and a pass whose shape has no kernel
 // yet runs in the interpreter.
 #pragma once
 

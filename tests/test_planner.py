@@ -81,3 +81,52 @@ def test_benchmark_pass_kernels_compile_without_scratch(jit_bench_check):
     out = subprocess.run([str(jit_bench_check), "4", "8", "1000"], capture_output=True, text=True, timeout=900)
     assert out.returncode == 0, out.stdout[-3000:] + out.stderr[-2000:]
     assert "0 with scratch or failed" in out.stdout
+
+
+@pytest.fixture(scope="module")
+def jit_exit_check(tmp_path_factory):
+    if not Path(HIPCC).exists():
+        pytest.skip("hipcc not available")
+    exe = tmp_path_factory.mktemp("jitexit") / "jit_exit_check"
+    cmd = [HIPCC, "-std=c++17", "-O1", f"-I{ROOT / 'quest_amd' / 'csrc' / 'gpu'}", f"-I{ROOT / 'include'}",
+           str(ROOT / "tests" / "jit_exit_check.cpp"), "-o", str(exe), f"-L{GPU_LIB.parent}", "-lquest_amd_gpu",
+           f"-Wl,-rpath,{GPU_LIB.parent}"]
+    subprocess.run(cmd, check=True, capture_output=True, text=True, timeout=300)
+    return exe
+
+
+@pytest.mark.parametrize("delay_ms", [100, 400, 900, 1500, 2500])
+def test_exit_while_pass_kernels_compile(jit_exit_check, delay_ms):
+    """A process that ends while generated pass kernels are being compiled
+    (no qa_finalize) exits with status 0 and at once: the compiles run in
+    qa_jitc child processes, which the exit handler kills.  (With compiles
+    inside the process, such exits crashed or hung when the compiler's
+    state was torn down under the compile threads.)"""
+    import time
+    t = time.perf_counter()
+    out = subprocess.run([str(jit_exit_check), "20", str(delay_ms)], capture_output=True, text=True, timeout=60)
+    wall = time.perf_counter() - t
+    assert out.returncode == 0, (out.returncode, out.stdout, out.stderr[-2000:])
+    assert "returning with compiles queued" in out.stdout
+    assert wall < delay_ms / 1e3 + 10, wall
+
+
+def test_compiler_process_replies():
+    """qa_jitc, the compiler process: a kernel source gets status 0 and a
+    gfx950 code object (an ELF image); a source with an error gets status 1
+    and the compiler's log."""
+    import struct
+    jitc = GPU_LIB.parent / "qa_jitc"
+    assert jitc.exists(), "quest_amd/lib/qa_jitc is not built"
+
+    def ask(src):
+        data = src.encode()
+        out = subprocess.run([str(jitc)], input=struct.pack("<Q", len(data)) + data, capture_output=True,
+                             timeout=120).stdout
+        status, n = struct.unpack("<IQ", out[:12])
+        return status, out[12:12 + n]
+
+    status, code = ask('extern "C" __global__ void qa_jit_pass(double* a) { a[threadIdx.x] *= 2.0; }')
+    assert status == 0 and code[:4] == b"\x7fELF", (status, code[:80])
+    status, log = ask('extern "C" __global__ void qa_jit_pass(double* a) { undeclared_name(a); }')
+    assert status == 1 and b"undeclared_name" in log, log[:400]

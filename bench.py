@@ -585,6 +585,7 @@ def main() -> int:
         # the fusion autotuner on one step's gates (queued whole, then timed
         # under candidate shapes on a scratch copy of the state)
         t = time.perf_counter()
+        os.environ["QA_TUNE_VERBOSE"] = "1"
         Q.set_fusion(True, fusion_cfg[1], fusion_cfg[2], 4096)
         gate_ops = [op for op in circ.ops if not op[0].startswith("OUT_")]
         for f, a in build_step(Q, qc, qc.Circuit(circ.num_qubits, circ.dens, gate_ops, circ.init, circ.name), q):

@@ -97,6 +97,11 @@ bool ensureRing() {
 
 uint64_t denseMask(const GOp& g) {
     switch (g.kind) {
+        case F_DENSEN: {
+            uint64_t m = 0;
+            for (int b = 0; b < g.numTargs; b++) m |= 1ull << g.targs[b];
+            return m;
+        }
         case F_DENSE1: case F_DENSE1V: return 1ull << g.t1;
         case F_DENSE2: case F_SWAP: case F_DEPOL1: case F_DEPH1: return (1ull << g.t1) | (1ull << g.t2);
         case F_PAULI: return g.m1;
@@ -321,7 +326,26 @@ TOp placeholderGroup(int k) {
 
 // a Pauli tensor/gadget with more X/Y qubits than register bits: applied
 // to the tile in LDS
-bool isLdsOp(const GOp& g) { return g.kind == F_PAULI && __builtin_popcountll(g.m1) > RB; }
+bool isLdsOp(const GOp& g) { return (g.kind == F_PAULI && __builtin_popcountll(g.m1) > RB) || g.kind == F_DENSEN; }
+
+// a 3- or 4-qubit dense matrix: applied to the tile in LDS, each thread
+// taking the groups of 2^n amplitudes that differ in its target slots
+// (body words: the target slots in matrix-bit order, n, conj, the matrix)
+TOp ldsDense(const GOp& g, const Layout& L) {
+    TOp t;
+    std::memset(&t, 0, sizeof(t));
+    t.h.kind = F_DENSE_LDS;
+    uint32_t* w = reinterpret_cast<uint32_t*>(t.b);
+    for (int b = 0; b < 4; b++) w[b] = b < g.numTargs ? (uint32_t)L.slotOf[g.targs[b]] : 0u;
+    w[4] = (uint32_t)g.numTargs;
+    w[5] = (uint32_t)g.conj;
+    const uint64_t p = (uint64_t)(uintptr_t)g.mat;
+    w[6] = (uint32_t)p;
+    w[7] = (uint32_t)(p >> 32);
+    return t;
+}
+
+TOp ldsOp(const GOp& g, const Layout& L);
 
 TOp ldsPauli(const GOp& g, const Layout& L) {
     Layout T = L;  // every local bit counted in the tile part
@@ -334,6 +358,8 @@ TOp ldsPauli(const GOp& g, const Layout& L) {
     t.b[6] = tile;  // X/Y bits (all local)
     return t;
 }
+
+TOp ldsOp(const GOp& g, const Layout& L) { return g.kind == F_DENSEN ? ldsDense(g, L) : ldsPauli(g, L); }
 
 // appends the PHASES terms of a diagonal gate (DIAG1, or a PARITY on one
 // qubit) under layout L; false, appending nothing, for any other gate or
@@ -384,7 +410,7 @@ void emitGroups(const GOp* ops, size_t from, size_t to, Layout& L, int k, int lo
     size_t i = from;
     while (i < to) {
         if (isLdsOp(ops[i])) {
-            prog.push_back(ldsPauli(ops[i], L));
+            prog.push_back(ldsOp(ops[i], L));
             if (st.numGroups == 0 && !st.anyItem) st.firstIsLds = true;
             st.lastIsLds = true;
             st.anyItem = true;
@@ -890,6 +916,7 @@ double opFlops(const GOp& g, int n) {
         case F_PAULI: perAmp = 14; break;    // multiply + multiply-add per amplitude
         case F_DEPOL1: perAmp = 4; break;    // 16 per 4 amplitudes
         case F_DEPH1: perAmp = 1; break;     // 4 per 4 amplitudes
+        case F_DENSEN: perAmp = 8.0 * (1 << g.numTargs); break;  // 2^n complex MACs per amplitude
         default: perAmp = 0; break;
     }
     return perAmp * std::ldexp(1.0, n - __builtin_popcountll(g.cMask));
@@ -917,7 +944,7 @@ void setLdsOffsets(TOp* prog, size_t num, TileArgs& a, int prec) {
             offsets(cur, w + RA);
             std::memcpy(cur, groupSlots(t), sizeof(cur));
             offsets(cur, w);
-        } else if (t.h.kind == F_PAULI_LDS || t.h.kind == F_BLOCK) {
+        } else if (t.h.kind == F_PAULI_LDS || t.h.kind == F_BLOCK || t.h.kind == F_DENSE_LDS) {
             offsets(cur, w + RA);
         }
     }
@@ -1080,8 +1107,15 @@ int launchPass(void* amps, const Queue& qu, const std::vector<GOp>& ops, size_t 
         return QA_ERR_HIP;
     }
     // the pass's generated kernel when its shape has one (checked for static
-    // LDS and scratch when it was loaded), else the interpreter
-    const void* jfn = jit::kernelFor(qu.prec, nt, pf, a, prog.data(), num);
+    // LDS and scratch when it was loaded), else the interpreter; a pass with
+    // a 3- or 4-qubit matrix (DENSE_LDS) has only a generated kernel
+    bool needsJit = false;
+    for (size_t i = 0; i < num; i++) needsJit = needsJit || prog[i].h.kind == F_DENSE_LDS;
+    const void* jfn = jit::kernelFor(qu.prec, nt, pf, a, prog.data(), num, needsJit);
+    if (needsJit && !jfn) {
+        setError("fusion: no generated kernel for a pass holding a 3- or 4-qubit matrix: %s", jit::lastError());
+        return QA_ERR_HIP;
+    }
     const unsigned grid = (unsigned)std::min<uint64_t>(
         a.numTiles, jfn ? jit::residentBlocks(jfn, block, lds) : residentBlocks(fn, block, lds));
     a.tileStep = 0;
@@ -1308,6 +1342,48 @@ int enqueue(void* amps, int prec, int numQubits, const GOp& op, bool& accepted) 
     return QA_OK;
 }
 
+// the matrices of queued DENSEN gates: a device ring, filled in order; a
+// wrap first applies every queue (their gates point into the ring) and
+// waits for the stream
+constexpr size_t MAT_RING_DOUBLES = 1u << 18;  // 2 MiB: 512 matrices of 16x16
+double* g_matRing = nullptr;
+size_t g_matPos = 0;
+
+const double* stageMatrix(const void* mat, size_t count) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (count > MAT_RING_DOUBLES) {
+        setError("fusion: a matrix of %zu doubles exceeds the matrix ring", count);
+        return nullptr;
+    }
+    if (!g_matRing) {
+        if (hipMalloc(&g_matRing, MAT_RING_DOUBLES * sizeof(double)) != hipSuccess) {
+            (void)hipGetLastError();
+            g_matRing = nullptr;
+            setError("fusion: matrix ring allocation failed");
+            return nullptr;
+        }
+        g_matPos = 0;
+    }
+    if (g_matPos + count > MAT_RING_DOUBLES) {
+        for (auto& kv : g_queues)
+            if (flushQueue(kv.first, kv.second) != QA_OK) return nullptr;
+        if (hipStreamSynchronize(stream()) != hipSuccess) {
+            setError("fusion: stream synchronisation failed");
+            return nullptr;
+        }
+        g_matPos = 0;
+    }
+    double* dst = g_matRing + g_matPos;
+    if (hipMemcpyAsync(dst, mat, count * sizeof(double), hipMemcpyDefault, stream()) != hipSuccess) {
+        (void)hipGetLastError();
+        setError("fusion: matrix staging copy failed");
+        return nullptr;
+    }
+    // 64-byte aligned entries
+    g_matPos += (count + 7) & ~size_t(7);
+    return dst;
+}
+
 int flush(void* amps) {
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_queues.find(amps);
@@ -1358,6 +1434,9 @@ void shutdown() {
     g_blockHost = nullptr;
     g_blockDev = nullptr;
     g_blockPos = 0;
+    if (g_matRing) (void)hipFree(g_matRing);
+    g_matRing = nullptr;
+    g_matPos = 0;
 }
 
 // lets the tile-pass and span-pass kernels use the LDS of a 2^tileQubits

@@ -18,7 +18,20 @@ struct GOp {
     double f[32] = {0};              // matrix / factors (re, im interleaved); DENSE1V: one 2x2
                                      // matrix per value w of the (at most 2) variant qubits, at
                                      // f[8w], bit b of w = the b-th lowest variant qubit
+    // DENSEN: targets (matrix index bit b <-> targs[b]), and the row-major
+    // 2^n x 2^n matrix {re, im} in device memory staged by stageMatrix
+    // (conj: its imaginary parts negated where read)
+    int targs[4] = {-1, -1, -1, -1};
+    int numTargs = 0;
+    int conj = 0;
+    const double* mat = nullptr;
 };
+
+// copies a DENSEN matrix (count doubles, device or host memory) into the
+// library's matrix ring, in stream order; the copy lives until the queue
+// that holds the gate is applied (a ring wrap applies every queue first).
+// nullptr on failure (error set).
+const double* stageMatrix(const void* mat, size_t count);
 
 struct Config {
     bool enabled = false;

@@ -16,6 +16,7 @@
 
 #include "qa_common.hpp"
 #include "qa_fused.hpp"
+#include "qa_jit.hpp"
 
 namespace qa {
 namespace {
@@ -557,6 +558,22 @@ int qa_statevec_anyCtrlAnyTargDenseMatr(void* amps, int prec, int numQubits, con
     InsertSpec ins;
     if (!buildInsert(ins, numQubits, ctrls, ctrlStates, numCtrls, targs, nullptr, numTargs, __func__))
         return QA_ERR_ARG;
+    if ((numTargs == 3 || numTargs == 4) && numCtrls == 0 && jit::settings().mode != 0 && jit::available() &&
+        fuse::enabledFor(amps, prec, numQubits)) {
+        // queued with a copy of its matrix (the caller's may change or go
+        // before the queue is applied); the tile pass applies it in LDS, in
+        // the pass's generated kernel (compiled at first sight when the pass
+        // runs), so with generated kernels off the gate runs unfused
+        const double* staged = fuse::stageMatrix(devMatr, (size_t)2 << (2 * numTargs));
+        if (!staged) return QA_ERR_HIP;
+        fuse::GOp op;
+        op.kind = fuse::F_DENSEN;
+        op.numTargs = numTargs;
+        for (int b = 0; b < numTargs; b++) op.targs[b] = targs[b];
+        op.conj = applyConj ? 1 : 0;
+        op.mat = staged;
+        QA_TRY_FUSE(amps, prec, numQubits, op);
+    }
     QA_FLUSH_ALL();
     TargSpec ts;
     ts.num = numTargs;

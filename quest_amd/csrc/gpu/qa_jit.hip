@@ -114,6 +114,15 @@ std::string compilerPath() {
     return "qa_jitc";
 }
 
+}  // namespace
+
+bool available() {
+    static const bool ok = access(compilerPath().c_str(), X_OK) == 0;
+    return ok;
+}
+
+namespace {
+
 bool sendAll(int fd, const void* p, size_t n) {
     const char* c = (const char*)p;
     while (n) {
@@ -217,6 +226,9 @@ std::string opShape(const TOp& t) {
             break;
         case fuse::F_PAULI_LDS:
             snprintf(buf, sizeof buf, "L;");
+            break;
+        case fuse::F_DENSE_LDS:
+            snprintf(buf, sizeof buf, "D%d,%d,%d,%d,%d;", w[4], w[0], w[1], w[2], w[3]);
             break;
         case fuse::F_BLOCK:
             snprintf(buf, sizeof buf, "B%d,%d,%d,%d,%d;", w[0], w[1], w[2], w[3], w[4]);
@@ -411,6 +423,16 @@ std::string generate(int prec, bool nt, bool pf, const TileArgs& a, const TOp* p
                          i);
                 s += buf;
                 break;
+            case fuse::F_DENSE_LDS:
+                snprintf(buf, sizeof buf,
+                         "        { const TOp* op_ = ops + %zu; u32x16 ch, cb; fetchOp(op_, ch, cb);\n"
+                         "          const uint32_t dslots[4] = {%du, %du, %du, %du};\n"
+                         "          stepDenseLds<T, NT, %d>(st, op_, dslots,\n"
+                         "              reinterpret_cast<const double*>(cb[6] | ((uint64_t)cb[7] << 32)), (int)cb[5], a.k,\n"
+                         "              r, tile, tid); }\n",
+                         i, w[0], w[1], w[2], w[3], w[4]);
+                s += buf;
+                break;
             case fuse::F_BLOCK:
                 snprintf(buf, sizeof buf,
                          "        { const TOp* op_ = ops + %zu; u32x16 ch, cb; fetchOp(op_, ch, cb);\n"
@@ -493,8 +515,9 @@ void setSettings(const Settings& s) {
     g_settings = s;
 }
 
-const void* kernelFor(int prec, bool nt, bool pf, const TileArgs& a, const TOp* prog, size_t num) {
+const void* kernelFor(int prec, bool nt, bool pf, const TileArgs& a, const TOp* prog, size_t num, bool force) {
     Settings s = settings();
+    if (force) s.mode = 2;
     if (s.mode == 0) return nullptr;
     const std::string key = shapeKey(prec, nt, pf, a, prog, num);
     std::unique_lock<std::mutex> lk(g_mu);

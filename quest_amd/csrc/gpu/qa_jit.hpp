@@ -31,12 +31,17 @@ struct Settings {
 
 // the settings (qa_setFusionJit)
 Settings settings();
+// whether the compiler program (qa_jitc) is there to run
+bool available();
 void setSettings(const Settings& s);
 
 // The kernel for this pass, or nullptr when the interpreter runs it (JIT
 // off, the shape not compiled yet, or its compile failed).  Counts the
-// sighting, queues a compile per the settings, and in mode 2 waits for it.
-const void* kernelFor(int prec, bool nt, bool pf, const fuse::TileArgs& a, const fuse::TOp* prog, size_t num);
+// sighting, queues a compile per the settings, and in mode 2 waits for it;
+// force: compile now and wait whatever the settings (a pass the interpreter
+// cannot run).
+const void* kernelFor(int prec, bool nt, bool pf, const fuse::TileArgs& a, const fuse::TOp* prog, size_t num,
+                      bool force = false);
 
 // launches a generated kernel (returns a HIP error code)
 int launch(const void* kernel, unsigned grid, unsigned block, size_t lds, void* amps, const fuse::TileArgs& a,

@@ -32,7 +32,7 @@ namespace fuse {
 
 constexpr int RB = 4;        // register slots per group
 constexpr int RA = 1 << RB;  // amplitudes per thread
-enum { F_GROUP = 100, F_BLOCK = 101, F_DENSE1N = 102, F_PAULI_LDS = 103, F_PHASES = 104 };
+enum { F_GROUP = 100, F_BLOCK = 101, F_DENSE1N = 102, F_PAULI_LDS = 103, F_PHASES = 104, F_DENSE_LDS = 105 };
 constexpr int BQ = 5;  // qubits of an MFMA block (32 x 32 complex)
 
 // the first 64 bytes of an op, fetched with one scalar load
@@ -917,6 +917,72 @@ __device__ __forceinline__ void stepPauliLds(TileState& st, const TileArgs& a, c
     st.inRegs = false;
 }
 
+// a DENSE_LDS op: a dense 2^N x 2^N matrix (N = 3 or 4, row-major {re, im}
+// in device memory, imaginary parts negated when conj) on the tile slots
+// slots[0..N) (matrix bit b <-> slots[b]), applied to the tile in LDS.
+// Generated kernels only: in the interpreter its registers would push the
+// op loop into scratch memory, so the host runs every pass holding one
+// through its generated kernel.
+// Each thread takes the groups of 2^N amplitudes that share the other tile
+// bits (one group for N = 4, two for N = 3) into its registers and writes
+// each row's sum back as it is formed, with the arithmetic of the
+// single-gate kernel (k_fewTargDense: terms accumulated from zero in
+// column order), so results are bit-identical to it.
+template <typename T, int NT, int N>
+__device__ __forceinline__ void stepDenseLds(TileState& st, const TOp* __restrict__ op, const uint32_t* slots,
+                                             const double* __restrict__ mat, int conj, int k, ct<T>* r,
+                                             ct<T>* tile, uint32_t tid) {
+    constexpr int D = 1 << N;
+    if (st.inRegs) RegIO<T, NT, 0>::ldsStore(r, tile, opaqueV(st.sj), storeOffsets(op));
+    __syncthreads();
+    uint32_t tmask = 0;
+#pragma unroll
+    for (int b = 0; b < N; b++) tmask |= 1u << slots[b];
+    const double sgn = conj ? -1.0 : 1.0;
+    // the matrix read by vector loads (a per-lane offset the compiler cannot
+    // see is zero): as scalar loads its 2^(2N) elements would crowd the
+    // scalar registers of the whole op loop
+    const double2* __restrict__ mrow = reinterpret_cast<const double2*>(mat) + opaqueV(0u);
+    const uint32_t groups = 1u << (k - N);
+    for (uint32_t g = tid; g < groups; g += blockDim.x) {
+        // g with zeros inserted at the target slots
+        uint32_t j0 = 0;
+        for (int bit = 0, src = 0; bit < k; bit++) {
+            if ((tmask >> bit) & 1u) continue;
+            j0 |= ((g >> src) & 1u) << bit;
+            src++;
+        }
+        uint32_t sb[N];
+#pragma unroll
+        for (int b = 0; b < N; b++) sb[b] = 1u << slots[b];
+#pragma unroll
+        for (int v = 0; v < D; v++) {
+            uint32_t o = j0;
+#pragma unroll
+            for (int b = 0; b < N; b++)
+                if ((v >> b) & 1) o |= sb[b];
+            r[v] = ldsGet(tile, swz<T>(o));
+        }
+#pragma unroll 1
+        for (int row = 0; row < D; row++) {
+            ct<T> acc = {0, 0};
+#pragma unroll
+            for (int c = 0; c < D; c++) {
+                const double2 m = mrow[row * D + c];
+                const ct<T> e = {(T)m.x, (T)(sgn * m.y)};
+                acc = cfmaT(e, r[c], acc);
+            }
+            uint32_t o = j0;
+#pragma unroll
+            for (int b = 0; b < N; b++)
+                if ((row >> b) & 1) o |= sb[b];
+            ldsPut(tile, swz<T>(o), acc);
+        }
+    }
+    __syncthreads();
+    st.inRegs = false;
+}
+
 // a BLOCK op (complex64 only): bslots = its 5 tile slots, A = its packed matrix
 template <typename T, int NT>
 __device__ __forceinline__ void stepBlock(TileState& st, const TOp* __restrict__ op, const int* bslots,
@@ -996,6 +1062,7 @@ __device__ __forceinline__ void tileBody(T* __restrict__ amps, const TileArgs& a
         } else if (op.h.kind == F_BLOCK) {
             const int bslots[BQ] = {(int)cb[0], (int)cb[1], (int)cb[2], (int)cb[3], (int)cb[4]};
             stepBlock<T, NT>(st, ops + o, bslots, a.blocks + (cb[6] | ((uint64_t)cb[7] << 32)), a.k, r, tile, tid);
+
         } else if (op.h.kind == F_GROUP) {
             int slots[RB];
 #pragma unroll

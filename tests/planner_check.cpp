@@ -14,6 +14,7 @@
 #include <complex>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <random>
 #include <vector>
 
@@ -61,6 +62,29 @@ static void apply(const GOp& g, std::vector<cx>& a) {
                 const cx x0 = a[i], x1 = a[i | t];
                 a[i] = cx(m[0], m[1]) * x0 + cx(m[2], m[3]) * x1;
                 a[i | t] = cx(m[4], m[5]) * x0 + cx(m[6], m[7]) * x1;
+            }
+            break;
+        }
+        case F_DENSEN: {
+            const int D = 1 << g.numTargs;
+            uint64_t tm = 0;
+            for (int b = 0; b < g.numTargs; b++) tm |= 1ull << g.targs[b];
+            std::vector<cx> x(D), y(D);
+            for (uint64_t i = 0; i < N; i++) {
+                if (i & tm) continue;
+                auto at = [&](int v) {
+                    uint64_t j = i;
+                    for (int b = 0; b < g.numTargs; b++)
+                        if ((v >> b) & 1) j |= 1ull << g.targs[b];
+                    return j;
+                };
+                for (int c = 0; c < D; c++) x[c] = a[at(c)];
+                for (int rr = 0; rr < D; rr++) {
+                    y[rr] = 0;
+                    for (int c = 0; c < D; c++)
+                        y[rr] += cx(g.mat[2 * (rr * D + c)], (g.conj ? -1 : 1) * g.mat[2 * (rr * D + c) + 1]) * x[c];
+                }
+                for (int rr = 0; rr < D; rr++) a[at(rr)] = y[rr];
             }
             break;
         }
@@ -165,6 +189,29 @@ static GOp randomGate(Rng& r, int n, bool dens, int hot) {
         g.cMask |= 1ull << c;
         if (r.below(4)) g.cVal |= 1ull << c;
     };
+    if (n >= 5 && r.below(12) == 0) {
+        // an uncontrolled 3- or 4-qubit dense matrix (its matrix kept in a
+        // pool that outlives the queue)
+        static std::vector<std::vector<double>> pool;
+        g.kind = F_DENSEN;
+        g.numTargs = 3 + r.below(2);
+        for (int b = 0; b < g.numTargs; b++) {
+            int q;
+            bool dup;
+            do {
+                q = qubit();
+                dup = false;
+                for (int c = 0; c < b; c++) dup = dup || g.targs[c] == q;
+            } while (dup);
+            g.targs[b] = q;
+        }
+        const int D = 1 << g.numTargs;
+        pool.emplace_back(2 * D * D);
+        for (double& x : pool.back()) x = r.unit() / D;
+        g.mat = pool.back().data();
+        g.conj = r.below(2);
+        return g;
+    }
     const int kind = r.below(dens ? 10 : 8);
     switch (kind) {
         case 0: case 1:
@@ -296,7 +343,11 @@ struct JitCheck {
                     std::fclose(fp);
                 }
             }
-            if (compiled >= maxCompiles) continue;
+            bool dense = false;
+            for (const TOp& t : prog) dense = dense || t.h.kind == F_DENSE_LDS;
+            // every pass with a 3-4 qubit matrix compiles (only generated
+            // kernels run them), a few others
+            if (compiled >= maxCompiles && !(dense && denseCompiled < 3)) continue;
             hiprtcProgram p;
             hiprtcCreateProgram(&p, src.c_str(), "qa_jit_pass.hip", qa_jit_num_headers, qa_jit_header_sources,
                                 qa_jit_header_names);
@@ -312,10 +363,53 @@ struct JitCheck {
                             log.substr(0, 2000).c_str());
                 return 1;
             }
+            size_t cn = 0;
+            hiprtcGetCodeSize(p, &cn);
+            std::vector<char> code(cn);
+            hiprtcGetCode(p, code.data());
             hiprtcDestroyProgram(&p);
-            compiled++;
+            if (dense) {
+                const long scratch = privateSegmentSize(code);
+                if (scratch != 0) {
+                    std::printf("case %d: generated kernel with a 3-4 qubit matrix uses %ld bytes of scratch\n", c,
+                                scratch);
+                    return 1;
+                }
+                denseCompiled++;
+            } else {
+                compiled++;
+            }
         }
         return 0;
+    }
+    int denseCompiled = 0;
+
+    // private segment size of the kernel descriptor "qa_jit_pass.kd" in an
+    // amdgcn code object (ELF64), -1 if not found
+    static long privateSegmentSize(const std::vector<char>& elf) {
+        if (elf.size() < 64 || std::memcmp(elf.data(), "\x7f" "ELF", 4) != 0) return -1;
+        auto u16 = [&](size_t o) { uint16_t v; std::memcpy(&v, elf.data() + o, 2); return v; };
+        auto u32 = [&](size_t o) { uint32_t v; std::memcpy(&v, elf.data() + o, 4); return v; };
+        auto u64 = [&](size_t o) { uint64_t v; std::memcpy(&v, elf.data() + o, 8); return v; };
+        const uint64_t shoff = u64(0x28);
+        const int shentsize = u16(0x3a), shnum = u16(0x3c);
+        for (int i = 0; i < shnum; i++) {
+            const size_t sh = shoff + (size_t)i * shentsize;
+            if (u32(sh + 4) != 2) continue;  // SHT_SYMTAB
+            const uint64_t off = u64(sh + 0x18), size = u64(sh + 0x20), entsize = u64(sh + 0x38);
+            const uint32_t strIdx = u32(sh + 0x28);
+            const uint64_t strOff = u64(shoff + (size_t)strIdx * shentsize + 0x18);
+            for (uint64_t q = 0; q + entsize <= size; q += entsize) {
+                const char* name = elf.data() + strOff + u32(off + q);
+                if (std::strcmp(name, "qa_jit_pass.kd") != 0) continue;
+                const uint16_t secIdx = u16(off + q + 6);
+                const uint64_t value = u64(off + q + 8);
+                const size_t sec = shoff + (size_t)secIdx * shentsize;
+                const uint64_t secAddr = u64(sec + 0x10), secOff = u64(sec + 0x18);
+                return (long)u32(secOff + (value - secAddr) + 4);
+            }
+        }
+        return -1;
     }
 };
 
@@ -628,7 +722,8 @@ int main(int argc, char** argv) {
         checked++;
     }
     if (jit.on)
-        std::printf("jit check: %zu shapes, %d generated kernels compiled\n", jit.byKey.size(), jit.compiled);
+        std::printf("jit check: %zu shapes, %d generated kernels compiled (%d with 3-4 qubit matrices)\n",
+                    jit.byKey.size(), jit.compiled, jit.denseCompiled);
     std::printf("planner check: %d cases agree\n", checked);
     return 0;
 }

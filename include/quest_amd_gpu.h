@@ -158,7 +158,8 @@ int qa_setFusionExact(int exact);
 /* Pass kernels generated at run time: a pass program's shape (op kinds,
    register bits and masks, group layouts, which conditions are present) is
    compiled with hiprtc (in a qa_jitc child process, beside the library or
-   at $QA_JITC) into a straight-line kernel, cached per shape, with
+   at $QA_JITC; with $QA_JIT_CACHE_DIR set, code objects are also kept there
+   for later processes) into a straight-line kernel, cached per shape, with
    the interpreter kernel's arithmetic bit for bit.  mode 0: off (every pass
    in the interpreter); 1 (default): compiles on background threads once a
    shape has been planned minSeen times (default 2), passes of a shape

@@ -259,9 +259,54 @@ const char* typeName(int prec) { return prec == QA_FP64 ? "double" : "float"; }
 
 // ---- compiles -------------------------------------------------------------
 
+// QA_JIT_CACHE_DIR: code objects kept across processes, one file per
+// kernel source (named by a 64-bit FNV-1a hash of the source, which holds
+// the whole shape), written to a temporary name and renamed into place
+std::string cachePath(const std::string& src) {
+    const char* dir = std::getenv("QA_JIT_CACHE_DIR");
+    if (!dir || !*dir) return std::string();
+    uint64_t h = 1469598103934665603ull;
+    for (unsigned char c : src) h = (h ^ c) * 1099511628211ull;
+    char name[64];
+    snprintf(name, sizeof name, "/qa_jit_%016llx_%zu.co", (unsigned long long)h, src.size());
+    return std::string(dir) + name;
+}
+
+bool cacheLoad(const std::string& path, std::vector<char>& code) {
+    if (path.empty()) return false;
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) return false;
+    std::fseek(f, 0, SEEK_END);
+    const long n = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    bool ok = n > 4;
+    if (ok) {
+        code.resize((size_t)n);
+        ok = std::fread(code.data(), 1, (size_t)n, f) == (size_t)n && std::memcmp(code.data(), "\x7f" "ELF", 4) == 0;
+    }
+    std::fclose(f);
+    return ok;
+}
+
+void cacheStore(const std::string& path, const std::vector<char>& code) {
+    if (path.empty()) return;
+    char tmp[64];
+    snprintf(tmp, sizeof tmp, ".tmp%d_%p", (int)getpid(), (const void*)&code);
+    const std::string t = path + tmp;
+    FILE* f = std::fopen(t.c_str(), "wb");
+    if (!f) return;
+    const bool ok = std::fwrite(code.data(), 1, code.size(), f) == code.size();
+    std::fclose(f);
+    if (!ok || std::rename(t.c_str(), path.c_str()) != 0) (void)std::remove(t.c_str());
+}
+
 bool compileSource(const std::string& src, hipModule_t& mod, hipFunction_t& fn, std::string& err) {
     std::vector<char> code;
-    if (!runCompiler(src, code, err)) return false;
+    const std::string cached = cachePath(src);
+    if (!cacheLoad(cached, code)) {
+        if (!runCompiler(src, code, err)) return false;
+        cacheStore(cached, code);
+    }
     {
         std::lock_guard<std::mutex> lk(g_mu);
         if (g_exiting) {

@@ -130,3 +130,16 @@ def test_compiler_process_replies():
     assert status == 0 and code[:4] == b"\x7fELF", (status, code[:80])
     status, log = ask('extern "C" __global__ void qa_jit_pass(double* a) { undeclared_name(a); }')
     assert status == 1 and b"undeclared_name" in log, log[:400]
+
+
+def test_generated_kernel_cache_dir(jit_exit_check, tmp_path):
+    """With QA_JIT_CACHE_DIR set, compiled pass kernels are kept there as code
+    objects (ELF images), one per kernel source, for later processes."""
+    import os
+    env = dict(os.environ, QA_JIT_CACHE_DIR=str(tmp_path))
+    out = subprocess.run([str(jit_exit_check), "4", "20000"], capture_output=True, text=True, timeout=120, env=env)
+    assert out.returncode == 0, out.stderr[-2000:]
+    files = sorted(tmp_path.glob("qa_jit_*.co"))
+    assert len(files) >= 4, [f.name for f in tmp_path.iterdir()]
+    assert all(f.read_bytes()[:4] == b"\x7fELF" for f in files)
+    assert not list(tmp_path.glob("*.tmp*"))

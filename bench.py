@@ -647,7 +647,7 @@ def main() -> int:
             f"per pass {(prog_ops.value - span['gates']) / tile_passes:.1f} program ops, "
             f"{layouts.value / tile_passes:.2f} register-layout changes")
         traffic = None
-        tfile = ROOT / "profiles" / f"k_tilePass_{n}q_fp64_traffic.json"
+        tfile = ROOT / "profiles" / f"tile_pass_{n}q_fp64_traffic.json"
         if tfile.exists():
             traffic = json.loads(tfile.read_text()).get("traffic_bytes_per_dispatch")
         roofline = {

@@ -953,6 +953,87 @@ void setLdsOffsets(TOp* prog, size_t num, TileArgs& a, int prec) {
 
 bool spanEligible(const GOp& g) { return (g.kind == F_PAULI || g.kind == F_PARITY) && g.cMask == 0; }
 
+bool gateFitsTile(const GOp& g, int n, int kMax, int low) {
+    const uint64_t d = denseMask(g);
+    const uint64_t lowMask = (1ull << std::min(low, std::min(n, kMax))) - 1ull;
+    return __builtin_popcountll(d | lowMask) <= std::min(n, kMax) && (__builtin_popcountll(d) <= RB || g.kind == F_PAULI);
+}
+
+// Groups a span pass's gadgets (in coset coordinates) into batches whose
+// X/Y coordinates span at most RB dimensions, each run with one LDS sweep:
+// a header record (SPAN_BATCH), a table record (S4) and the batch's
+// gadgets with their X/Y and sign masks over the batch's own register
+// basis (c, r); d stays over all k coordinates.
+void batchSpanOps(std::vector<SpanOp>& prog, int k) {
+    std::vector<SpanOp> in;
+    in.swap(prog);
+    size_t i = 0;
+    while (i < in.size()) {
+        // the batch basis in reduced echelon form (pivot: the highest bit)
+        uint32_t bv[RB];
+        int bp[RB];
+        int nb = 0;
+        auto reduce = [&](uint32_t c) {
+            for (int q = 0; q < nb; q++)
+                if ((c >> bp[q]) & 1u) c ^= bv[q];
+            return c;
+        };
+        auto add = [&](uint32_t c) {
+            const int p = 31 - __builtin_clz(c);
+            for (int q = 0; q < nb; q++)
+                if ((bv[q] >> p) & 1u) bv[q] ^= c;
+            bv[nb] = c;
+            bp[nb++] = p;
+        };
+        size_t j = i;
+        for (; j < in.size(); j++) {
+            if (in[j].kind != F_PAULI) continue;
+            const uint32_t c = reduce(in[j].c);
+            if (!c) continue;
+            if (nb == RB) break;
+            add(c);
+        }
+        // padded with unit vectors of the highest free coordinates (the
+        // lanes of a wave then take the lowest ones)
+        for (int x = k - 1; x >= 0 && nb < RB; x--) {
+            const uint32_t c = reduce(1u << x);
+            if (c) add(c);
+        }
+        uint32_t pivMask = 0;
+        for (int q = 0; q < nb; q++) pivMask |= 1u << bp[q];
+        SpanOp hdr, tab;
+        std::memset(&hdr, 0, sizeof(hdr));
+        std::memset(&tab, 0, sizeof(tab));
+        hdr.kind = SPAN_BATCH;
+        hdr.c = (uint32_t)(j - i);
+        int nd = 0;
+        for (int x = 0; x < k; x++)
+            if (!((pivMask >> x) & 1u)) hdr.dep |= (uint64_t)x << (4 * nd++);
+        hdr.r = (uint32_t)nd;
+        uint32_t* S4 = reinterpret_cast<uint32_t*>(&tab);
+        for (int v = 0; v < RA; v++) {
+            uint32_t t = 0;
+            for (int q = 0; q < RB; q++)
+                if ((v >> q) & 1) t ^= bv[q];
+            S4[v] = t;
+        }
+        prog.push_back(hdr);
+        prog.push_back(tab);
+        for (size_t m = i; m < j; m++) {
+            SpanOp o = in[m];
+            uint32_t c4 = 0, r4 = 0;
+            for (int q = 0; q < RB; q++) {
+                if (o.kind == F_PAULI) c4 |= ((o.c >> bp[q]) & 1u) << q;
+                r4 |= (uint32_t)(__builtin_popcount(bv[q] & o.d) & 1) << q;
+            }
+            o.c = c4;
+            o.r = r4;
+            prog.push_back(o);
+        }
+        i = j;
+    }
+}
+
 size_t spanRun(const GOp* ops, size_t num, int n, int k, int low, SpanArgs& a, std::vector<SpanOp>& prog) {
     prog.clear();
     if (k < 10 || k > 13 || n <= k || low < 0 || low > k - RB) return 0;
@@ -1013,7 +1094,6 @@ size_t spanRun(const GOp* ops, size_t num, int n, int k, int low, SpanArgs& a, s
         if (!((pivMask >> q) & 1ull)) a.repPos[a.numRep++] = q;
     a.repMask = ((n == 64) ? ~0ull : ((1ull << n) - 1ull)) & ~pivMask;
     a.numCosets = 1ull << (n - k);
-    a.numOps = (int)run;
     // a mask in V has coordinate j equal to its bit piv[j]; a sign mask's
     // coordinate j is its parity over b_j
     auto coords = [&](uint64_t m) {
@@ -1033,7 +1113,6 @@ size_t spanRun(const GOp* ops, size_t num, int n, int k, int low, SpanArgs& a, s
         o.kind = g.kind;
         if (g.kind == F_PAULI) {
             o.c = coords(g.m1);
-            o.top = 31u - (uint32_t)__builtin_clz(o.c);
             o.d = parities(g.m2);
             o.z = g.m2;
         } else {
@@ -1043,6 +1122,8 @@ size_t spanRun(const GOp* ops, size_t num, int n, int k, int low, SpanArgs& a, s
         std::memcpy(o.f, g.f, 4 * sizeof(double));
         prog.push_back(o);
     }
+    batchSpanOps(prog, k);
+    a.numOps = (int)prog.size();
     return run;
 }
 
@@ -1170,8 +1251,8 @@ int collectTimes() {
 
 // launches one span pass applying ops [begin, begin + prog.size()) of `ops`
 // as planned by spanRun
-int launchSpan(void* amps, const Queue& qu, const std::vector<GOp>& ops, size_t begin, const SpanArgs& plan,
-               const std::vector<SpanOp>& prog) {
+int launchSpan(void* amps, const Queue& qu, const std::vector<GOp>& ops, size_t begin, size_t numGates,
+               const SpanArgs& plan, const std::vector<SpanOp>& prog) {
     SpanArgs a = plan;
     const size_t bytes = prog.size() * sizeof(SpanOp);
     const size_t slots = (bytes + sizeof(TOp) - 1) / sizeof(TOp);
@@ -1211,11 +1292,11 @@ int launchSpan(void* amps, const Queue& qu, const std::vector<GOp>& ops, size_t 
     QA_LAUNCH_CHECK();
     if (ev) QA_HIP(hipEventRecord(ev->second, stream()));
     g_stats.passes++;
-    g_stats.gates += (int64_t)prog.size();
+    g_stats.gates += (int64_t)numGates;
     g_stats.spanPasses++;
-    g_stats.spanGates += (int64_t)prog.size();
-    g_stats.progOps += (int64_t)prog.size();
-    for (size_t i = begin; i < begin + prog.size(); i++) g_stats.flops += opFlops(ops[i], qu.numQubits);
+    g_stats.spanGates += (int64_t)numGates;
+    g_stats.progOps += (int64_t)numGates;
+    for (size_t i = begin; i < begin + numGates; i++) g_stats.flops += opFlops(ops[i], qu.numQubits);
     return QA_OK;
 }
 
@@ -1238,11 +1319,13 @@ int flushQueue(void* amps, Queue& qu) {
     SpanArgs span;
     std::vector<SpanOp>& spanProg = g_spanProg;
     // a span pass instead of the next tile pass when it takes more gates
-    auto spanFirst = [&](size_t from, size_t tileGates) {
+    // (force: the front gadget fits no tile, so a span pass takes it alone
+    // if need be)
+    auto spanFirst = [&](size_t from, size_t tileGates, bool force) {
         const int ks = std::min(g_cfg.spanQubits, n - 1);
         if (ks < 10) return (size_t)0;
         const size_t r = spanRun(ops.data() + from, ops.size() - from, n, ks, std::min(low, ks - RB), span, spanProg);
-        return (r >= 2 && r > tileGates) ? r : (size_t)0;
+        return ((force && r >= 1) || (r >= 2 && r > tileGates)) ? r : (size_t)0;
     };
     if (g_cfg.exact) {
         size_t i = 0;
@@ -1250,12 +1333,17 @@ int flushQueue(void* amps, Queue& qu) {
             uint64_t L = 0;
             size_t run = takeRun(ops.data() + i, ops.size() - i, n, kMax, low, L);
             if (run == 0) {
+                if (const size_t r = spanFirst(i, 0, true)) {
+                    rc = launchSpan(amps, qu, ops, i, r, span, spanProg);
+                    i += r;
+                    continue;
+                }
                 setError("fusion: a gate needs more local qubits than the tile holds");
                 rc = QA_ERR_ARG;
                 break;
             }
-            if (const size_t r = spanFirst(i, run)) {
-                rc = launchSpan(amps, qu, ops, i, span, spanProg);
+            if (const size_t r = spanFirst(i, run, false)) {
+                rc = launchSpan(amps, qu, ops, i, r, span, spanProg);
                 i += r;
                 continue;
             }
@@ -1269,6 +1357,13 @@ int flushQueue(void* amps, Queue& qu) {
     std::vector<GOp> pass, rest, merged;
     while (!ops.empty() && rc == QA_OK) {
         uint64_t L = 0;
+        if (!gateFitsTile(ops[0], n, kMax, low)) {
+            if (const size_t r = spanFirst(0, 0, true)) {
+                rc = launchSpan(amps, qu, ops, 0, r, span, spanProg);
+                ops.erase(ops.begin(), ops.begin() + (ptrdiff_t)r);
+                continue;
+            }
+        }
         picked.clear();
         pickPass(ops.data(), ops.size(), n, kMax, low, L, picked);
         if (picked.empty()) {
@@ -1276,8 +1371,8 @@ int flushQueue(void* amps, Queue& qu) {
             rc = QA_ERR_ARG;
             break;
         }
-        if (const size_t r = spanFirst(0, picked.size())) {
-            rc = launchSpan(amps, qu, ops, 0, span, spanProg);
+        if (const size_t r = spanFirst(0, picked.size(), false)) {
+            rc = launchSpan(amps, qu, ops, 0, r, span, spanProg);
             ops.erase(ops.begin(), ops.begin() + (ptrdiff_t)r);
             continue;
         }
@@ -1327,14 +1422,14 @@ int enqueue(void* amps, int prec, int numQubits, const GOp& op, bool& accepted) 
     }
     qu.prec = prec;
     qu.numQubits = numQubits;
-    // a gate whose dense part cannot fit any tile runs unfused
-    int kMax = std::min(numQubits, g_cfg.tileQubits);
-    int lowCount = std::min(g_cfg.lowQubits, kMax);
-    uint64_t lowMask = (1ull << lowCount) - 1ull;
-    // (a Pauli tensor or gadget with more than RB X/Y qubits is applied to
-    // the LDS copy of the tile instead of in registers)
-    if (__builtin_popcountll(denseMask(op) | lowMask) > kMax ||
-        (__builtin_popcountll(denseMask(op)) > RB && op.kind != F_PAULI))
+    // a gate whose dense part cannot fit any tile runs unfused (a Pauli
+    // tensor or gadget with more than RB X/Y qubits is applied to the LDS
+    // copy of the tile instead of in registers), unless it is a gadget a
+    // span pass takes whatever its width
+    const int kMax = std::min(numQubits, g_cfg.tileQubits);
+    const int lowCount = std::min(g_cfg.lowQubits, kMax);
+    if (!gateFitsTile(op, numQubits, kMax, lowCount) &&
+        !(spanEligible(op) && std::min(g_cfg.spanQubits, numQubits - 1) >= 10))
         return QA_OK;
     qu.ops.push_back(op);
     accepted = true;
@@ -1492,14 +1587,10 @@ std::string tuneKey(int dev, int prec, int n, const std::vector<GOp>& ops) {
 
 // whether every gate of the batch is taken by a queue of this shape (the
 // queue runs a gate whose dense part does not fit a tile unfused)
-bool batchFits(const std::vector<GOp>& ops, int n, int tile, int low) {
-    const int kMax = std::min(n, tile);
-    const uint64_t lowMask = (1ull << std::min(low, kMax)) - 1ull;
-    for (const GOp& g : ops) {
-        const uint64_t d = denseMask(g);
-        if (__builtin_popcountll(d | lowMask) > kMax || (__builtin_popcountll(d) > RB && g.kind != F_PAULI))
-            return false;
-    }
+bool batchFits(const std::vector<GOp>& ops, int n, int tile, int low, int spanQubits) {
+    const bool spans = std::min(spanQubits, n - 1) >= 10;
+    for (const GOp& g : ops)
+        if (!gateFitsTile(g, n, tile, low) && !(spans && spanEligible(g))) return false;
     return true;
 }
 
@@ -1578,7 +1669,7 @@ extern "C" int qa_tuneFusion(void* amps, int flags, qa_fusionTrial* trials, int 
         auto f = seen.find(k);
         if (f != seen.end()) return f->second;
         double ms = -1;
-        if (batchFits(batch, n, c.tile, c.low) && allowTileLds(c.tile) == QA_OK) {
+        if (batchFits(batch, n, c.tile, c.low, c.span) && allowTileLds(c.tile) == QA_OK) {
             apply(c);
             for (int run = 0; run < 2 && rc == QA_OK; run++) {
                 Queue q;

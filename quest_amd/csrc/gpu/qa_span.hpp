@@ -14,9 +14,19 @@
 // X/Y mask m and Y/Z mask z is the pair update over (s, s ^ c), c = the
 // coordinates of m, with the sign of an amplitude the parity of
 // (s & d) ^ (rep & z), d_j = parity(b_j & z); a phase gadget is the same
-// parity rule on its Z mask.  Each pair gets the arithmetic of the
-// single-gate kernels (cmat2T, cmulT), so a span pass is bit-identical to
-// the gates one by one.
+// parity rule on its Z mask.
+//
+// In LDS the gadgets go in batches whose X/Y coordinates span at most 4
+// dimensions (a register basis B_0..B_3 in reduced echelon form over the
+// coordinates): each thread takes one coset of span(B) -- its 16
+// coordinates s0 ^ S4[v], s0 its thread id deposited over the coordinates
+// that are not pivots of B -- into registers, applies the batch there with
+// the tile passes' register ops (regPauli, regParity: the pair (v, v ^ c4),
+// c4 = the gadget's X/Y coordinates over B, the sign parity(v & r4) ^
+// parity(s0 & d) ^ parity(rep & z)) and puts it back: one LDS sweep and
+// barrier per batch instead of per gadget.  Each pair gets the arithmetic
+// of the single-gate kernels (cmat2T, cmulT), so a span pass is
+// bit-identical to the gates one by one.
 //
 // The first basis vectors are the unit vectors of the lowest qubits, so the
 // lanes of a wave read rows of 2^low contiguous amplitudes.  A tile pass
@@ -30,14 +40,20 @@
 namespace qa {
 namespace fuse {
 
-// one gadget in coset coordinates
+// the records of a span pass: per batch, a header (SPAN_BATCH), a table
+// (the 16 coordinates of the batch's register span, as 16 uint32 in the
+// record's 64 bytes) and the batch's gadgets
+constexpr int SPAN_BATCH = 200;
+
 struct SpanOp {
-    int kind;       // F_PAULI or F_PARITY
-    uint32_t c;     // PAULI: the X/Y mask's coordinates
-    uint32_t top;   // PAULI: the highest bit of c (pairs are enumerated with it clear)
-    uint32_t d;     // the Y/Z (PARITY: Z) mask's parity over the basis vectors
+    int kind;       // F_PAULI, F_PARITY, or SPAN_BATCH
+    uint32_t c;     // PAULI: the X/Y mask over the batch's register basis (1..15);
+                    // SPAN_BATCH: the number of gadget records after the table
+    uint32_t r;     // the sign (PAULI: Y/Z, PARITY: Z) mask's parity over the batch's
+                    // register basis; SPAN_BATCH: the number of non-pivot coordinates
+    uint32_t d;     // the sign mask's parity over all k coordinates
     uint64_t z;     // the same mask over the global qubits (the coset's own parity)
-    uint64_t pad;
+    uint64_t dep;   // SPAN_BATCH: the non-pivot coordinates, 4 bits each, ascending
     double f[4];    // PAULI: ampFac, pairFac; PARITY: the factors for even and odd parity
 };
 static_assert(sizeof(SpanOp) == 64, "span ops are 64 bytes");
@@ -74,7 +90,6 @@ __device__ __forceinline__ void spanPassLoop(T* __restrict__ amps, const SpanArg
     uint64_t own = 0;
     for (int j = 0; j < lowK; j++)
         if ((tid >> j) & 1u) own ^= a.basis[j];
-    const uint32_t half = 1u << (a.k - 1);
     uint64_t w = blockIdx.x;
     if (w >= a.numCosets) return;
     uint64_t rep = deposit(w, a.repPos, a.numRep);
@@ -86,34 +101,41 @@ __device__ __forceinline__ void spanPassLoop(T* __restrict__ amps, const SpanArg
         const uint64_t repN = ((rep | ~a.repMask) + a.repStep) & a.repMask;
         if (w + gridDim.x < a.numCosets) spanLoad<T, NT>(rn, amps, repN ^ own, a);
         __syncthreads();
-        for (int o = 0; o < a.numOps; o++) {
-            const SpanOp& op = ops[o];
-            const int rp = parity64(rep & op.z);
-            if (op.kind == F_PARITY) {
-                const ct<T> f0 = cf<T>(op.f, 0), f1 = cf<T>(op.f, 1);
+        for (int o = 0; o < a.numOps;) {
+            // a batch: thread t takes the 16 coordinates s0 ^ S4[v] of the
+            // coset of the batch's register span whose representative s0 is
+            // t deposited over the non-pivot coordinates, applies the
+            // batch's gadgets in registers, and puts them back
+            const SpanOp& hb = ops[o];
+            const uint32_t* __restrict__ S4 = reinterpret_cast<const uint32_t*>(&ops[o + 1]);
+            const int nops = (int)hb.c, ndep = (int)hb.r;
+            uint32_t s0 = 0;
+            for (int j = 0; j < ndep; j++) s0 |= ((tid >> j) & 1u) << ((hb.dep >> (4 * j)) & 15u);
+            ct<T> x[RA];
 #pragma unroll
-                for (int i = 0; i < RA; i++) {
-                    const uint32_t s = tid + i * nth;
-                    const ct<T> f = ((__popc(s & op.d) & 1) ^ rp) ? f1 : f0;
-                    tile[s] = cmulT(f, tile[s]);
-                }
-                // a run of phase gadgets touches each amplitude from the same thread
-                if (o + 1 < a.numOps && ops[o + 1].kind == F_PARITY) continue;
-            } else {
-                const ct<T> af = cf<T>(op.f, 0), pf = cf<T>(op.f, 1);
-                const uint32_t lowMask = (1u << op.top) - 1u;
-                for (uint32_t j = tid; j < half; j += nth) {
-                    const uint32_t s = ((j & ~lowMask) << 1) | (j & lowMask);
-                    const uint32_t v = s ^ op.c;
-                    const T sA = ((__popc(s & op.d) & 1) ^ rp) ? (T)-1 : (T)1;
-                    const T sB = ((__popc(v & op.d) & 1) ^ rp) ? (T)-1 : (T)1;
-                    ct<T> x0 = tile[s], x1 = tile[v];
-                    cmat2T(af, ct<T>{pf.re * sB, pf.im * sB}, ct<T>{pf.re * sA, pf.im * sA}, af, x0, x1);
-                    tile[s] = x0;
-                    tile[v] = x1;
+            for (int v = 0; v < RA; v++) x[v] = tile[s0 ^ S4[v]];
+            for (int g = o + 2; g < o + 2 + nops; g++) {
+                const SpanOp& op = ops[g];
+                TOpHead h;
+                h.rm1 = op.r;
+                h.rm2 = op.r;
+                const int par = parity64(rep & op.z) ^ (__popc(s0 & op.d) & 1);
+                if (op.kind == F_PARITY) {
+                    regParity<T, false>(x, h, cf<T>(op.f, 0), cf<T>(op.f, 1), par);
+                } else {
+                    switch (op.c) {
+#define QA_SPAN_CASE(M) case M: regPauli<T, M, false, false>(x, h, op.f, par, true); break;
+                        QA_SPAN_CASE(1) QA_SPAN_CASE(2) QA_SPAN_CASE(3) QA_SPAN_CASE(4) QA_SPAN_CASE(5)
+                        QA_SPAN_CASE(6) QA_SPAN_CASE(7) QA_SPAN_CASE(8) QA_SPAN_CASE(9) QA_SPAN_CASE(10)
+                        QA_SPAN_CASE(11) QA_SPAN_CASE(12) QA_SPAN_CASE(13) QA_SPAN_CASE(14) QA_SPAN_CASE(15)
+#undef QA_SPAN_CASE
+                    }
                 }
             }
+#pragma unroll
+            for (int v = 0; v < RA; v++) tile[s0 ^ S4[v]] = x[v];
             __syncthreads();
+            o += 2 + nops;
         }
         const uint64_t at = rep ^ own;
 #pragma unroll

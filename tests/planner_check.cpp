@@ -498,21 +498,49 @@ static bool emulateSpan(const SpanArgs& a, const std::vector<SpanOp>& prog, uint
                     tile[s] = amps[addr[s]];
                 }
             }
-            for (const SpanOp& o : prog) {
-                const int rp = parity(rep & o.z);
-                const cx f0(o.f[0], o.f[1]), f1(o.f[2], o.f[3]);
-                if (o.kind == F_PARITY) {
-                    for (uint32_t s = 0; s < size; s++) tile[s] *= (parity(s & o.d) ^ rp) ? f1 : f0;
-                    continue;
+            for (size_t o = 0; o < prog.size();) {
+                // a batch: header, table, gadgets (the kernel's register batches)
+                const SpanOp& hb = prog[o];
+                if (hb.kind != SPAN_BATCH || o + 2 + hb.c > prog.size()) {
+                    std::printf("case %d: malformed span program at record %zu\n", c, o);
+                    return false;
                 }
-                const uint32_t lowMask = (1u << o.top) - 1u;
-                for (uint32_t j = 0; j < half; j++) {
-                    const uint32_t s = ((j & ~lowMask) << 1) | (j & lowMask), v = s ^ o.c;
-                    const double sA = (parity(s & o.d) ^ rp) ? -1 : 1, sB = (parity(v & o.d) ^ rp) ? -1 : 1;
-                    const cx xa = tile[s], xb = tile[v];
-                    tile[s] = f0 * xa + f1 * sB * xb;
-                    tile[v] = f0 * xb + f1 * sA * xa;
+                const uint32_t* S4 = reinterpret_cast<const uint32_t*>(&prog[o + 1]);
+                std::vector<uint8_t> cover(size, 0);
+                for (uint32_t t = 0; t < nth; t++) {
+                    uint32_t s0 = 0;
+                    for (uint32_t j = 0; j < hb.r; j++) s0 |= ((t >> j) & 1u) << ((hb.dep >> (4 * j)) & 15u);
+                    cx x[RA];
+                    for (int v = 0; v < RA; v++) {
+                        const uint32_t sc = s0 ^ S4[v];
+                        if (sc >= size || cover[sc]++) {
+                            std::printf("case %d: batch coordinate %u out of range or taken twice\n", c, sc);
+                            return false;
+                        }
+                        x[v] = tile[sc];
+                    }
+                    for (uint32_t g = 0; g < hb.c; g++) {
+                        const SpanOp& op = prog[o + 2 + g];
+                        const int par = parity(rep & op.z) ^ parity(s0 & op.d);
+                        const cx f0(op.f[0], op.f[1]), f1(op.f[2], op.f[3]);
+                        if (op.kind == F_PARITY) {
+                            for (int v = 0; v < RA; v++) x[v] *= (parity((uint64_t)v & op.r) ^ par) ? f1 : f0;
+                            continue;
+                        }
+                        const int top = 31 - __builtin_clz(op.c);
+                        for (int v = 0; v < RA; v++) {
+                            if ((v >> top) & 1) continue;
+                            const int w = v ^ (int)op.c;
+                            const double sA = (parity((uint64_t)v & op.r) ^ par) ? -1 : 1;
+                            const double sB = (parity((uint64_t)w & op.r) ^ par) ? -1 : 1;
+                            const cx xa = x[v], xb = x[w];
+                            x[v] = f0 * xa + f1 * sB * xb;
+                            x[w] = f0 * xb + f1 * sA * xa;
+                        }
+                    }
+                    for (int v = 0; v < RA; v++) tile[s0 ^ S4[v]] = x[v];
                 }
+                o += 2 + hb.c;
             }
             for (uint32_t s = 0; s < size; s++) amps[addr[s]] = tile[s];
             rep = ((rep | ~a.repMask) + step) & a.repMask;
@@ -594,8 +622,11 @@ static int spanMain(int cases, uint64_t seed0) {
                 ops.erase(ops.begin());
                 continue;
             }
-            if (prog.size() != run || a.numOps != (int)run || a.k != k) {
-                std::printf("case %d: plan of %zu ops has %zu ops\n", c, run, prog.size());
+            size_t gadgets = 0;
+            for (const SpanOp& o : prog) gadgets += o.kind == F_PAULI || o.kind == F_PARITY;
+            if (gadgets != run || a.numOps != (int)prog.size() || a.k != k) {
+                std::printf("case %d: plan of %zu ops has %zu gadgets in %zu records\n", c, run, gadgets,
+                            prog.size());
                 return 1;
             }
             const uint64_t grid = 1 + (uint64_t)r.below((int)std::min<uint64_t>(a.numCosets, 40));

@@ -90,6 +90,9 @@ EXACT_CASES = [
     ("sv16_fp32_span11", lambda: gadget_circuit(16, 120, 15), 1, (13, 3, 2048), 11),
     ("dm8", lambda: gadget_circuit(8, 80, 6, dens=True), 2, (13, 3, 2048), 13),
     ("sv20_only_gadgets", lambda: gadget_circuit(20, 200, 7, others=False), 2, (13, 3, 2048), 13),
+    # gadgets wider than the tile: queued all the same, taken by span passes
+    ("sv16_tile8_wide_gadgets", lambda: gadget_circuit(16, 120, 16, max_k=16), 2, (8, 2, 2048), 12),
+    ("sv14_tile6_wide_gadgets_fp32", lambda: gadget_circuit(14, 100, 17, max_k=14), 1, (6, 2, 2048), 11),
 ]
 
 

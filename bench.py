@@ -501,7 +501,9 @@ def main() -> int:
                          "for the complex128 modes, 12,4,2048 for the complex64 precision mode, the shapes "
                          "measured fastest for each)")
     ap.add_argument("--no-unfused", action="store_true", help="skip the one-step unfused comparison")
-    ap.add_argument("--prefetch", type=int, default=1, help="tile passes prefetch the next tile (1) or not (0)")
+    ap.add_argument("--prefetch", type=int, default=2,
+                    help="tile passes prefetch the next tile: 2 loads spread over the program, 3 as 2 with the "
+                         "stores deferred a tile and spread too, 1 at the tile's start, 0 not")
     ap.add_argument("--exact", type=int, default=0,
                     help="exact fusion (1): gates in queue order with single-gate arithmetic; 0: passes reorder "
                          "commuting gates and merge runs on the same one or two qubits into one matrix")

@@ -138,9 +138,14 @@ int qa_flush(void* amps);
    amplitudes).  Results then differ from the unfused kernels by rounding.
    0 (the default) turns blocks off. */
 int qa_setFusionBlocks(int minGates);
-/* tile passes with (1, the default) or without (0) each workgroup's next
-   tile prefetched into registers while it works on the current one */
-int qa_setFusionPrefetch(int enabled);
+/* how tile passes prefetch each workgroup's next tile into registers while
+   it works on the current one: 2 (the default) with the loads spread over
+   the pass's generated program in four groups, 3 as 2 with each tile's
+   stores deferred into the next tile's work (two register sets taking
+   turns), 1 all at the start of the tile, 0 not at all (load, work,
+   store).  Passes that run in the interpreter treat 2 and 3 as 1.  Other
+   values: QA_ERR_ARG. */
+int qa_setFusionPrefetch(int mode);
 /* exact = 1: passes take queued gates in queue order and apply each with
    the arithmetic of its single-gate kernel.  exact = 0 (the default):
    passes may take gates out of order -- two gates are reordered only when

@@ -547,6 +547,16 @@ def set_fusion_exact(exact: bool) -> None:
         raise QuESTError("qa_setFusionExact", f"failed ({rc}): {_gpu_last_error()}")
 
 
+def set_fusion_prefetch(mode: int = 2) -> None:
+    """How tile passes load each workgroup's next tile while working on the
+    current one (qa_setFusionPrefetch): 2 (the default) loads spread over
+    the generated program, 3 the stores deferred a tile and spread too, 1
+    all loads at the tile's start, 0 no prefetch."""
+    rc = gpu().qa_setFusionPrefetch(C.c_int(int(mode)))
+    if rc != 0:
+        raise QuESTError("qa_setFusionPrefetch", f"failed ({rc}): {_gpu_last_error()}")
+
+
 def set_fusion_jit(mode: int, min_seen: int = 2) -> None:
     """Pass kernels generated at run time (qa_setFusionJit): mode 0 off, 1
     background compiles once a program shape has been planned min_seen

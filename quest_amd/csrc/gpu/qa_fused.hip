@@ -1181,8 +1181,8 @@ int launchPass(void* amps, const Queue& qu, const std::vector<GOp>& ops, size_t 
     const size_t lds = a.usesLds ? ldsBytes(qu.prec, k) : 0;
     const unsigned block = 1u << (k - RB);
     const bool nt = std::min(g_cfg.lowQubits, k) >= STREAM_LOW_BIT;
-    const bool pf = g_cfg.prefetch;
-    const void* fn = tileKernel(qu.prec, nt, pf);
+    const int pf = g_cfg.prefetch;
+    const void* fn = tileKernel(qu.prec, nt, pf != 0);
     if (!ldsTileAtZero(fn)) {
         setError("fusion: the tile-pass kernel has static LDS; its LDS tile would not start at address 0");
         return QA_ERR_HIP;
@@ -1832,7 +1832,12 @@ int qa_setFusionPrefetch(int enabled) {
     QA_REQUIRE_INIT();
     QA_FLUSH_ALL();
     std::lock_guard<std::mutex> lk(qa::fuse::g_mu);
-    qa::fuse::g_cfg.prefetch = enabled != 0;
+    if (enabled < 0 || enabled > 3) {
+        qa::setError("qa_setFusionPrefetch: mode %d is not 0 (off), 1 (at the loop head), 2 (spread loads) or 3 "
+                     "(spread loads and stores)", enabled);
+        return QA_ERR_ARG;
+    }
+    qa::fuse::g_cfg.prefetch = enabled;
     return QA_OK;
 }
 

@@ -39,7 +39,9 @@ struct Config {
     int lowQubits = 4;     // always-local low qubits: tile rows are 2^4 contiguous amplitudes
     int maxQueue = 512;    // flush when this many gates are queued
     int minQubits = 1;
-    bool prefetch = true;  // tile passes prefetch each workgroup's next tile into registers
+    int prefetch = 2;      // tile passes prefetch each workgroup's next tile into registers: 1 at
+                           // the loop head, 2 spread over the program (generated kernels), 3 as 2
+                           // with the stores deferred a tile and spread too, 0 not
     bool exact = false;    // passes apply gates in queue order, each with its single-gate arithmetic
     int spanQubits = 12;   // runs of Pauli and phase gadgets may go to span passes over cosets of
                            // 2^spanQubits amplitudes (0 = off)

@@ -27,6 +27,11 @@
 
 extern char** environ;
 
+// the device headers of the generated kernels, embedded at build time
+extern "C" const int qa_jit_num_headers;
+extern "C" const char* const qa_jit_header_names[];
+extern "C" const char* const qa_jit_header_sources[];
+
 namespace qa {
 namespace jit {
 namespace {
@@ -48,7 +53,7 @@ struct Entry {
 struct Job {
     std::string key;
     std::string src;      // the kernel as asked for
-    std::string srcLean;  // the same program without next-tile prefetch (or empty)
+    std::string srcLean;  // the same program with less prefetch (or empty)
     int prec;
 };
 
@@ -259,14 +264,34 @@ const char* typeName(int prec) { return prec == QA_FP64 ? "double" : "float"; }
 
 // ---- compiles -------------------------------------------------------------
 
+uint64_t fnv1a(uint64_t h, const char* p, size_t n) {
+    for (size_t i = 0; i < n; i++) h = (h ^ (unsigned char)p[i]) * 1099511628211ull;
+    return h;
+}
+
+// the FNV-1a hash of the device headers compiled into every generated
+// kernel (their names and text, as embedded in this library)
+uint64_t headersHash() {
+    static const uint64_t h = [] {
+        uint64_t v = 1469598103934665603ull;
+        for (int i = 0; i < qa_jit_num_headers; i++) {
+            v = fnv1a(v, qa_jit_header_names[i], std::strlen(qa_jit_header_names[i]) + 1);
+            v = fnv1a(v, qa_jit_header_sources[i], std::strlen(qa_jit_header_sources[i]) + 1);
+        }
+        return v;
+    }();
+    return h;
+}
+
 // QA_JIT_CACHE_DIR: code objects kept across processes, one file per
-// kernel source (named by a 64-bit FNV-1a hash of the source, which holds
-// the whole shape), written to a temporary name and renamed into place
+// kernel (named by a 64-bit FNV-1a hash of the source, which holds the
+// whole shape, continued from the hash of the device headers it includes,
+// so that a library built from other headers finds none of the old
+// objects), written to a temporary name and renamed into place
 std::string cachePath(const std::string& src) {
     const char* dir = std::getenv("QA_JIT_CACHE_DIR");
     if (!dir || !*dir) return std::string();
-    uint64_t h = 1469598103934665603ull;
-    for (unsigned char c : src) h = (h ^ c) * 1099511628211ull;
+    const uint64_t h = fnv1a(headersHash(), src.data(), src.size());
     char name[64];
     snprintf(name, sizeof name, "/qa_jit_%016llx_%zu.co", (unsigned long long)h, src.size());
     return std::string(dir) + name;
@@ -339,14 +364,16 @@ bool compileSource(const std::string& src, hipModule_t& mod, hipFunction_t& fn, 
 }
 
 // A program too large for the registers of the prefetching kernel (it
-// would spill to scratch) is compiled again without the prefetch, which
-// frees the 64 registers of the next tile.
+// would spill to scratch) is compiled again with less: a kernel with
+// deferred stores (pf 3) as one with spread loads only (pf 2), which keeps
+// fewer addresses live; otherwise without the prefetch, which frees the 64
+// registers of the next tile.
 bool compileOne(const Job& job, hipModule_t& mod, hipFunction_t& fn, std::string& err) {
     if (compileSource(job.src, mod, fn, err)) return true;
     if (job.srcLean.empty()) return false;
     std::string err2;
     if (compileSource(job.srcLean, mod, fn, err2)) return true;
-    err += "; without prefetch: " + err2;
+    err += "; with less prefetch: " + err2;
     return false;
 }
 
@@ -419,26 +446,31 @@ void ensureWorkers() {
 
 // ---- source generation ----------------------------------------------------
 
-std::string shapeKey(int prec, bool nt, bool pf, const TileArgs& a, const TOp* prog, size_t num) {
+std::string shapeKey(int prec, bool nt, int pf, const TileArgs& a, const TOp* prog, size_t num) {
     char buf[128];
-    snprintf(buf, sizeof buf, "p%d n%d f%d k%d d%d%d%d o%d|", prec, nt ? 1 : 0, pf ? 1 : 0, a.k, a.firstDirect,
+    snprintf(buf, sizeof buf, "p%d n%d f%d k%d d%d%d%d o%d|", prec, nt ? 1 : 0, pf, a.k, a.firstDirect,
              a.lastDirect, a.usesLds, a.numOps);
     std::string k = buf;
     for (size_t i = 0; i < num; i++) k += opShape(prog[i]);
     return k;
 }
 
-std::string generate(int prec, bool nt, bool pf, const TileArgs& a, const TOp* prog, size_t num) {
+// ops that exchange the tile through LDS between workgroup barriers
+static bool exchanges(const TOp& t) {
+    return t.h.kind == fuse::F_GROUP || t.h.kind == fuse::F_PAULI_LDS || t.h.kind == fuse::F_DENSE_LDS;
+}
+
+std::string generate(int prec, bool nt, int pf, const TileArgs& a, const TOp* prog, size_t num) {
     std::string s;
     s.reserve(4096 + num * 512);
     s += "#include \"qa_tiledev.hpp\"\n"
          "namespace qa {\nnamespace fuse {\n"
          "struct JitProgram {\n"
-         "    template <typename T, int NT>\n"
+         "    template <typename T, int NT, typename Pf>\n"
          "    __device__ __forceinline__ static void run(T* __restrict__ amps, const TileArgs& args,\n"
          "                                               const TOp* __restrict__ ops, uint64_t base, ct<T>* r,\n"
          "                                               ct<T>* tile, const RegLayout& row, const RegLayout& first,\n"
-         "                                               uint32_t tid) {\n"
+         "                                               uint32_t tid, Pf& pf) {\n"
          // the program and the pass arguments addressed through a value that
          // changes per tile (base >> 62 is 0 for every state): loop-invariant
          // addresses would be hoisted out of the tile loop and, one per op,
@@ -448,8 +480,30 @@ std::string generate(int prec, bool nt, bool pf, const TileArgs& a, const TOp* p
          "        TileState st;\n"
          "        tileBegin<T, NT>(st, a, r, tile, row, first);\n";
     char buf[512];
-    for (size_t i = a.firstDirect ? 1 : 0; i < num; i++) {
+    // pf >= 2: the next tile's loads in PF_CHUNKS groups (pf 3: each group
+    // after the stores of the previous tile's results held in the same
+    // registers).  The ops that
+    // exchange the tile through LDS (with a workgroup barrier, which loads
+    // are not moved across) cut the program into S = exchanges + 1
+    // segments; group c starts segment c * S / PF_CHUNKS, so the loads
+    // leave through the tile's work and none in its last segment when
+    // there are enough of them.
+    const size_t first = a.firstDirect ? 1 : 0;
+    int S = 1;
+    for (size_t i = first; i < num; i++)
+        if (exchanges(prog[i])) S++;
+    int chunk = 0, seg = 0;
+    auto loadsUpTo = [&](int upToSeg) {
+        for (; pf >= 2 && chunk < PF_CHUNKS && chunk * S / PF_CHUNKS <= upToSeg; chunk++) {
+            snprintf(buf, sizeof buf, "        pf.template part<%d, %d>();\n", chunk * (fuse::RA / PF_CHUNKS),
+                     (chunk + 1) * (fuse::RA / PF_CHUNKS));
+            s += buf;
+        }
+    };
+    for (size_t i = first; i < num; i++) {
+        loadsUpTo(seg);
         const TOp& t = prog[i];
+        if (exchanges(t)) seg++;
         const int* w = reinterpret_cast<const int*>(t.b);
         switch (t.h.kind) {
             case fuse::F_GROUP:
@@ -534,7 +588,8 @@ std::string generate(int prec, bool nt, bool pf, const TileArgs& a, const TOp* p
                 s += buf;
         }
     }
-    s += "        tileEnd<T, NT>(st, a, amps, r, tile, row, base);\n"
+    loadsUpTo(S);
+    s += "        pf.template finish<T, NT>(st, a, amps, r, tile, row, base);\n"
          "    }\n"
          "};\n"
          "}  // namespace fuse\n}  // namespace qa\n";
@@ -543,7 +598,7 @@ std::string generate(int prec, bool nt, bool pf, const TileArgs& a, const TOp* p
              "qa_jit_pass(%s* __restrict__ amps, qa::fuse::TileArgs a, const qa::fuse::TOp* __restrict__ ops) {\n"
              "    qa::fuse::tilePassLoop<%s, %d, %d, qa::fuse::JitProgram>(amps, a, ops);\n"
              "}\n",
-             typeName(prec), typeName(prec), nt ? 1 : 0, pf ? 1 : 0);
+             typeName(prec), typeName(prec), nt ? 1 : 0, pf);
     s += buf;
     return s;
 }
@@ -560,7 +615,7 @@ void setSettings(const Settings& s) {
     g_settings = s;
 }
 
-const void* kernelFor(int prec, bool nt, bool pf, const TileArgs& a, const TOp* prog, size_t num, bool force) {
+const void* kernelFor(int prec, bool nt, int pf, const TileArgs& a, const TOp* prog, size_t num, bool force) {
     Settings s = settings();
     if (force) s.mode = 2;
     if (s.mode == 0) return nullptr;
@@ -573,7 +628,7 @@ const void* kernelFor(int prec, bool nt, bool pf, const TileArgs& a, const TOp* 
         e.state = QUEUED;
         lk.unlock();
         std::string src = generate(prec, nt, pf, a, prog, num);
-        std::string lean = pf ? generate(prec, nt, false, a, prog, num) : std::string();
+        std::string lean = pf ? generate(prec, nt, pf == 3 ? 2 : 0, a, prog, num) : std::string();
         lk.lock();
         if (src.empty()) {
             e.state = FAILED;

@@ -40,7 +40,7 @@ void setSettings(const Settings& s);
 // sighting, queues a compile per the settings, and in mode 2 waits for it;
 // force: compile now and wait whatever the settings (a pass the interpreter
 // cannot run).
-const void* kernelFor(int prec, bool nt, bool pf, const fuse::TileArgs& a, const fuse::TOp* prog, size_t num,
+const void* kernelFor(int prec, bool nt, int pf, const fuse::TileArgs& a, const fuse::TOp* prog, size_t num,
                       bool force = false);
 
 // launches a generated kernel (returns a HIP error code)
@@ -67,8 +67,11 @@ const char* lastError();
 void shutdown();
 
 // the source of a pass's kernel and its cache key (exposed for tests)
-std::string shapeKey(int prec, bool nt, bool pf, const fuse::TileArgs& a, const fuse::TOp* prog, size_t num);
-std::string generate(int prec, bool nt, bool pf, const fuse::TileArgs& a, const fuse::TOp* prog, size_t num);
+std::string shapeKey(int prec, bool nt, int pf, const fuse::TileArgs& a, const fuse::TOp* prog, size_t num);
+// pf: 0 no prefetch, 1 the next tile loaded at the loop head, 2 its loads
+// spread over the program in PF_CHUNKS groups
+constexpr int PF_CHUNKS = 4;
+std::string generate(int prec, bool nt, int pf, const fuse::TileArgs& a, const fuse::TOp* prog, size_t num);
 
 }  // namespace jit
 }  // namespace qa

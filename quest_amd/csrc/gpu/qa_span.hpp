@@ -80,6 +80,14 @@ __device__ __forceinline__ void spanLoad(ct<T>* r, const T* __restrict__ amps, u
     for (int i = 0; i < RA; i++) r[i] = ldx<NT>(amps, at ^ a.top[i]);
 }
 
+// the thread's address part, made opaque at each use so that the sixteen
+// register addresses are formed there, not hoisted out of the coset loop
+// into 32 more live registers
+__device__ __forceinline__ uint64_t opaqueOwn(uint64_t own) {
+    asm volatile("" : "+v"(own));
+    return own;
+}
+
 template <typename T, int NT>
 __device__ __forceinline__ void spanPassLoop(T* __restrict__ amps, const SpanArgs& a,
                                              const SpanOp* __restrict__ ops) {
@@ -95,11 +103,12 @@ __device__ __forceinline__ void spanPassLoop(T* __restrict__ amps, const SpanArg
     uint64_t rep = deposit(w, a.repPos, a.numRep);
     ct<T> rn[RA];
     spanLoad<T, NT>(rn, amps, rep ^ own, a);
+    waitLoadsBeforeStores();
     for (; w < a.numCosets; w += gridDim.x) {
 #pragma unroll
         for (int i = 0; i < RA; i++) tile[tid + i * nth] = rn[i];
         const uint64_t repN = ((rep | ~a.repMask) + a.repStep) & a.repMask;
-        if (w + gridDim.x < a.numCosets) spanLoad<T, NT>(rn, amps, repN ^ own, a);
+        if (w + gridDim.x < a.numCosets) spanLoad<T, NT>(rn, amps, repN ^ opaqueOwn(own), a);
         __syncthreads();
         for (int o = 0; o < a.numOps;) {
             // a batch: thread t takes the 16 coordinates s0 ^ S4[v] of the
@@ -137,7 +146,10 @@ __device__ __forceinline__ void spanPassLoop(T* __restrict__ amps, const SpanArg
             __syncthreads();
             o += 2 + nops;
         }
-        const uint64_t at = rep ^ own;
+        const uint64_t at = rep ^ opaqueOwn(own);
+        // the next coset's loads retire before the stores, which then drain
+        // while the next coset is worked on
+        waitLoadsBeforeStores();
 #pragma unroll
         for (int i = 0; i < RA; i++) stx<NT>(amps, at ^ a.top[i], tile[tid + i * nth]);
         rep = repN;

@@ -1029,11 +1029,23 @@ __device__ __forceinline__ void stepFixed(const TileState& st, const TOp* __rest
     }
 }
 
+// Waits for every vector memory load in flight -- in a prefetching pass
+// loop, the next tile's -- before this tile's stores are issued.  Loads and
+// stores share one counter that retires in issue order, so a wait for the
+// prefetched registers placed after the stores (where the compiler would put
+// it, at the next iteration's first use) waits for the stores as well, and
+// the tile's store traffic would never overlap the next tile's work.  With
+// the loads retired here, the stores drain while the next tile computes.
+__device__ __forceinline__ void waitLoadsBeforeStores() {
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0); expcnt, lgkmcnt unchanged
+}
+
 // the tile back to HBM: directly in the last group's layout, or through
 // LDS in the row layout
 template <typename T, int NT>
 __device__ __forceinline__ void tileEnd(const TileState& st, const TileArgs& a, T* __restrict__ amps, ct<T>* r,
                                         ct<T>* tile, const RegLayout& row, uint64_t base) {
+    waitLoadsBeforeStores();
     if (a.lastDirect) {
         RegIO<T, NT, 0>::hbmStore(r, amps, base | st.cur.tg, st.cur, a.localPos);
     } else {
@@ -1042,6 +1054,18 @@ __device__ __forceinline__ void tileEnd(const TileState& st, const TileArgs& a, 
         RegIO<T, NT, 0>::ldsLoad(r, tile, opaqueV(ldsByte<T>(row.jb)), a.rowOff);
         RegIO<T, NT, 0>::hbmStore(r, amps, base | row.tg, row, a.localPos);
     }
+}
+
+// the tile's registers put in the layout they are stored from (the last
+// group's, or the row layout through LDS), which is returned; nothing stored
+template <typename T, int NT>
+__device__ __forceinline__ RegLayout tileFinish(const TileState& st, const TileArgs& a, ct<T>* r, ct<T>* tile,
+                                                const RegLayout& row) {
+    if (a.lastDirect) return st.cur;
+    if (st.inRegs) RegIO<T, NT, 0>::ldsStore(r, tile, opaqueV(st.sj), a.lastOff);
+    __syncthreads();
+    RegIO<T, NT, 0>::ldsLoad(r, tile, opaqueV(ldsByte<T>(row.jb)), a.rowOff);
+    return row;
 }
 
 // the interpreter: one tile of the pass whose program is ops[0, a.numOps)
@@ -1075,13 +1099,107 @@ __device__ __forceinline__ void tileBody(T* __restrict__ amps, const TileArgs& a
     tileEnd<T, NT>(st, a, amps, r, tile, row, base);
 }
 
+// The next tile's loads as a program issues them: part<B, E>() loads
+// registers B..E-1.  A generated program calls it between its ops, so the
+// loads leave in small groups while the tile is worked on; issued at once
+// (sixteen per wave, every wave together), they back up the memory pipeline
+// and in-order issue stalls each wave at the group's last load until the
+// group has been accepted.
+template <typename T, int NT, int V, int E>
+struct PrefetchRange {
+    __device__ __forceinline__ static void load(ct<T>* rn, const T* amps, uint64_t at, const RegLayout& L,
+                                                const int* pos) {
+        rn[V] = ldx<NT>(amps, at | offQ<V>(L, pos));
+        PrefetchRange<T, NT, V + 1, E>::load(rn, amps, at, L, pos);
+    }
+};
+template <typename T, int NT, int E>
+struct PrefetchRange<T, NT, E, E> {
+    __device__ __forceinline__ static void load(ct<T>*, const T*, uint64_t, const RegLayout&, const int*) {}
+};
+template <typename T, int NT, int V, int E>
+struct StoreRange {
+    __device__ __forceinline__ static void store(const ct<T>* x, T* amps, uint64_t at, const RegLayout& L,
+                                                 const int* pos) {
+        stx<NT>(amps, at | offQ<V>(L, pos), x[V]);
+        StoreRange<T, NT, V + 1, E>::store(x, amps, at, L, pos);
+    }
+};
+template <typename T, int NT, int E>
+struct StoreRange<T, NT, E, E> {
+    __device__ __forceinline__ static void store(const ct<T>*, T*, uint64_t, const RegLayout&, const int*) {}
+};
+
+// a generated program ends with pf.finish: the tile stored (tileEnd)
+template <typename T, int NT>
+struct TilePrefetch {
+    ct<T>* rn;
+    const T* amps;
+    uint64_t at;
+    const RegLayout& L;
+    const int* pos;
+    template <int B, int E>
+    __device__ __forceinline__ void part() {
+        PrefetchRange<T, NT, B, E>::load(rn, amps, at, L, pos);
+    }
+    template <typename U, int NU>
+    __device__ __forceinline__ void finish(const TileState& st, const TileArgs& a, U* __restrict__ amps_, ct<U>* r,
+                                           ct<U>* tile, const RegLayout& row, uint64_t base) {
+        tileEnd<U, NU>(st, a, amps_, r, tile, row, base);
+    }
+};
+// the loop's stand-in when it loads the next tile itself (or not at all)
+struct NoPrefetch {
+    template <int B, int E>
+    __device__ __forceinline__ void part() {}
+    template <typename U, int NU>
+    __device__ __forceinline__ void finish(const TileState& st, const TileArgs& a, U* __restrict__ amps, ct<U>* r,
+                                           ct<U>* tile, const RegLayout& row, uint64_t base) {
+        tileEnd<U, NU>(st, a, amps, r, tile, row, base);
+    }
+};
+
+// PF = 3: two register sets take turns.  While a tile is worked on in one,
+// the other holds the previous tile's results; each part<B, E>() stores
+// their registers B..E-1 and loads the next tile's into the same registers,
+// so stores and loads both leave in small groups through the work.  finish
+// leaves the tile's results in its registers (put in their store layout)
+// and records where they go.
+template <typename T, int NT>
+struct TileStream {
+    ct<T>* x;           // the other register set
+    T* amps;
+    bool pending;       // x holds results to store first (not before a workgroup's first tile)
+    uint64_t storeAt;
+    RegLayout storeL;
+    uint64_t loadAt;
+    const RegLayout& loadL;
+    const int* pos;
+    uint64_t doneAt;    // set by finish: where this tile's results go
+    RegLayout doneL;
+    template <int B, int E>
+    __device__ __forceinline__ void part() {
+        if (pending) StoreRange<T, NT, B, E>::store(x, amps, storeAt, storeL, pos);
+        PrefetchRange<T, NT, B, E>::load(x, amps, loadAt, loadL, pos);
+    }
+    template <typename U, int NU>
+    __device__ __forceinline__ void finish(const TileState& st, const TileArgs& a, U* __restrict__, ct<U>* r,
+                                           ct<U>* tile, const RegLayout& row, uint64_t base) {
+        doneL = tileFinish<U, NU>(st, a, r, tile, row);
+        doneAt = base | doneL.tg;
+    }
+};
+
 // The persistent pass loop around one tile's program (Body::run), shared by
 // the interpreter kernels (qa_tilepass.hpp) and the generated ones.  NT:
 // non-temporal HBM loads/stores, used when the contiguous low run of a tile
 // row is at least 2^STREAM_LOW_BIT amplitudes.  PF = 1: each workgroup
 // prefetches its next tile into registers before working on the current
 // one, so HBM traffic overlaps the register and LDS work (16 more registers
-// of amplitudes per thread); PF = 0: load, work, store.
+// of amplitudes per thread); PF = 2: the same, with the loads issued by the
+// program between its ops (TilePrefetch; generated programs only); PF = 3:
+// the stores too, a tile later (TileStream; generated programs only);
+// PF = 0: load, work, store.
 template <typename T, int NT, int PF, typename Body>
 __device__ __forceinline__ void tilePassLoop(T* __restrict__ amps, const TileArgs& a, const TOp* __restrict__ ops) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -1095,16 +1213,65 @@ __device__ __forceinline__ void tilePassLoop(T* __restrict__ amps, const TileArg
     const uint64_t nlMask = a.nonLocalMask, step = a.tileStep;
     uint64_t t = blockIdx.x;
     uint64_t base = deposit(t, a.nonLocalPos, a.numNonLocal);
-    if constexpr (PF) {
+    if constexpr (PF == 3) {
+        ct<T> x[RA], y[RA];
+        if (t < a.numTiles) RegIO<T, NT, 0>::hbmLoad(x, amps, base | loadL.tg, loadL, a.localPos);
+        bool pendY = false;
+        uint64_t atY = 0;
+        RegLayout LY = row;
+        // two tiles per trip, one in each register set; a workgroup's last
+        // tile loads its own rows again into the other set (dropped)
+        while (t < a.numTiles) {
+            uint64_t baseN = ((base | ~nlMask) + step) & nlMask;
+            TileStream<T, NT> sx{y,     amps, pendY, atY, LY, (t + gridDim.x < a.numTiles ? baseN : base) | loadL.tg,
+                                 loadL, a.localPos, 0, row};
+            Body::template run<T, NT>(amps, a, ops, base, x, tile, row, first, tid, sx);
+            base = baseN;
+            t += gridDim.x;
+            if (a.usesLds) __syncthreads();
+            if (t >= a.numTiles) {
+                RegIO<T, NT, 0>::hbmStore(x, amps, sx.doneAt, sx.doneL, a.localPos);
+                break;
+            }
+            baseN = ((base | ~nlMask) + step) & nlMask;
+            TileStream<T, NT> sy{x,     amps, true, sx.doneAt, sx.doneL,
+                                 (t + gridDim.x < a.numTiles ? baseN : base) | loadL.tg, loadL, a.localPos, 0, row};
+            Body::template run<T, NT>(amps, a, ops, base, y, tile, row, first, tid, sy);
+            base = baseN;
+            t += gridDim.x;
+            if (a.usesLds) __syncthreads();
+            if (t >= a.numTiles) {
+                RegIO<T, NT, 0>::hbmStore(y, amps, sy.doneAt, sy.doneL, a.localPos);
+                break;
+            }
+            pendY = true;
+            atY = sy.doneAt;
+            LY = sy.doneL;
+        }
+    } else if constexpr (PF) {
         ct<T> rn[RA];
         if (t < a.numTiles) RegIO<T, NT, 0>::hbmLoad(rn, amps, base | loadL.tg, loadL, a.localPos);
+        // the first tile's loads retired before the loop, so that the loop
+        // head needs no wait (which would also wait for the last tile's stores)
+        waitLoadsBeforeStores();
         for (; t < a.numTiles; t += gridDim.x) {
             ct<T> r[RA];
 #pragma unroll
             for (int v = 0; v < RA; v++) r[v] = rn[v];
             const uint64_t baseN = ((base | ~nlMask) + step) & nlMask;
-            if (t + gridDim.x < a.numTiles) RegIO<T, NT, 0>::hbmLoad(rn, amps, baseN | loadL.tg, loadL, a.localPos);
-            Body::template run<T, NT>(amps, a, ops, base, r, tile, row, first, tid);
+            if constexpr (PF == 2) {
+                // the program's loads are unconditional: a workgroup's last
+                // tile loads its own rows again (read before its stores,
+                // then dropped)
+                const uint64_t nextAt = (t + gridDim.x < a.numTiles ? baseN : base) | loadL.tg;
+                TilePrefetch<T, NT> pf{rn, amps, nextAt, loadL, a.localPos};
+                Body::template run<T, NT>(amps, a, ops, base, r, tile, row, first, tid, pf);
+            } else {
+                if (t + gridDim.x < a.numTiles)
+                    RegIO<T, NT, 0>::hbmLoad(rn, amps, baseN | loadL.tg, loadL, a.localPos);
+                NoPrefetch pf;
+                Body::template run<T, NT>(amps, a, ops, base, r, tile, row, first, tid, pf);
+            }
             base = baseN;
             // the next tile's first LDS writes must not overtake this tile's reads
             if (a.usesLds) __syncthreads();
@@ -1113,7 +1280,8 @@ __device__ __forceinline__ void tilePassLoop(T* __restrict__ amps, const TileArg
         for (; t < a.numTiles; t += gridDim.x) {
             ct<T> r[RA];
             RegIO<T, NT, 0>::hbmLoad(r, amps, base | loadL.tg, loadL, a.localPos);
-            Body::template run<T, NT>(amps, a, ops, base, r, tile, row, first, tid);
+            NoPrefetch pf;
+            Body::template run<T, NT>(amps, a, ops, base, r, tile, row, first, tid, pf);
             base = ((base | ~nlMask) + step) & nlMask;
             if (a.usesLds) __syncthreads();
         }
@@ -1121,10 +1289,10 @@ __device__ __forceinline__ void tilePassLoop(T* __restrict__ amps, const TileArg
 }
 
 struct Interpreter {
-    template <typename T, int NT>
+    template <typename T, int NT, typename Pf>
     __device__ __forceinline__ static void run(T* __restrict__ amps, const TileArgs& a, const TOp* __restrict__ ops,
                                                uint64_t base, ct<T>* r, ct<T>* tile, const RegLayout& row,
-                                               const RegLayout& first, uint32_t tid) {
+                                               const RegLayout& first, uint32_t tid, Pf&) {
         tileBody<T, NT>(amps, a, ops, base, r, tile, row, first, tid);
     }
 };

@@ -39,6 +39,9 @@ using namespace qa::fuse;
 namespace {
 
 constexpr int N = 30, TILE = 13, LOW = 3, MAXQ_QUEUE = 2048;
+#ifndef PF_MODE
+#define PF_MODE 2
+#endif
 
 std::mt19937_64 rng(2024);
 double unit() { return std::uniform_real_distribution<double>(0, 1)(rng); }
@@ -205,9 +208,9 @@ int main(int argc, char** argv) {
             setLdsOffsets(prog.data(), prog.size(), a, QA_FP64);
             ops.swap(rest);
             passes++;
-            const std::string key = qa::jit::shapeKey(QA_FP64, true, true, a, prog.data(), prog.size());
+            const std::string key = qa::jit::shapeKey(QA_FP64, true, PF_MODE, a, prog.data(), prog.size());
             if (!seen.insert(key).second) continue;
-            const std::string src = qa::jit::generate(QA_FP64, true, true, a, prog.data(), prog.size());
+            const std::string src = qa::jit::generate(QA_FP64, true, PF_MODE, a, prog.data(), prog.size());
             if (dumpDir) {
                 char path[512];
                 std::snprintf(path, sizeof path, "%s/bench_pass_%03d.hip", dumpDir, passes);
@@ -221,7 +224,7 @@ int main(int argc, char** argv) {
             // use scratch the same program without prefetch
             const long scratch = compileScratch(src);
             long lean = -2;
-            if (scratch != 0) lean = compileScratch(qa::jit::generate(QA_FP64, true, false, a, prog.data(), prog.size()));
+            if (scratch != 0) lean = compileScratch(qa::jit::generate(QA_FP64, true, PF_MODE == 3 ? 2 : 0, a, prog.data(), prog.size()));
             compiled++;
             std::printf("pass %d: %zu ops, scratch %ld bytes%s\n", passes, prog.size(), scratch,
                         lean == -2 ? "" : lean == 0 ? "; without prefetch none" : "; without prefetch too");

@@ -47,7 +47,7 @@ int main(int argc, char** argv) {
         std::vector<TOp> prog;
         planPass(pass.data(), pass.size(), N, L, LOW, a, prog, true);
         setLdsOffsets(prog.data(), prog.size(), a, QA_FP64);
-        qa::jit::kernelFor(QA_FP64, true, true, a, prog.data(), prog.size());
+        qa::jit::kernelFor(QA_FP64, true, 2, a, prog.data(), prog.size());
     }
     std::this_thread::sleep_for(std::chrono::milliseconds(delayMs));
     std::printf("returning with compiles queued\n");

@@ -323,8 +323,8 @@ struct JitCheck {
             else planPass(ops, num, n, L, low, a, prog, !exact);
             setLdsOffsets(prog.data(), prog.size(), a, prec);
             const bool nt = low >= 3;
-            const std::string key = qa::jit::shapeKey(prec, nt, true, a, prog.data(), prog.size());
-            const std::string src = qa::jit::generate(prec, nt, true, a, prog.data(), prog.size());
+            const std::string key = qa::jit::shapeKey(prec, nt, 2, a, prog.data(), prog.size());
+            const std::string src = qa::jit::generate(prec, nt, 2, a, prog.data(), prog.size());
             auto it = byKey.find(key);
             if (it != byKey.end()) {
                 if (it->second != src) {

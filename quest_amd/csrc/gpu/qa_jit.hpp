@@ -69,7 +69,8 @@ void shutdown();
 // the source of a pass's kernel and its cache key (exposed for tests)
 std::string shapeKey(int prec, bool nt, int pf, const fuse::TileArgs& a, const fuse::TOp* prog, size_t num);
 // pf: 0 no prefetch, 1 the next tile loaded at the loop head, 2 its loads
-// spread over the program in PF_CHUNKS groups
+// spread over the program in PF_CHUNKS groups, 3 as 2 with each group
+// after the stores of the previous tile's results (two register sets)
 constexpr int PF_CHUNKS = 4;
 std::string generate(int prec, bool nt, int pf, const fuse::TileArgs& a, const fuse::TOp* prog, size_t num);
 

@@ -39,6 +39,7 @@ using namespace qa::fuse;
 namespace {
 
 constexpr int N = 30, TILE = 13, LOW = 3, MAXQ_QUEUE = 2048;
+// the library's default prefetch mode (Config::prefetch)
 #ifndef PF_MODE
 #define PF_MODE 2
 #endif
@@ -220,14 +221,15 @@ int main(int argc, char** argv) {
                 }
             }
             if (compiled >= maxCompiles) continue;
-            // as the library does: the prefetching kernel, and when it would
-            // use scratch the same program without prefetch
+            // as the library does: the kernel of the default prefetch mode
+            // (PF_MODE), and when it would use scratch the same program with
+            // less prefetch
             const long scratch = compileScratch(src);
             long lean = -2;
             if (scratch != 0) lean = compileScratch(qa::jit::generate(QA_FP64, true, PF_MODE == 3 ? 2 : 0, a, prog.data(), prog.size()));
             compiled++;
             std::printf("pass %d: %zu ops, scratch %ld bytes%s\n", passes, prog.size(), scratch,
-                        lean == -2 ? "" : lean == 0 ? "; without prefetch none" : "; without prefetch too");
+                        lean == -2 ? "" : lean == 0 ? "; with less prefetch none" : "; with less prefetch too");
             if (scratch != 0 && lean != 0) failures++;
         }
     }
